@@ -1,0 +1,228 @@
+"""Benchmark: steady Newton solve of the 1025x1025 cylinder O-grid on one GPU.
+
+Workload (the headline configuration): O-grid 1025x1025 around a cylinder of
+radius 0.5 to radius 50, geometric radial stretching, uniform theta, seam
+stored twice; Re = 40, steady, alpha = 0, Dirichlet far field, pure Newton
+(dtau = inf), initial state of vorti2d/solver.py:103-114.  A "step" is one
+Newton solve from that state until ||R||_2 < 1e-10 (R = the FP64 residual-form
+RHS of the assembly), each Newton iteration = device assembly + block-LU solve
++ update.  Mesh upload, metrics and pattern build are one-time setup outside the
+timed region.  The mixed-precision solve is the reported value; the plain FP64
+solve is timed beside it.
+
+Prints one JSON line (see the keys below).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+RTOL_R = 1e-10
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def assembly_bytes(imax, jmax, nnz, value_bytes, n_state=2):
+    """Algorithmic HBM bytes of one assembly: 8 metric planes + n_state state
+    planes + 2 FP64 RHS planes per node, the Jacobian values, x/y on the two
+    boundary rows."""
+    N = imax * jmax
+    return N * (8 * 8 + 8 * n_state + 2 * 8) + value_bytes * nnz + 16 * 2 * imax
+
+
+def spmv_bytes(imax, jmax, nnz, value_bytes, vec_bytes):
+    """Stencil SpMV with implied columns: values + x + y."""
+    return value_bytes * nnz + 2 * (2 * imax * jmax) * vec_bytes
+
+
+def factor_flops(levels):
+    """Dense Gauss-Jordan inversions, 2 n^3 per level."""
+    return sum(2.0 * n ** 3 for n in levels)
+
+
+def newton_solve(ctx, re, max_iter=30):
+    """One timed Newton solve; returns (seconds, iterations, ||R|| history, res history)."""
+    import vorti2d_amd  # noqa: F401
+    ctx.init_state()
+    ctx.assemble(re)
+    rn = [ctx.rhs_norm()]
+    res_hist = []
+    t0 = time.perf_counter()
+    it = 0
+    while it < max_iter:
+        res, _, _ = ctx.newton_step(re)
+        it += 1
+        res_hist.append(res)
+        ctx.assemble(re)
+        rn.append(ctx.rhs_norm())
+        if rn[-1] < RTOL_R:
+            break
+        if it >= 3 and rn[-1] > 0.5 * rn[-2] and res < 1e-12:
+            break   # round-off floor reached
+    dt = time.perf_counter() - t0
+    return dt, it, rn, res_hist
+
+
+def cpu_baseline(imax, jmax, xg, yg, n_newton):
+    """One Newton iteration of the FP64 path on the host (reference Fortran
+    assemble_coo when oracle/_ref is present, else the oracle's C restatement;
+    the exact LU by scipy SuperLU), scaled by the Newton iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    core = oracle.load_ref()
+    kind = "reference"
+    if core is None:
+        core = oracle.load_c()
+        kind = "port"
+    dk, de = 1.0 / (imax - 1), 1.0 / (jmax - 1)
+    m = core.compute_metrics(dk, de, xg, yg)
+    N = imax * jmax
+    psi, ome = oracle.initial_state(imax, jmax, m[8], m[9])
+    z = np.zeros(N)
+    t0 = time.perf_counter()
+    ci, cj, cv, nnz, b = core.assemble_coo(imax, jmax, 40.0, 0.0, 0.0, 0.0, dk, de, *m, psi, ome,
+                                           z, z, 0, 2 * N, 13 * 2 * N, 0, 1.0, 0.0)
+    t1 = time.perf_counter()
+    A = sp.csc_matrix((cv[:nnz], (ci[:nnz], cj[:nnz])), shape=(2 * N, 2 * N))
+    t2 = time.perf_counter()
+    spla.spsolve(A, b)
+    t3 = time.perf_counter()
+    per_iter = (t1 - t0) + (t3 - t2) + (t2 - t1)
+    return dict(value=per_iter * n_newton, unit="s", cores=1, kind=kind,
+                sample=(f"one FP64 Newton iteration at {imax}x{jmax} measured on the host "
+                        f"(assemble_coo {t1 - t0:.2f} s [{'reference Fortran' if kind == 'reference' else 'C restatement'}], "
+                        f"COO->CSC {t2 - t1:.2f} s, SuperLU factor+solve {t3 - t2:.2f} s, "
+                        f"1 thread) x {n_newton} Newton iterations"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=1025)
+    ap.add_argument("--re", type=float, default=40.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                          "(1-D circumferential slabs) but not implemented", "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import vorti2d_amd as v
+
+    imax = jmax = a.size
+    xg, yg = v.generate_annulus(imax, jmax, 0.5, 50.0)
+    out = {}
+    for prec in (("mixed",) if a.no_fp64 else ("mixed", "fp64")):
+        t0 = time.perf_counter()
+        ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, prec)
+        if prec == "mixed":
+            ctx.set_solver(refactor_every=1, gmres_rtol=1e-6, gmres_max_iter=20)
+        setup = time.perf_counter() - t0
+        log(f"[{prec}] setup {setup:.2f} s, device memory {ctx.device_bytes() / 2**30:.1f} GiB")
+        nsteps = a.steps if prec == "mixed" else max(1, a.steps // 2)
+        for w in range(a.warmup):
+            dt, it, rn, rh = newton_solve(ctx, a.re)
+            log(f"[{prec}] warmup solve {dt:.3f} s, {it} iterations, ||R|| {rn[-1]:.3e}")
+        ctx.timers(reset=True)
+        times = []
+        for s in range(nsteps):
+            dt, it, rn, rh = newton_solve(ctx, a.re)
+            times.append(dt)
+            log(f"[{prec}] solve {s}: {dt:.3f} s, {it} Newton iterations, ||R|| "
+                + " ".join(f"{x:.2e}" for x in rn))
+        tm = ctx.timers()
+        fc = ctx.forces(a.re)
+        entry = dict(seconds=statistics.median(times), times=times, newton_iters=it,
+                     rnorm_final=rn[-1], rnorm_hist=rn, res_final=rh[-1], cd=fc[0], cl=fc[1],
+                     timers=tm, setup_s=setup)
+        if prec == "mixed":
+            asm_ms = ctx.time_assembly(a.re, launches=50)
+            spmv_ms = ctx.time_spmv(launches=50)
+            entry.update(assembly_ms=asm_ms, spmv_ms=spmv_ms, nnz=ctx.nnz())
+        out[prec] = entry
+        ctx.close()
+        del ctx
+
+    mx = out["mixed"]
+    nnz = mx["nnz"]
+    levels = [4 * imax] + [2 * imax] * (jmax - 4) + [4 * imax]
+    tmx = mx["timers"]
+    n_fac = max(tmx["factorizations"], 1)
+    fac_ms = tmx["factor_ms"] / n_fac
+    fac_tf = factor_flops(levels) / (fac_ms * 1e-3) / 1e12
+    asm_b = assembly_bytes(imax, jmax, nnz, 4)
+    asm_gbs = asm_b / (mx["assembly_ms"] * 1e-3) / 1e9
+    spmv_b = spmv_bytes(imax, jmax, nnz, 4, 4)
+    spmv_gbs = spmv_b / (mx["spmv_ms"] * 1e-3) / 1e9
+    step_ms = tmx["step_ms"] / max(tmx["steps"], 1)
+    res = {
+        "metric": "Newton solve s to ||R||<1e-10 (mixed vs FP64, 1 GPU); assembly HBM GB/s",
+        "value": mx["seconds"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": mx["seconds"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 Jacobian/LU/GMRES, fp64 residual/state",
+        "data": "synthetic (closed-form O-grid, reference initial state)",
+        "config": {"workload": f"steady cylinder {imax}x{jmax} O-grid, Re {a.re:g}, "
+                               "geometric radial stretching, Newton to ||R||<1e-10",
+                   "imax": imax, "jmax": jmax, "dof": 2 * imax * jmax, "nnz": nnz,
+                   "stretch_ratio": v.mesh.stretch_ratio(jmax)},
+        "newton_iters": mx["newton_iters"],
+        "rnorm_final": mx["rnorm_final"],
+        "cd": mx["cd"],
+        "assembly_gbs": asm_gbs,
+        "assembly": {"bound": "hbm", "achieved": asm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": asm_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": asm_b, "ms_per_launch": mx["assembly_ms"]},
+        "spmv": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": spmv_gbs / HBM_PEAK_GBS, "ms_per_launch": mx["spmv_ms"]},
+        "roofline": {"bound": "mfma", "achieved": fac_tf, "peak": PEAK_TFLOPS["fp32"],
+                     "unit": "TFLOP/s", "frac": fac_tf / PEAK_TFLOPS["fp32"], "traffic": None,
+                     "kernel": "block-LU factorisation (dense Gauss-Jordan per radial level)",
+                     "flops_per_launch": factor_flops(levels), "ms_per_launch": fac_ms},
+        "phase_ms_per_newton_iter": {
+            "assemble": tmx["assemble_ms"] / max(tmx["steps"], 1),
+            "factor": tmx["factor_ms"] / max(tmx["steps"], 1),
+            "solve": tmx["solve_ms"] / max(tmx["steps"], 1), "step": step_ms},
+    }
+    if "fp64" in out:
+        f = out["fp64"]
+        res["fp64_seconds"] = f["seconds"]
+        res["fp64_newton_iters"] = f["newton_iters"]
+        res["fp64_rnorm_final"] = f["rnorm_final"]
+        res["mixed_speedup_vs_fp64"] = f["seconds"] / mx["seconds"]
+        res["cd_fp64"] = f["cd"]
+        res["cd_mixed_minus_fp64"] = mx["cd"] - f["cd"]
+    if not a.no_cpu:
+        cb = cpu_baseline(imax, jmax, xg, yg,
+                          out.get("fp64", mx)["newton_iters"])
+        res["cpu_baseline"] = cb
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * vorti2d.h -- C ABI of libvorti2d, the MI355X (gfx950) metric / assembly /
+ * Krylov hot path of the vorticity-streamfunction Newton solver.
+ *
+ * Every function returns 0 on success and -1 on failure; v2d_last_error()
+ * then holds a message.  Pointers are plain host pointers unless the name ends
+ * in _dev.  Arrays follow the reference conventions: mesh arrays (imax, jmax)
+ * column-major (i fastest), field arrays of length N = imax*jmax in node order
+ * k = imax*j + i, Newton system field-blocked (psi rows/cols [0,N), omega
+ * [N,2N)), COO indices 0-based int32, values float64.
+ */
+#ifndef VORTI2D_H
+#define VORTI2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failure on this thread ("" if none). */
+const char *v2d_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int v2d_version(void);
+
+/* ------------------------------------------------------------------------
+ * f2py-equivalent core entry points (host pointers).  Together they replace
+ * the compiled module vorti2d._core.vorti2d_core (vorti2d/__init__.py:13-15).
+ * ---------------------------------------------------------------------- */
+
+/* Replaces node_neighbors (src/fortran/vorti2d_core.f90:33-61).  1-based i, j;
+ * out = k, e, w, n, s, ne, nw, se, sw (1-based pointers, seam wrapped). */
+int v2d_node_neighbors(int i, int j, int imax, int out[9]);
+
+/* Replaces compute_metrics (vorti2d_core.f90:66-189; called at
+ * vorti2d/solver.py:72-74).  xg, yg: (imax, jmax) column-major.  out: 10*N
+ * doubles, planes jac, alfa, beta, gama, pmet, qmet, detadx, detady, xphys,
+ * yphys.  Requires imax >= 4, jmax >= 4. */
+int v2d_compute_metrics(int imax, int jmax, double dksi, double deta, const double *xg,
+                        const double *yg, double *out);
+
+/* Device-pointer, stream-ordered form of v2d_compute_metrics.  stream is a
+ * hipStream_t (NULL = default stream). */
+int v2d_compute_metrics_dev(int imax, int jmax, double dksi, double deta, const double *d_xg,
+                            const double *d_yg, double *d_out, void *stream);
+
+/* Replaces assemble_coo (vorti2d_core.f90:219-383; called at
+ * vorti2d/solver.py:121-127).  metrics: 10 pointers in compute_metrics order.
+ * Rows [r0, r1) of the field-blocked system in the reference emission order.
+ * Unlike the reference, the capacity maxnnz is checked (error, nothing
+ * written) and *nnz is the number of entries written. */
+int v2d_assemble_coo(int imax, int jmax, double re, double invdtau, double inv2dt, double urot,
+                     double dksi, double deta, const double *const *metrics, const double *psi,
+                     const double *ome, const double *omeold, const double *omeoldold, int r0,
+                     int r1, int maxnnz, int ff_bc, double ca, double sa, int *coo_i, int *coo_j,
+                     double *coo_v, int *nnz, double *bvec);
+
+/* Replaces compute_metrics_local (vorti2d_core.f90:392-495; called at
+ * vorti2d/dist_solver.py:92-93).  il0, il1 are 1-based owned columns. */
+int v2d_compute_metrics_local(int gxm, int jmax, int il0, int il1, double dksi, double deta,
+                              const double *xgl, const double *ygl, double *out);
+
+/* Replaces assemble_coo_local (vorti2d_core.f90:510-661; called at
+ * vorti2d/dist_solver.py:310-317).  lg: 2*gxm*jmax int32 local-to-global map. */
+int v2d_assemble_coo_local(int gxm, int jmax, int il0, int il1, double re, double invdtau,
+                           double inv2dt, double urot, double dksi, double deta,
+                           const double *const *metrics, const double *psi, const double *ome,
+                           const double *omeold, const double *omeoldold, int r0, int nloc,
+                           const int *lg, int maxnnz, int ff_bc, double ca, double sa,
+                           int *coo_i, int *coo_j, double *coo_v, int *nnz, double *bvec);
+
+/* Replaces compute_force_coeffs (vorti2d/forces.py:82-194).  Arrays of length
+ * N.  ref_length <= 0 selects the wall-estimated diameter.  out: cd, cl, cm,
+ * cdp, cdf, clp, clf, cmp, cmf, ref_length. */
+int v2d_force_coeffs(int imax, int jmax, double re, double dksi, double deta,
+                     const double *beta, const double *gama, const double *detadx,
+                     const double *detady, const double *xphys, const double *yphys,
+                     const double *ome, double ref_length, double x0, double y0, double *out);
+
+/* ------------------------------------------------------------------------
+ * Device-resident solver context.  Replaces the per-iteration host work of
+ * Solver._assemble_and_solve_once (vorti2d/solver.py:117-134): the assembly
+ * (core.assemble_coo), the COO->CSR pattern of MumpsSolver.assemble
+ * (vorti2d/petsc_solver.py:76-95) and the MUMPS solve (:97-115).  Mesh,
+ * metrics, state, Jacobian and factorisation stay in HBM; only scalars cross
+ * the boundary per Newton step.  One HIP stream per context; calls on one
+ * context are not re-entrant.
+ * ---------------------------------------------------------------------- */
+typedef struct v2d_ctx v2d_ctx;
+
+enum { V2D_FP64 = 0, V2D_MIXED = 1 };
+
+/* Upload the mesh, compute the metrics on the device, build the fixed
+ * Jacobian pattern for (ff_bc, ca, sa) and allocate the solver.  precision:
+ * V2D_FP64 (FP64 Jacobian, exact FP64 block LU: the reference's exact Newton
+ * step) or V2D_MIXED (FP32 Jacobian and block-LU preconditioner, FP32 GMRES,
+ * FP64 residual and state). */
+int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const double *yg,
+                   int ff_bc, double ca, double sa, int precision);
+int v2d_ctx_destroy(v2d_ctx *ctx);
+/* Bytes of device memory held by the context. */
+int v2d_ctx_device_bytes(v2d_ctx *ctx, int64_t *bytes);
+/* D2H copy of the 10 metric planes (10*N doubles). */
+int v2d_ctx_metrics(v2d_ctx *ctx, double *out);
+/* State upload / download; any pointer may be NULL to skip that plane. */
+int v2d_ctx_set_state(v2d_ctx *ctx, const double *psi, const double *ome, const double *omeold,
+                      const double *omeoldold);
+int v2d_ctx_get_state(v2d_ctx *ctx, double *psi, double *ome, double *omeold, double *omeoldold);
+/* Set the initial state of vorti2d/solver.py:103-114 on the device: zeros,
+ * far-field psi = ca*y - sa*x. */
+int v2d_ctx_init_state(v2d_ctx *ctx);
+/* Linear-solver knobs for V2D_MIXED: refactor the FP32 block LU every
+ * refactor_every Newton steps (0 = only when GMRES needs more than
+ * gmres_max_iter iterations), inner relative tolerance, iteration cap. */
+int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int gmres_max_iter);
+
+/* Device assembly of the Jacobian values and the FP64 residual-form RHS for
+ * the current state (the scalars of assemble_coo). */
+int v2d_ctx_assemble(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot);
+/* ||rhs||_2 in FP64 of the last assembly. */
+int v2d_ctx_rhs_norm(v2d_ctx *ctx, double *out);
+/* Number of Jacobian entries (equals the reference nnz). */
+int v2d_ctx_nnz(v2d_ctx *ctx, int64_t *nnz);
+/* Export of the last assembly in the reference COO order (all 2N rows) and
+ * the RHS; values widened to FP64. */
+int v2d_ctx_export_coo(v2d_ctx *ctx, int *coo_i, int *coo_j, double *coo_v, double *bvec);
+/* Canonical CSR of the last assembly (columns ascending, 2N+1 row pointers). */
+int v2d_ctx_export_csr(v2d_ctx *ctx, int64_t *indptr, int *indices, double *data);
+/* y = A x with the last assembly (host vectors of 2N doubles). */
+int v2d_ctx_spmv(v2d_ctx *ctx, const double *x, double *y);
+/* Factor the last assembly with the block LU of the context precision. */
+int v2d_ctx_factor(v2d_ctx *ctx);
+/* x = M b with the current factorisation (host vectors of 2N doubles). */
+int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x);
+
+/* One inner iteration of _assemble_and_solve_once (vorti2d/solver.py:117-134)
+ * on the device: assemble, solve A d = b, psi += d[:N], ome += d[N:].
+ * res = ||d||_2 / (2N) (the reference convergence measure), rnorm = ||b||_2 of
+ * this assembly, lin_iters = GMRES iterations (0 for the direct FP64 step). */
+int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot,
+                        double *res, double *rnorm, int *lin_iters);
+/* BDF2 history shift omeoldold <- omeold <- ome (vorti2d/solver.py:186-188). */
+int v2d_ctx_bdf2_shift(v2d_ctx *ctx);
+/* Force coefficients of the current state (vorti2d/solver.py:272-283). */
+int v2d_ctx_forces(v2d_ctx *ctx, double re, double ref_length, double x0, double y0,
+                   double *out);
+/* Accumulated device time (ms) of assembly, factorisation, linear solve and
+ * whole Newton steps since creation or the last reset; counts[0] = steps,
+ * counts[1] = factorisations, counts[2] = GMRES iterations. */
+int v2d_ctx_timers(v2d_ctx *ctx, double *ms4, int64_t *counts3, int reset);
+/* Average duration (ms) of the last `launches` assembly kernels, measured with
+ * HIP events on the context stream around a loop of that many launches. */
+int v2d_ctx_time_assembly(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot,
+                          int launches, double *ms_per_launch);
+/* Same for the stencil SpMV y = A x on the last assembly (x = ones). */
+int v2d_ctx_time_spmv(v2d_ctx *ctx, int launches, double *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VORTI2D_H */

@@ -1,0 +1,337 @@
+"""GPU parity of the HIP path against the oracle and the reference fixtures.
+
+Bars: integer outputs (COO indices, row pointers, nnz) bit-exact; FP64 metrics,
+Jacobian values and RHS bit-exact (the kernels keep the reference operation
+order, IEEE division and square root, no FMA contraction); FP32 Jacobian values
+equal to the FP32 rounding of the FP64 values; linear solves to 1e-9 relative;
+converged psi, omega within 1e-8 and force coefficients within 1e-6 of the FP64
+oracle Newton solve (reference assembly + exact sparse LU).
+"""
+from __future__ import annotations
+
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def v():
+    import vorti2d_amd
+    return vorti2d_amd
+
+
+# ------------------------------------------------------------------ metrics --
+@pytest.mark.parametrize("name", ["cyl_25x17", "cyl_41x41", "ann_33x21"])
+def test_metrics_golden_bitexact(v, name):
+    g = golden("metrics_" + name)
+    m = v.core.compute_metrics(float(g["dksi"]), float(g["deta"]), g["xg"], g["yg"])
+    for q in range(10):
+        assert np.array_equal(m[q], g["metrics"][q]), q
+
+
+@pytest.mark.parametrize("imax,jmax,kind", [(181, 181, "ann"), (181, 181, "cyl"),
+                                            (257, 129, "cyl"), (1025, 1025, "ann")])
+def test_metrics_vs_oracle(v, C, oracle_mod, imax, jmax, kind):
+    say(f"metrics {kind} {imax}x{jmax}")
+    gen = oracle_mod.generate_annulus if kind == "ann" else oracle_mod.generate_cylinder
+    xg, yg = gen(imax, jmax)
+    dk, de = 1.0 / (imax - 1), 1.0 / (jmax - 1)
+    a = v.core.compute_metrics(dk, de, xg, yg)
+    b = C.compute_metrics(dk, de, xg, yg)
+    for q in range(10):
+        assert np.array_equal(a[q], b[q]), q
+
+
+# ----------------------------------------------------------------- assembly --
+@pytest.mark.parametrize("name", ["cyl_25x17_ff1", "cyl_41x41_ff0", "ann_33x21_ff2_a12"])
+def test_assemble_coo_golden_bitexact(v, name):
+    g = golden("assemble_" + name)
+    xg, yg = g["xg"], g["yg"]
+    imax, jmax = xg.shape
+    dk, de = 1.0 / (imax - 1), 1.0 / (jmax - 1)
+    m = v.core.compute_metrics(dk, de, xg, yg)
+    N = imax * jmax
+    ci, cj, cv, nnz, b = v.core.assemble_coo(
+        imax, jmax, float(g["re"]), float(g["invdtau"]), float(g["inv2dt"]), float(g["urot"]),
+        dk, de, *m, *g["state"], 0, 2 * N, 26 * N, int(g["ff_bc"]), float(g["ca"]), float(g["sa"]))
+    assert nnz == g["coo_i"].size
+    assert np.array_equal(ci[:nnz], g["coo_i"]) and np.array_equal(cj[:nnz], g["coo_j"])
+    assert np.array_equal(cv[:nnz], g["coo_v"]) and np.array_equal(b, g["bvec"])
+
+
+def test_assemble_coo_row_ranges_vs_oracle(v, C, oracle_mod):
+    imax, jmax = 61, 41
+    xg, yg = oracle_mod.generate_cylinder(imax, jmax)
+    dk, de = 1.0 / (imax - 1), 1.0 / (jmax - 1)
+    m = C.compute_metrics(dk, de, xg, yg)
+    N = imax * jmax
+    rng = np.random.default_rng(11)
+    st = [rng.standard_normal(N) for _ in range(4)]
+    for (r0, r1) in ((0, 2 * N), (0, N), (N, 2 * N), (37, 2 * N - 5), (N - 61, N + 61), (5, 5)):
+        for ff, ca, sa in ((0, 1.0, 0.0), (1, 0.8, 0.6), (2, -0.6, 0.8)):
+            args = (imax, jmax, 45.0, 0.2, 2.5, 0.3, dk, de, *m, *st, r0, r1,
+                    13 * max(r1 - r0, 1), ff, ca, sa)
+            a, b = v.core.assemble_coo(*args), C.assemble_coo(*args)
+            assert a[3] == b[3], (r0, r1, ff)
+            n = a[3]
+            for q in range(3):
+                assert np.array_equal(a[q][:n], b[q][:n]), (r0, r1, ff, q)
+            assert np.array_equal(a[4], b[4])
+
+
+def test_assemble_coo_capacity_checked(v, oracle_mod):
+    from vorti2d_amd._lib import V2DError
+    imax, jmax = 9, 7
+    xg, yg = oracle_mod.generate_cylinder(imax, jmax)
+    m = v.core.compute_metrics(1 / 8, 1 / 6, xg, yg)
+    z = np.zeros(imax * jmax)
+    with pytest.raises(V2DError, match="maxnnz"):
+        v.core.assemble_coo(imax, jmax, 40, 0, 0, 0, 1 / 8, 1 / 6, *m, z, z, z, z, 0,
+                            2 * imax * jmax, 100, 0, 1.0, 0.0)
+
+
+def test_local_variants_golden(v):
+    g = golden("local_cyl_25x17")
+    ml = v.core.compute_metrics_local(int(g["il0"]), int(g["il1"]), float(g["dksi"]),
+                                      float(g["deta"]), g["xgl"], g["ygl"])
+    for q in range(10):
+        assert np.array_equal(ml[q], g["metrics_local"][q]), q
+    gxm, jmax = g["xgl"].shape
+    ci, cj, cv, nnz, b = v.core.assemble_coo_local(
+        gxm, jmax, int(g["il0"]), int(g["il1"]), float(g["re"]), float(g["invdtau"]),
+        float(g["inv2dt"]), float(g["urot"]), float(g["dksi"]), float(g["deta"]),
+        *g["loc_metrics"], *g["state"], 0, int(g["nloc"]), g["lg"], 2 * int(g["nloc"]) * 13,
+        int(g["ff_bc"]), float(g["ca"]), float(g["sa"]))
+    assert nnz == g["coo_i"].size
+    assert np.array_equal(ci[:nnz], g["coo_i"]) and np.array_equal(cj[:nnz], g["coo_j"])
+    assert np.array_equal(cv[:nnz], g["coo_v"]) and np.array_equal(b, g["bvec"])
+
+
+# ------------------------------------------------- device context / pattern --
+def _ctx_case(v, oracle_mod, C, imax, jmax, kind, prec, ff=0, ca=1.0, sa=0.0, seed=3):
+    gen = oracle_mod.generate_annulus if kind == "ann" else oracle_mod.generate_cylinder
+    xg, yg = gen(imax, jmax)
+    ctx = v.DeviceContext(xg, yg, ff, ca, sa, prec)
+    m = C.compute_metrics(1.0 / (imax - 1), 1.0 / (jmax - 1), xg, yg)
+    N = imax * jmax
+    psi, ome, o1, o2 = oracle_mod.smooth_state(m[8], m[9])
+    rng = np.random.default_rng(seed)
+    psi = psi + 0.01 * rng.standard_normal(N)
+    ctx.set_state(psi, ome, o1, o2)
+    return ctx, m, (psi, ome, o1, o2)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("ff,ca,sa", [(0, 1.0, 0.0), (1, 1.0, 0.0), (2, 0.8, 0.6)])
+def test_ctx_export_matches_oracle(v, C, oracle_mod, prec, ff, ca, sa):
+    imax, jmax = 49, 33
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, "cyl", prec, ff, ca, sa)
+    N = imax * jmax
+    sc = (70.0, 0.5, 2.5, 0.25)
+    ctx.assemble(*sc)
+    ci, cj, cv, b = ctx.export_coo()
+    oi, oj, ov, nnz, ob = C.assemble_coo(imax, jmax, sc[0], sc[1], sc[2], sc[3], 1 / 48, 1 / 32,
+                                         *m, *st, 0, 2 * N, 26 * N, ff, ca, sa)
+    assert ctx.nnz() == nnz
+    assert np.array_equal(ci, oi[:nnz]) and np.array_equal(cj, oj[:nnz])
+    if prec == "fp64":
+        assert np.array_equal(cv, ov[:nnz])
+    else:
+        assert np.array_equal(cv, ov[:nnz].astype(np.float32).astype(np.float64))
+    assert np.array_equal(b, ob)
+    # canonical CSR == scipy's COO->CSR (vorti2d/petsc_solver.py:76-95)
+    ip, ix, dv = ctx.export_csr()
+    A = sp.csr_matrix((cv, (ci, cj)), shape=(2 * N, 2 * N))
+    A.sort_indices()
+    assert np.array_equal(ip, A.indptr) and np.array_equal(ix, A.indices)
+    assert np.array_equal(dv, A.data)
+    # stencil SpMV
+    x = np.random.default_rng(4).standard_normal(2 * N)
+    y = ctx.spmv(x)
+    ref = A @ x
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(A).dot(np.abs(x)).max()
+    assert abs(ctx.rhs_norm() - np.linalg.norm(ob)) <= 1e-13 * np.linalg.norm(ob)
+
+
+@pytest.mark.parametrize("imax,jmax,kind,ff", [(41, 41, "cyl", 0), (57, 33, "ann", 2),
+                                               (181, 181, "ann", 1)])
+def test_direct_solve_fp64(v, C, oracle_mod, imax, jmax, kind, ff):
+    say(f"direct fp64 {kind} {imax}x{jmax} ff={ff}")
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "fp64", ff)
+    ctx.assemble(40.0, 0.0, 2.5, 0.1)
+    ci, cj, cv, b = ctx.export_coo()
+    N = imax * jmax
+    A = sp.csc_matrix((cv, (ci, cj)), shape=(2 * N, 2 * N))
+    ctx.factor()
+    x = ctx.solve(b)
+    xs = spla.spsolve(A, b)
+    assert np.linalg.norm(b - A @ x) <= 1e-11 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_direct_solve_mixed_preconditioner(v, C, oracle_mod):
+    imax = jmax = 91
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, "cyl", "mixed")
+    ctx.assemble(40.0, 0.0, 0.0, 0.0)
+    ci, cj, cv, b = ctx.export_coo()
+    N = imax * jmax
+    A = sp.csc_matrix((cv, (ci, cj)), shape=(2 * N, 2 * N))
+    ctx.factor()
+    x = ctx.solve(b)
+    # FP32 block LU of the FP32 matrix: a preconditioner good to FP32 accuracy
+    assert np.linalg.norm(b - A @ x) <= 1e-3 * np.linalg.norm(b)
+
+
+# ------------------------------------------------------------ Newton solve --
+def _run(v, xg, yg, **kw):
+    with tempfile.TemporaryDirectory() as tmp:
+        cfg = v.Config(out_dir=tmp, verbose=False, write_csv=False, write_xdmf=False,
+                       compute_forces=False, **kw)
+        return v.Solver(cfg, mesh=(xg, yg)).run()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_newton_41_matches_reference_state(v, prec):
+    g = golden("newton_41x41_re40")
+    s = _run(v, g["xg"], g["yg"], re=40.0, steady=True, pseudo_tol=1e-10, max_pseudo_iter=50,
+             precision=prec)
+    psi, ome = s.psi, s.ome
+    assert np.abs(psi - g["psi"]).max() < 1e-8
+    assert np.abs(ome - g["ome"]).max() < 1e-8
+    if prec == "fp64":
+        assert len(s.res_history[0]) == int(g["iters"])
+        assert np.allclose(s.res_history[0], np.log10(g["res_hist"]), atol=1e-3)
+    fc = s.force_coeffs()
+    want = g["forces"]
+    assert abs(fc.cd - want[0]) < 1e-6 and abs(fc.cl - want[1]) < 1e-6
+    assert abs(fc.cm - want[2]) < 1e-6 and abs(fc.ref_length - want[9]) < 1e-12
+    imax, jmax = g["xg"].shape
+    N = imax * jmax
+    ff = slice(imax * (jmax - 1), N)
+    wall = slice(0, imax)
+    assert np.max(np.abs(psi[ff] - s.yphys[ff])) < 1e-10     # test_smoke.py:50-56
+    assert np.max(np.abs(psi[wall])) < 1e-10
+    assert np.max(np.abs(ome[ff])) < 1e-12
+    assert abs(psi.max() + psi.min()) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["ann", "cyl"])
+def test_newton_181_matches_oracle(v, C, oracle_mod, kind):
+    """Config C1: 181x181 O-grid, Re=40 steady, Dirichlet far field, alpha 0."""
+    imax = jmax = 181
+    gen = oracle_mod.generate_annulus if kind == "ann" else oracle_mod.generate_cylinder
+    xg, yg = gen(imax, jmax)
+    m = C.compute_metrics(1 / 180, 1 / 180, xg, yg)
+    say(f"oracle Newton 181 {kind}")
+    r = oracle_mod.newton_solve(C, imax, jmax, m, 40.0, tol=1e-10)
+    fo = oracle_mod.force_coeffs(imax, jmax, 40.0, 1 / 180, 1 / 180, m[2], m[3], m[6], m[7],
+                                 m[8], m[9], r.ome)
+    for prec in ("fp64", "mixed"):
+        say(f"device Newton 181 {kind} {prec}")
+        s = _run(v, xg, yg, re=40.0, steady=True, pseudo_tol=1e-10, max_pseudo_iter=50,
+                 precision=prec)
+        assert np.abs(s.psi - r.psi).max() < 1e-8, prec
+        assert np.abs(s.ome - r.ome).max() < 1e-8, prec
+        fc = s.force_coeffs()
+        for k in ("cd", "cl", "cm"):
+            assert abs(getattr(fc, k) - fo[k]) < 1e-6, (prec, k)
+        assert s.rnorm_history[0][-1] < 1e-6
+
+
+def test_forces_host_entry_vs_oracle(v, C, oracle_mod):
+    g = golden("newton_41x41_re40")
+    xg, yg = g["xg"], g["yg"]
+    imax, jmax = xg.shape
+    m = C.compute_metrics(1 / 40, 1 / 40, xg, yg)
+    for ref_len, mc in ((None, (0.0, 0.0)), (2.0, (0.25, -0.1))):
+        fc = v.compute_force_coeffs(imax, jmax, 40.0, 1 / 40, 1 / 40, m[2], m[3], m[6], m[7],
+                                    m[8], m[9], g["ome"], ref_length=ref_len, moment_ref=mc)
+        fo = oracle_mod.force_coeffs(imax, jmax, 40.0, 1 / 40, 1 / 40, m[2], m[3], m[6], m[7],
+                                     m[8], m[9], g["ome"], ref_length=ref_len, moment_ref=mc)
+        for k, val in fo.items():
+            assert abs(getattr(fc, k) - val) <= 1e-12 * max(1.0, abs(val)), k
+
+
+def test_angle_of_attack_farfield(v):
+    """tests/test_postproc.py:121-142 (41x41 for speed)."""
+    xg, yg = v.generate_cylinder(41, 41, 0.5, 50.0)
+    s = _run(v, xg, yg, re=20.0, steady=True, alpha_deg=12.0, pseudo_tol=1e-11,
+             max_pseudo_iter=80)
+    imax, jmax = s.imax, s.jmax
+    ff = slice(imax * (jmax - 1), imax * jmax)
+    ca, sa = s.cfg.cos_alpha(), s.cfg.sin_alpha()
+    assert np.max(np.abs(s.psi[ff] - (ca * s.yphys[ff] - sa * s.xphys[ff]))) < 1e-9
+
+
+def test_farfield_outflow_option(v):
+    """tests/test_postproc.py:87-118: omega-only outflow leaves the steady Cd
+    unchanged on a large domain."""
+    xg, yg = v.generate_cylinder(101, 151, 0.5, 100.0)
+    cd = {}
+    for bc in ("dirichlet", "outflow"):
+        s = _run(v, xg, yg, re=20.0, steady=True, farfield_bc=bc, pseudo_tol=1e-11,
+                 max_pseudo_iter=80)
+        fc = s.force_coeffs()
+        assert abs(fc.cl) < 1e-3
+        cd[bc] = fc.cd
+    assert abs(cd["outflow"] - cd["dirichlet"]) < 1e-6
+
+
+def test_unsteady_restart_equals_continuous(v):
+    """tests/test_smoke.py:59-74 and tests/test_restart.py:37-54: an unsteady run
+    resumed from its restart file matches the continuous run."""
+    xg, yg = v.generate_cylinder(41, 41, 0.5, 50.0)
+    common = dict(re=40.0, steady=False, dt_phys=0.5, rot_speed=0.5, rot_until=2.0,
+                  pseudo_tol=1e-9, max_pseudo_iter=50, verbose=False, write_csv=False,
+                  write_xdmf=False)
+    with tempfile.TemporaryDirectory() as tmp:
+        full = v.Solver(v.Config(out_dir=os.path.join(tmp, "a"), t_end=1.5, **common),
+                        mesh=(xg, yg)).run()
+        part = v.Solver(v.Config(out_dir=os.path.join(tmp, "b"), t_end=1.0,
+                                 restart_out="r.npz", **common), mesh=(xg, yg)).run()
+        rpath = os.path.join(tmp, "b", "r.npz")
+        assert os.path.exists(rpath)
+        res = v.Solver(v.Config(out_dir=os.path.join(tmp, "c"), t_end=1.5, restart_in=rpath,
+                                **common), mesh=(xg, yg)).run()
+        assert res.t == pytest.approx(1.5) and full.t == pytest.approx(1.5)
+        assert np.abs(res.psi - full.psi).max() < 1e-9
+        assert np.abs(res.ome - full.ome).max() < 1e-9
+        rows = open(os.path.join(tmp, "a", "forces.csv")).read().strip().splitlines()
+        assert rows[0].startswith("t,cd,cl,cm") and len(rows) == 1 + 4   # t = 0, .5, 1, 1.5
+
+
+def test_full_size_assembly_1025(v, C, oracle_mod):
+    """Config C3 size (1025x1025, nnz 24,124,400): the device pattern, values and
+    RHS equal the oracle's at full size."""
+    imax = jmax = 1025
+    xg, yg = oracle_mod.generate_annulus(imax, jmax)
+    say("full-size ctx create")
+    ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, "fp64")
+    m = ctx.metrics()
+    N = imax * jmax
+    psi, ome, o1, o2 = oracle_mod.smooth_state(m[8], m[9])
+    ctx.set_state(psi, ome, o1, o2)
+    ctx.assemble(40.0)
+    assert ctx.nnz() == 24124400
+    ci, cj, cv, b = ctx.export_coo()
+    del ctx
+    say("full-size oracle assembly")
+    oi, oj, ov, nnz, ob = C.assemble_coo(imax, jmax, 40.0, 0, 0, 0, 1 / 1024, 1 / 1024, *m, psi,
+                                         ome, o1, o2, 0, 2 * N, 23 * N, 0, 1.0, 0.0)
+    assert nnz == 24124400
+    assert np.array_equal(ci, oi[:nnz]) and np.array_equal(cj, oj[:nnz])
+    assert np.array_equal(cv, ov[:nnz]) and np.array_equal(b, ob)

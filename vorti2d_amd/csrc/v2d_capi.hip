@@ -1,0 +1,957 @@
+// v2d_capi.hip -- C ABI of libvorti2d: parity entry points, the device-resident
+// solver context, the mixed-precision GMRES driver and the Newton step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/vorti2d.h"
+#include "v2d_internal.h"
+
+namespace v2d {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+// RAII device buffer for the host-pointer entry points
+struct DBuf {
+    void *p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    template <typename T> T *as() { return static_cast<T *>(p); }
+};
+#define V2D_ALLOC(buf, bytes) V2D_HIP(hipMalloc(&(buf).p, std::max<size_t>((bytes), 8)))
+
+static int check_grid(int imax, int jmax)
+{
+    if (imax < 4 || jmax < 4) {
+        set_error("grid must have imax >= 4 and jmax >= 4 (got " + std::to_string(imax) + "x" +
+                  std::to_string(jmax) + ")");
+        return -1;
+    }
+    if ((int64_t)imax * jmax * kSlots >= (int64_t)INT32_MAX) {
+        set_error("grid too large for 32-bit value indices");
+        return -1;
+    }
+    return 0;
+}
+
+// outflow flags of the far-field row (vorti2d_core.f90:255, 308) and their
+// exclusive prefix counts (the COO offsets of the reference emission order)
+static void outflow_flags(int imax, const double *ex_last, const double *ey_last, int ff_bc,
+                          double ca, double sa, std::vector<uint8_t> &fl, std::vector<int> &pp,
+                          std::vector<int> &po, int &npsi)
+{
+    fl.assign(imax, 0);
+    pp.assign(imax, 0);
+    po.assign(imax, 0);
+    int cp = 0, co = 0;
+    for (int i = 0; i < imax; ++i) {
+        const bool out = (ca * ex_last[i] + sa * ey_last[i]) > 0.0;
+        uint8_t f = 0;
+        if (ff_bc >= 1 && out) f |= kOutOme;
+        if (ff_bc >= 2 && out) f |= kOutPsi;
+        fl[i] = f;
+        pp[i] = cp;
+        po[i] = co;
+        cp += (f & kOutPsi) ? 1 : 0;
+        co += (f & kOutOme) ? 1 : 0;
+    }
+    npsi = cp;
+}
+
+static int64_t count_rows(int imax, int jmax, int64_t r0, int64_t r1, const std::vector<uint8_t> &fl)
+{
+    const int64_t N = (int64_t)imax * jmax;
+    int64_t c = 0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int comp = r >= N;
+        const int64_t k = comp ? r - N : r;
+        const int j = (int)(k / imax), i = (int)(k % imax);
+        c += row_count(jmax, j, comp, j == jmax - 1 ? fl[i] : 0);
+    }
+    return c;
+}
+
+// ----------------------------------------------------------- context -----
+struct Ctx {
+    int imax = 0, jmax = 0, prec = V2D_FP64, ff_bc = 0;
+    int64_t N = 0;
+    double dksi = 0, deta = 0, ca = 1, sa = 0;
+    hipStream_t s = nullptr;
+    int64_t bytes = 0;
+    double *x = nullptr, *y = nullptr, *m = nullptr;
+    double *psi = nullptr, *ome = nullptr, *omeold = nullptr, *omeoldold = nullptr;
+    double *rhs = nullptr, *delta = nullptr, *red = nullptr, *partial = nullptr, *hbuf = nullptr;
+    uint8_t *flags = nullptr;
+    int *pre_psi = nullptr, *pre_ome = nullptr;
+    int npsi_out = 0;
+    int64_t nnz = 0;
+    std::vector<uint8_t> hflags;
+    void *val = nullptr;   // kSlots * N of T
+    DirectPattern pat;
+    DirectWork<double> wd;
+    DirectWork<float> wf;
+    // GMRES (mixed) work, FP32 vectors
+    int gm_cap = 30;   // allocated Krylov vectors
+    int gm_max = 30;
+    double gm_rtol = 1e-6;
+    int refactor_every = 1;
+    float *gV = nullptr, *gZ = nullptr, *gb = nullptr, *gx = nullptr;
+    double *gxd = nullptr;
+    bool factored = false;
+    int since_factor = 0;
+    double last_re = 0, last_invdtau = 0, last_inv2dt = 0, last_urot = 0;
+    // timers
+    hipEvent_t ev[4];
+    double t_asm = 0, t_fac = 0, t_sol = 0, t_tot = 0;
+    int64_t n_steps = 0, n_fac = 0, n_lin = 0;
+    std::vector<void *> allocs;
+
+    template <typename T> T *alloc(int64_t count, int &rc)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * std::max<int64_t>(count, 1)) != hipSuccess) {
+            set_error("hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed");
+            rc = -1;
+            return nullptr;
+        }
+        allocs.push_back(p);
+        bytes += sizeof(T) * std::max<int64_t>(count, 1);
+        return static_cast<T *>(p);
+    }
+    ~Ctx()
+    {
+        for (void *p : allocs) (void)hipFree(p);
+        direct_free_pattern(pat);
+        if (s) (void)hipStreamDestroy(s);
+        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    }
+    AsmArgs args(double re, double invdtau, double inv2dt, double urot) const
+    {
+        AsmArgs a;
+        a.imax = imax; a.jmax = jmax; a.ff_bc = ff_bc;
+        a.re = re; a.invdtau = invdtau; a.inv2dt = inv2dt; a.urot = urot;
+        a.dksi = dksi; a.deta = deta; a.ca = ca; a.sa = sa;
+        for (int q = 0; q < 10; ++q) a.m[q] = m + q * N;
+        a.psi = psi; a.ome = ome; a.omeold = omeold; a.omeoldold = omeoldold;
+        return a;
+    }
+};
+
+static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
+{
+    c->ff_bc = ff_bc; c->ca = ca; c->sa = sa;
+    const int64_t N = c->N;
+    std::vector<double> ex(c->imax), ey(c->imax);
+    V2D_HIP(hipMemcpy(ex.data(), c->m + 6 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(ey.data(), c->m + 7 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
+    std::vector<int> pp, po;
+    outflow_flags(c->imax, ex.data(), ey.data(), ff_bc, ca, sa, c->hflags, pp, po, c->npsi_out);
+    V2D_HIP(hipMemcpy(c->flags, c->hflags.data(), c->imax, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(c->pre_psi, pp.data(), sizeof(int) * c->imax, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(c->pre_ome, po.data(), sizeof(int) * c->imax, hipMemcpyHostToDevice));
+    c->nnz = count_rows(c->imax, c->jmax, 0, 2 * N, c->hflags);
+    direct_free_pattern(c->pat);
+    if (direct_build_pattern(c->pat, c->imax, c->jmax, c->hflags.data())) return -1;
+    c->factored = false;
+    return 0;
+}
+
+// ------------------------------------------------------------- GMRES ------
+// Right-preconditioned GMRES with the block LU M, classical Gram-Schmidt with
+// one re-orthogonalisation (fused multi-dot / multi-axpy), Givens rotations on
+// the host.  Solves A x = b for unit-norm b (FP32 vectors, FP64 dot products);
+// Z_j = M V_j is kept so the solution is x = Z y (flexible form).
+static int gmres_f32(Ctx *c, int *iters, double *relres)
+{
+    const int64_t n = 2 * c->N;
+    const int m = c->gm_max;
+    const float *val = static_cast<const float *>(c->val);
+    const unsigned nb = std::min<unsigned>(1024, grid_for(n));
+    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1);
+    // v0 = b (unit norm already) -> gV[0]
+    V2D_HIP(hipMemcpyAsync(c->gV, c->gb, sizeof(float) * n, hipMemcpyDeviceToDevice, c->s));
+    g[0] = 1.0;
+    int j = 0;
+    double est = 1.0;
+    for (; j < m; ++j) {
+        float *vj = c->gV + j * n, *zj = c->gZ + j * n, *w = c->gV + (j + 1) * n;
+        if (direct_solve<float>(c->pat, c->wf, val, vj, zj, c->s)) return -1;
+        hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
+                           c->jmax, c->flags, val, zj, w);
+        V2D_LAUNCH();
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, j + 1, c->gV, n, w,
+                               c->partial);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, j + 1,
+                               c->partial, c->hbuf);
+            hipLaunchKernelGGL(k_multiaxpy<float>, dim3(grid_for(n)), dim3(256), 0, c->s, n, j + 1,
+                               c->gV, n, c->hbuf, w);
+            V2D_LAUNCH();
+            V2D_HIP(hipMemcpyAsync(pass ? h2.data() : h.data(), c->hbuf, sizeof(double) * (j + 1),
+                                   hipMemcpyDeviceToHost, c->s));
+        }
+        hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, 1, w, n, w, c->partial);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, 1, c->partial,
+                           c->red);
+        V2D_LAUNCH();
+        double wn2 = 0.0;
+        V2D_HIP(hipMemcpyAsync(&wn2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        V2D_HIP(hipStreamSynchronize(c->s));
+        const double hn = std::sqrt(std::max(wn2, 0.0));
+        for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
+        H[(j + 1) * m + j] = hn;
+        {   // v_{j+1} = w / hn
+            const double hv = hn > 0.0 ? hn : 1.0;
+            V2D_HIP(hipMemcpyAsync(c->red, &hv, sizeof(double), hipMemcpyHostToDevice, c->s));
+            hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, w,
+                               c->red, 1, w);
+            V2D_LAUNCH();
+        }
+        for (int i = 0; i < j; ++i) {
+            const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+            H[(i + 1) * m + j] = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+            H[i * m + j] = t;
+        }
+        const double d = std::hypot(H[j * m + j], H[(j + 1) * m + j]);
+        cs[j] = d > 0.0 ? H[j * m + j] / d : 1.0;
+        sn[j] = d > 0.0 ? H[(j + 1) * m + j] / d : 0.0;
+        H[j * m + j] = d;
+        H[(j + 1) * m + j] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        est = std::fabs(g[j + 1]);
+        if (!std::isfinite(est)) {
+            set_error("GMRES breakdown: non-finite residual estimate");
+            return -1;
+        }
+        if (est <= c->gm_rtol || hn == 0.0) { ++j; break; }
+    }
+    const int k = std::min(j, m);
+    std::vector<double> yv(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+        double acc = g[i];
+        for (int q = i + 1; q < k; ++q) acc -= H[i * m + q] * yv[q];
+        yv[i] = acc / H[i * m + i];
+    }
+    V2D_HIP(hipMemcpyAsync(c->hbuf, yv.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->s));
+    hipLaunchKernelGGL((k_fill_zero_d), dim3(grid_for(n)), dim3(256), 0, c->s, n, c->gxd);
+    hipLaunchKernelGGL((k_update_sol<float, double>), dim3(grid_for(n)), dim3(256), 0, c->s, n, k, c->gZ,
+                       n, c->hbuf, 1.0, c->gxd);
+    V2D_LAUNCH();
+    *iters = k;
+    *relres = est;
+    return 0;
+}
+
+}  // namespace v2d
+
+using namespace v2d;
+
+struct v2d_ctx {
+    std::unique_ptr<Ctx> c;
+};
+
+extern "C" {
+
+const char *v2d_last_error(void) { return g_err.c_str(); }
+int v2d_version(void) { return 100; }
+
+int v2d_node_neighbors(int i, int j, int imax, int out[9])
+{
+    if (imax < 4 || i < 1 || i > imax || j < 1) {
+        set_error("node_neighbors: index out of range");
+        return -1;
+    }
+    const Nbr b = neighbors(i - 1, j - 1, imax);
+    const int v[9] = {b.k, b.e, b.w, b.n, b.s, b.ne, b.nw, b.se, b.sw};
+    for (int q = 0; q < 9; ++q) out[q] = v[q] + 1;
+    return 0;
+}
+
+int v2d_compute_metrics_dev(int imax, int jmax, double dksi, double deta, const double *d_xg,
+                            const double *d_yg, double *d_out, void *stream)
+{
+    if (check_grid(imax, jmax)) return -1;
+    const int64_t N = (int64_t)imax * jmax;
+    hipLaunchKernelGGL(k_metrics, dim3(grid_for(N)), dim3(256), 0, (hipStream_t)stream, imax, jmax,
+                       dksi, deta, d_xg, d_yg, d_out);
+    V2D_LAUNCH();
+    return 0;
+}
+
+int v2d_compute_metrics(int imax, int jmax, double dksi, double deta, const double *xg,
+                        const double *yg, double *out)
+{
+    if (check_grid(imax, jmax)) return -1;
+    const int64_t N = (int64_t)imax * jmax;
+    DBuf dx, dy, dout;
+    V2D_ALLOC(dx, sizeof(double) * N);
+    V2D_ALLOC(dy, sizeof(double) * N);
+    V2D_ALLOC(dout, sizeof(double) * 10 * N);
+    V2D_HIP(hipMemcpy(dx.p, xg, sizeof(double) * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(dy.p, yg, sizeof(double) * N, hipMemcpyHostToDevice));
+    if (v2d_compute_metrics_dev(imax, jmax, dksi, deta, dx.as<double>(), dy.as<double>(),
+                                dout.as<double>(), nullptr))
+        return -1;
+    V2D_HIP(hipMemcpy(out, dout.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int v2d_compute_metrics_local(int gxm, int jmax, int il0, int il1, double dksi, double deta,
+                              const double *xgl, const double *ygl, double *out)
+{
+    if (gxm < 3 || jmax < 4 || il0 < 2 || il1 > gxm - 1 || il0 > il1) {
+        set_error("compute_metrics_local: owned columns must satisfy 2 <= il0 <= il1 <= gxm-1");
+        return -1;
+    }
+    const int64_t N = (int64_t)gxm * jmax;
+    DBuf dx, dy, dout;
+    V2D_ALLOC(dx, sizeof(double) * N);
+    V2D_ALLOC(dy, sizeof(double) * N);
+    V2D_ALLOC(dout, sizeof(double) * 10 * N);
+    V2D_HIP(hipMemcpy(dx.p, xgl, sizeof(double) * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(dy.p, ygl, sizeof(double) * N, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_metrics_local, dim3(grid_for(N)), dim3(256), 0, nullptr, gxm, jmax, il0 - 1,
+                       il1 - 1, dksi, deta, dx.as<double>(), dy.as<double>(), dout.as<double>());
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpy(out, dout.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// upload 10 metric planes + 4 state planes into one device block (14 N)
+static int upload_fields(DBuf &buf, int64_t N, const double *const *metrics, const double *psi,
+                         const double *ome, const double *omeold, const double *omeoldold,
+                         AsmArgs &a)
+{
+    V2D_ALLOC(buf, sizeof(double) * 14 * N);
+    double *b = buf.as<double>();
+    for (int q = 0; q < 10; ++q) {
+        V2D_HIP(hipMemcpy(b + q * N, metrics[q], sizeof(double) * N, hipMemcpyHostToDevice));
+        a.m[q] = b + q * N;
+    }
+    const double *st[4] = {psi, ome, omeold, omeoldold};
+    for (int q = 0; q < 4; ++q)
+        V2D_HIP(hipMemcpy(b + (10 + q) * N, st[q], sizeof(double) * N, hipMemcpyHostToDevice));
+    a.psi = b + 10 * N; a.ome = b + 11 * N; a.omeold = b + 12 * N; a.omeoldold = b + 13 * N;
+    return 0;
+}
+
+int v2d_assemble_coo(int imax, int jmax, double re, double invdtau, double inv2dt, double urot,
+                     double dksi, double deta, const double *const *metrics, const double *psi,
+                     const double *ome, const double *omeold, const double *omeoldold, int r0,
+                     int r1, int maxnnz, int ff_bc, double ca, double sa, int *coo_i, int *coo_j,
+                     double *coo_v, int *nnz, double *bvec)
+{
+    if (check_grid(imax, jmax)) return -1;
+    const int64_t N = (int64_t)imax * jmax;
+    if (r0 < 0 || r1 > 2 * N || r0 > r1) {
+        set_error("assemble_coo: row range outside [0, 2*imax*jmax)");
+        return -1;
+    }
+    std::vector<uint8_t> fl;
+    std::vector<int> pp, po;
+    int npsi = 0;
+    outflow_flags(imax, metrics[6] + (N - imax), metrics[7] + (N - imax), ff_bc, ca, sa, fl, pp, po, npsi);
+    const int64_t cnt = count_rows(imax, jmax, r0, r1, fl);
+    if (cnt > maxnnz) {
+        set_error("assemble_coo: " + std::to_string(cnt) + " entries exceed maxnnz=" +
+                  std::to_string(maxnnz));
+        return -1;
+    }
+    AsmArgs a;
+    a.imax = imax; a.jmax = jmax; a.ff_bc = ff_bc;
+    a.re = re; a.invdtau = invdtau; a.inv2dt = inv2dt; a.urot = urot;
+    a.dksi = dksi; a.deta = deta; a.ca = ca; a.sa = sa;
+    DBuf f, dfl, dpp, dpo, dci, dcj, dcv, db;
+    if (upload_fields(f, N, metrics, psi, ome, omeold, omeoldold, a)) return -1;
+    V2D_ALLOC(dfl, imax);
+    V2D_ALLOC(dpp, sizeof(int) * imax);
+    V2D_ALLOC(dpo, sizeof(int) * imax);
+    V2D_ALLOC(dci, sizeof(int) * cnt);
+    V2D_ALLOC(dcj, sizeof(int) * cnt);
+    V2D_ALLOC(dcv, sizeof(double) * cnt);
+    V2D_ALLOC(db, sizeof(double) * (r1 - r0));
+    V2D_HIP(hipMemcpy(dfl.p, fl.data(), imax, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(dpp.p, pp.data(), sizeof(int) * imax, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(dpo.p, po.data(), sizeof(int) * imax, hipMemcpyHostToDevice));
+    if (r1 > r0) {
+        hipLaunchKernelGGL(k_assemble_coo, dim3(grid_for(r1 - r0)), dim3(256), 0, nullptr, a,
+                           dfl.as<uint8_t>(), dpp.as<int>(), dpo.as<int>(), npsi, (int64_t)r0,
+                           (int64_t)r1, dci.as<int>(), dcj.as<int>(), dcv.as<double>(),
+                           db.as<double>());
+        V2D_LAUNCH();
+    }
+    V2D_HIP(hipMemcpy(coo_i, dci.p, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(coo_j, dcj.p, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(coo_v, dcv.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(bvec, db.p, sizeof(double) * (r1 - r0), hipMemcpyDeviceToHost));
+    *nnz = (int)cnt;
+    return 0;
+}
+
+int v2d_assemble_coo_local(int gxm, int jmax, int il0, int il1, double re, double invdtau,
+                           double inv2dt, double urot, double dksi, double deta,
+                           const double *const *metrics, const double *psi, const double *ome,
+                           const double *omeold, const double *omeoldold, int r0, int nloc,
+                           const int *lg, int maxnnz, int ff_bc, double ca, double sa,
+                           int *coo_i, int *coo_j, double *coo_v, int *nnz, double *bvec)
+{
+    if (gxm < 3 || jmax < 4 || il0 < 2 || il1 > gxm - 1 || il0 > il1) {
+        set_error("assemble_coo_local: owned columns must satisfy 2 <= il0 <= il1 <= gxm-1");
+        return -1;
+    }
+    const int64_t N = (int64_t)gxm * jmax;
+    const int ow = il1 - il0 + 1;
+    const int64_t nown = (int64_t)ow * jmax;
+    // entry offsets per owned node (j outer, i inner; psi row then omega row)
+    std::vector<int64_t> off(nown + 1, 0);
+    const double *ex = metrics[6], *ey = metrics[7];
+    for (int64_t t = 0; t < nown; ++t) {
+        const int j = (int)(t / ow), i = il0 - 1 + (int)(t % ow);
+        uint8_t f = 0;
+        if (j == jmax - 1) {
+            const int64_t k = (int64_t)gxm * j + i;
+            const bool out = (ca * ex[k] + sa * ey[k]) > 0.0;
+            if (ff_bc >= 1 && out) f |= kOutOme;
+            if (ff_bc >= 2 && out) f |= kOutPsi;
+        }
+        off[t + 1] = off[t] + row_count(jmax, j, 0, f) + row_count(jmax, j, 1, f);
+    }
+    const int64_t cnt = off[nown];
+    if (cnt > maxnnz) {
+        set_error("assemble_coo_local: " + std::to_string(cnt) + " entries exceed maxnnz=" +
+                  std::to_string(maxnnz));
+        return -1;
+    }
+    AsmArgs a;
+    a.imax = gxm; a.jmax = jmax; a.ff_bc = ff_bc;
+    a.re = re; a.invdtau = invdtau; a.inv2dt = inv2dt; a.urot = urot;
+    a.dksi = dksi; a.deta = deta; a.ca = ca; a.sa = sa;
+    DBuf f, dlg, doff, dci, dcj, dcv, db;
+    if (upload_fields(f, N, metrics, psi, ome, omeold, omeoldold, a)) return -1;
+    V2D_ALLOC(dlg, sizeof(int) * 2 * N);
+    V2D_ALLOC(doff, sizeof(int64_t) * (nown + 1));
+    V2D_ALLOC(dci, sizeof(int) * cnt);
+    V2D_ALLOC(dcj, sizeof(int) * cnt);
+    V2D_ALLOC(dcv, sizeof(double) * cnt);
+    V2D_ALLOC(db, sizeof(double) * nloc);
+    V2D_HIP(hipMemcpy(dlg.p, lg, sizeof(int) * 2 * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(doff.p, off.data(), sizeof(int64_t) * (nown + 1), hipMemcpyHostToDevice));
+    V2D_HIP(hipMemset(db.p, 0, sizeof(double) * std::max(nloc, 1)));
+    hipLaunchKernelGGL(k_assemble_coo_local, dim3(grid_for(nown)), dim3(256), 0, nullptr, a, il0 - 1,
+                       ow, dlg.as<int>(), doff.as<int64_t>(), r0, nloc, dci.as<int>(), dcj.as<int>(),
+                       dcv.as<double>(), db.as<double>());
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpy(coo_i, dci.p, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(coo_j, dcj.p, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(coo_v, dcv.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(bvec, db.p, sizeof(double) * nloc, hipMemcpyDeviceToHost));
+    *nnz = (int)cnt;
+    return 0;
+}
+
+int v2d_force_coeffs(int imax, int jmax, double re, double dksi, double deta,
+                     const double *beta, const double *gama, const double *detadx,
+                     const double *detady, const double *xphys, const double *yphys,
+                     const double *ome, double ref_length, double x0, double y0, double *out)
+{
+    if (check_grid(imax, jmax)) return -1;
+    const int64_t N = (int64_t)imax * jmax;
+    DBuf dm, dome, dout;
+    V2D_ALLOC(dm, sizeof(double) * 10 * N);
+    V2D_ALLOC(dome, sizeof(double) * N);
+    V2D_ALLOC(dout, sizeof(double) * 10);
+    const double *planes[10] = {nullptr, nullptr, beta, gama, nullptr, nullptr, detadx, detady, xphys, yphys};
+    for (int q = 0; q < 10; ++q)
+        if (planes[q])
+            V2D_HIP(hipMemcpy(dm.as<double>() + q * N, planes[q], sizeof(double) * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(dome.p, ome, sizeof(double) * N, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(1024), 0, nullptr, imax, re, dksi, deta, dm.as<double>(),
+                       N, dome.as<double>(), ref_length, x0, y0, dout.as<double>());
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpy(out, dout.p, sizeof(double) * 10, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ------------------------------------------------------------ context -----
+int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const double *yg,
+                   int ff_bc, double ca, double sa, int precision)
+{
+    *out = nullptr;
+    if (check_grid(imax, jmax)) return -1;
+    if (precision != V2D_FP64 && precision != V2D_MIXED) {
+        set_error("precision must be V2D_FP64 (0) or V2D_MIXED (1)");
+        return -1;
+    }
+    auto h = std::make_unique<v2d_ctx>();
+    h->c = std::make_unique<Ctx>();
+    Ctx *c = h->c.get();
+    c->imax = imax; c->jmax = jmax; c->prec = precision;
+    c->N = (int64_t)imax * jmax;
+    c->dksi = 1.0 / (imax - 1);
+    c->deta = 1.0 / (jmax - 1);
+    const int64_t N = c->N;
+    V2D_HIP(hipStreamCreate(&c->s));
+    for (auto &e : c->ev) V2D_HIP(hipEventCreate(&e));
+    int rc = 0;
+    c->x = c->alloc<double>(N, rc); c->y = c->alloc<double>(N, rc);
+    c->m = c->alloc<double>(10 * N, rc);
+    c->psi = c->alloc<double>(N, rc); c->ome = c->alloc<double>(N, rc);
+    c->omeold = c->alloc<double>(N, rc); c->omeoldold = c->alloc<double>(N, rc);
+    c->rhs = c->alloc<double>(2 * N, rc); c->delta = c->alloc<double>(2 * N, rc);
+    c->red = c->alloc<double>(16, rc);
+    c->partial = c->alloc<double>(1024 * 64, rc);
+    c->hbuf = c->alloc<double>(128, rc);
+    c->flags = c->alloc<uint8_t>(imax, rc);
+    c->pre_psi = c->alloc<int>(imax, rc); c->pre_ome = c->alloc<int>(imax, rc);
+    if (rc) return -1;
+    V2D_HIP(hipMemcpy(c->x, xg, sizeof(double) * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(c->y, yg, sizeof(double) * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemset(c->psi, 0, sizeof(double) * N));
+    V2D_HIP(hipMemset(c->ome, 0, sizeof(double) * N));
+    V2D_HIP(hipMemset(c->omeold, 0, sizeof(double) * N));
+    V2D_HIP(hipMemset(c->omeoldold, 0, sizeof(double) * N));
+    if (v2d_compute_metrics_dev(imax, jmax, c->dksi, c->deta, c->x, c->y, c->m, c->s)) return -1;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    if (ctx_bc(c, ff_bc, ca, sa)) return -1;
+    const size_t tsz = precision == V2D_FP64 ? sizeof(double) : sizeof(float);
+    c->val = c->alloc<unsigned char>((int64_t)kSlots * N * tsz, rc);
+    const int64_t nm = c->pat.nmax;
+    if (precision == V2D_FP64) {
+        c->wd.sinv = c->alloc<double>(c->pat.sinv_elems, rc);
+        c->wd.tmat = c->alloc<double>(nm * nm, rc);
+        c->wd.colk = c->alloc<double>(nm * 64, rc);
+        c->wd.dinv = c->alloc<double>(64 * 64, rc);
+        c->wd.z = c->alloc<double>(2 * N, rc);
+    } else {
+        c->wf.sinv = c->alloc<float>(c->pat.sinv_elems, rc);
+        c->wf.tmat = c->alloc<float>(nm * nm, rc);
+        c->wf.colk = c->alloc<float>(nm * 64, rc);
+        c->wf.dinv = c->alloc<float>(64 * 64, rc);
+        c->wf.z = c->alloc<float>(2 * N, rc);
+        c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);
+        c->gZ = c->alloc<float>((int64_t)c->gm_cap * 2 * N, rc);
+        c->gb = c->alloc<float>(2 * N, rc);
+        c->gx = c->alloc<float>(2 * N, rc);
+        c->gxd = c->alloc<double>(2 * N, rc);
+    }
+    if (rc) return -1;
+    *out = h.release();
+    return 0;
+}
+
+int v2d_ctx_destroy(v2d_ctx *ctx)
+{
+    delete ctx;
+    return 0;
+}
+
+int v2d_ctx_device_bytes(v2d_ctx *ctx, int64_t *bytes)
+{
+    *bytes = ctx->c->bytes;
+    return 0;
+}
+
+int v2d_ctx_metrics(v2d_ctx *ctx, double *out)
+{
+    Ctx *c = ctx->c.get();
+    V2D_HIP(hipStreamSynchronize(c->s));
+    V2D_HIP(hipMemcpy(out, c->m, sizeof(double) * 10 * c->N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int v2d_ctx_set_state(v2d_ctx *ctx, const double *psi, const double *ome, const double *omeold,
+                      const double *omeoldold)
+{
+    Ctx *c = ctx->c.get();
+    const double *src[4] = {psi, ome, omeold, omeoldold};
+    double *dst[4] = {c->psi, c->ome, c->omeold, c->omeoldold};
+    V2D_HIP(hipStreamSynchronize(c->s));
+    for (int q = 0; q < 4; ++q)
+        if (src[q]) V2D_HIP(hipMemcpy(dst[q], src[q], sizeof(double) * c->N, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int v2d_ctx_get_state(v2d_ctx *ctx, double *psi, double *ome, double *omeold, double *omeoldold)
+{
+    Ctx *c = ctx->c.get();
+    double *dst[4] = {psi, ome, omeold, omeoldold};
+    const double *src[4] = {c->psi, c->ome, c->omeold, c->omeoldold};
+    V2D_HIP(hipStreamSynchronize(c->s));
+    for (int q = 0; q < 4; ++q)
+        if (dst[q]) V2D_HIP(hipMemcpy(dst[q], src[q], sizeof(double) * c->N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int v2d_ctx_init_state(v2d_ctx *ctx)
+{
+    Ctx *c = ctx->c.get();
+    const int64_t N = c->N;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    std::vector<double> psi(N, 0.0), z(N, 0.0);
+    std::vector<double> xl(c->imax), yl(c->imax);
+    V2D_HIP(hipMemcpy(xl.data(), c->m + 8 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(yl.data(), c->m + 9 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->imax; ++i) psi[N - c->imax + i] = c->ca * yl[i] - c->sa * xl[i];
+    return v2d_ctx_set_state(ctx, psi.data(), z.data(), z.data(), z.data());
+}
+
+int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int gmres_max_iter)
+{
+    Ctx *c = ctx->c.get();
+    if (gmres_max_iter < 1 || gmres_max_iter > c->gm_cap) {
+        set_error("gmres_max_iter must be in [1, " + std::to_string(c->gm_cap) + "]");
+        return -1;
+    }
+    if (!(gmres_rtol > 0.0) || refactor_every < 0) {
+        set_error("gmres_rtol must be > 0 and refactor_every >= 0");
+        return -1;
+    }
+    c->refactor_every = refactor_every;
+    c->gm_rtol = gmres_rtol;
+    c->gm_max = gmres_max_iter;
+    return 0;
+}
+
+static int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double urot)
+{
+    const AsmArgs a = c->args(re, invdtau, inv2dt, urot);
+    c->last_re = re; c->last_invdtau = invdtau; c->last_inv2dt = inv2dt; c->last_urot = urot;
+    if (c->prec == V2D_FP64)
+        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_for(c->N)), dim3(256), 0, c->s, a, c->flags,
+                           static_cast<double *>(c->val), c->rhs);
+    else
+        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_for(c->N)), dim3(256), 0, c->s, a, c->flags,
+                           static_cast<float *>(c->val), c->rhs);
+    V2D_LAUNCH();
+    return 0;
+}
+
+int v2d_ctx_assemble(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot)
+{
+    return ctx_assemble(ctx->c.get(), re, invdtau, inv2dt, urot);
+}
+
+static int dnorm2(Ctx *c, const double *v, int64_t n, double *out)
+{
+    const unsigned nb = std::min<unsigned>(1024, grid_for(n));
+    hipLaunchKernelGGL(k_multidot<double>, dim3(nb), dim3(256), 0, c->s, n, 1, v, n, v, c->partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, 1, c->partial, c->red);
+    V2D_LAUNCH();
+    double s2 = 0.0;
+    V2D_HIP(hipMemcpyAsync(&s2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
+    *out = std::sqrt(s2);
+    return 0;
+}
+
+int v2d_ctx_rhs_norm(v2d_ctx *ctx, double *out)
+{
+    Ctx *c = ctx->c.get();
+    return dnorm2(c, c->rhs, 2 * c->N, out);
+}
+
+int v2d_ctx_nnz(v2d_ctx *ctx, int64_t *nnz)
+{
+    *nnz = ctx->c->nnz;
+    return 0;
+}
+
+int v2d_ctx_export_coo(v2d_ctx *ctx, int *coo_i, int *coo_j, double *coo_v, double *bvec)
+{
+    Ctx *c = ctx->c.get();
+    const int64_t N = c->N, nnz = c->nnz;
+    DBuf dci, dcj, dcv;
+    V2D_ALLOC(dci, sizeof(int) * nnz);
+    V2D_ALLOC(dcj, sizeof(int) * nnz);
+    V2D_ALLOC(dcv, sizeof(double) * nnz);
+    if (c->prec == V2D_FP64)
+        hipLaunchKernelGGL(k_slots_to_coo<double>, dim3(grid_for(2 * N)), dim3(256), 0, c->s, c->imax,
+                           c->jmax, c->flags, c->pre_psi, c->pre_ome, c->npsi_out,
+                           static_cast<const double *>(c->val), dci.as<int>(), dcj.as<int>(), dcv.as<double>());
+    else
+        hipLaunchKernelGGL(k_slots_to_coo<float>, dim3(grid_for(2 * N)), dim3(256), 0, c->s, c->imax,
+                           c->jmax, c->flags, c->pre_psi, c->pre_ome, c->npsi_out,
+                           static_cast<const float *>(c->val), dci.as<int>(), dcj.as<int>(), dcv.as<double>());
+    V2D_LAUNCH();
+    V2D_HIP(hipStreamSynchronize(c->s));
+    V2D_HIP(hipMemcpy(coo_i, dci.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(coo_j, dcj.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(coo_v, dcv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    if (bvec) V2D_HIP(hipMemcpy(bvec, c->rhs, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int v2d_ctx_export_csr(v2d_ctx *ctx, int64_t *indptr, int *indices, double *data)
+{
+    Ctx *c = ctx->c.get();
+    const int64_t N = c->N, nnz = c->nnz;
+    DBuf dp, di, dd;
+    V2D_ALLOC(dp, sizeof(int64_t) * (2 * N + 1));
+    V2D_ALLOC(di, sizeof(int) * nnz);
+    V2D_ALLOC(dd, sizeof(double) * nnz);
+    if (c->prec == V2D_FP64)
+        hipLaunchKernelGGL(k_slots_to_csr<double>, dim3(grid_for(2 * N + 1)), dim3(256), 0, c->s, c->imax,
+                           c->jmax, c->flags, c->pre_psi, c->pre_ome, c->npsi_out,
+                           static_cast<const double *>(c->val), dp.as<int64_t>(), di.as<int>(), dd.as<double>());
+    else
+        hipLaunchKernelGGL(k_slots_to_csr<float>, dim3(grid_for(2 * N + 1)), dim3(256), 0, c->s, c->imax,
+                           c->jmax, c->flags, c->pre_psi, c->pre_ome, c->npsi_out,
+                           static_cast<const float *>(c->val), dp.as<int64_t>(), di.as<int>(), dd.as<double>());
+    V2D_LAUNCH();
+    V2D_HIP(hipStreamSynchronize(c->s));
+    V2D_HIP(hipMemcpy(indptr, dp.p, sizeof(int64_t) * (2 * N + 1), hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(indices, di.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(data, dd.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int v2d_ctx_spmv(v2d_ctx *ctx, const double *x, double *y)
+{
+    Ctx *c = ctx->c.get();
+    const int64_t n = 2 * c->N;
+    DBuf dx, dy;
+    V2D_ALLOC(dx, sizeof(double) * n);
+    V2D_ALLOC(dy, sizeof(double) * n);
+    V2D_HIP(hipStreamSynchronize(c->s));
+    V2D_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    // FP64 vectors through the FP64-widening residual kernel: y = 0 - A x
+    DBuf dz;
+    V2D_ALLOC(dz, sizeof(double) * n);
+    V2D_HIP(hipMemset(dz.p, 0, sizeof(double) * n));
+    if (c->prec == V2D_FP64)
+        hipLaunchKernelGGL(k_residual<double>, dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax, c->jmax,
+                           c->flags, static_cast<const double *>(c->val), dx.as<double>(), dz.as<double>(),
+                           dy.as<double>());
+    else
+        hipLaunchKernelGGL(k_residual<float>, dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax, c->jmax,
+                           c->flags, static_cast<const float *>(c->val), dx.as<double>(), dz.as<double>(),
+                           dy.as<double>());
+    V2D_LAUNCH();
+    V2D_HIP(hipStreamSynchronize(c->s));
+    std::vector<double> t(n);
+    V2D_HIP(hipMemcpy(t.data(), dy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < n; ++q) y[q] = -t[q];
+    return 0;
+}
+
+static int ctx_factor(Ctx *c)
+{
+    int rc;
+    if (c->prec == V2D_FP64)
+        rc = direct_factor<double>(c->pat, c->wd, static_cast<const double *>(c->val), c->s);
+    else
+        rc = direct_factor<float>(c->pat, c->wf, static_cast<const float *>(c->val), c->s);
+    if (rc) return rc;
+    c->factored = true;
+    c->since_factor = 0;
+    ++c->n_fac;
+    return 0;
+}
+
+int v2d_ctx_factor(v2d_ctx *ctx)
+{
+    Ctx *c = ctx->c.get();
+    if (ctx_factor(c)) return -1;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
+{
+    Ctx *c = ctx->c.get();
+    if (!c->factored) {
+        set_error("solve: no factorisation (call v2d_ctx_factor first)");
+        return -1;
+    }
+    const int64_t n = 2 * c->N;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    if (c->prec == V2D_FP64) {
+        DBuf db, dx;
+        V2D_ALLOC(db, sizeof(double) * n);
+        V2D_ALLOC(dx, sizeof(double) * n);
+        V2D_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val), db.as<double>(),
+                                 dx.as<double>(), c->s))
+            return -1;
+        V2D_HIP(hipStreamSynchronize(c->s));
+        V2D_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<float> bf(n), xf(n);
+    for (int64_t q = 0; q < n; ++q) bf[q] = (float)b[q];
+    DBuf db, dx;
+    V2D_ALLOC(db, sizeof(float) * n);
+    V2D_ALLOC(dx, sizeof(float) * n);
+    V2D_HIP(hipMemcpy(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    if (direct_solve<float>(c->pat, c->wf, static_cast<const float *>(c->val), db.as<float>(),
+                            dx.as<float>(), c->s))
+        return -1;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    V2D_HIP(hipMemcpy(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < n; ++q) x[q] = xf[q];
+    return 0;
+}
+
+int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot,
+                        double *res, double *rnorm, int *lin_iters)
+{
+    Ctx *c = ctx->c.get();
+    const int64_t N = c->N, n = 2 * N;
+    V2D_HIP(hipEventRecord(c->ev[0], c->s));
+    if (ctx_assemble(c, re, invdtau, inv2dt, urot)) return -1;
+    V2D_HIP(hipEventRecord(c->ev[1], c->s));
+    double bn = 0.0;
+    if (dnorm2(c, c->rhs, n, &bn)) return -1;
+    if (!std::isfinite(bn)) {
+        set_error("Newton step: non-finite residual norm (degenerate mesh or diverged state)");
+        return -1;
+    }
+    *rnorm = bn;
+    int its = 0;
+    float fac_ms = 0.f, sol_ms = 0.f;
+    if (c->prec == V2D_FP64) {
+        if (ctx_factor(c)) return -1;
+        V2D_HIP(hipEventRecord(c->ev[2], c->s));
+        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val), c->rhs, c->delta, c->s))
+            return -1;
+    } else {
+        const bool need = !c->factored || (c->refactor_every > 0 && c->since_factor >= c->refactor_every);
+        if (need && ctx_factor(c)) return -1;
+        V2D_HIP(hipEventRecord(c->ev[2], c->s));
+        if (bn > 0.0) {
+            // unit-norm FP32 right-hand side
+            V2D_HIP(hipMemcpyAsync(c->red, &bn, sizeof(double), hipMemcpyHostToDevice, c->s));
+            hipLaunchKernelGGL((k_scale_copy<double, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n,
+                               c->rhs, c->red, 1, c->gb);
+            V2D_LAUNCH();
+            double rr = 1.0;
+            if (gmres_f32(c, &its, &rr)) return -1;
+            if (rr > c->gm_rtol && !need) {
+                // stale factorisation: refactor and solve again
+                if (ctx_factor(c)) return -1;
+                int its2 = 0;
+                if (gmres_f32(c, &its2, &rr)) return -1;
+                its += its2;
+            }
+            V2D_HIP(hipMemcpyAsync(c->red, &bn, sizeof(double), hipMemcpyHostToDevice, c->s));
+            hipLaunchKernelGGL((k_scale_copy<double, double>), dim3(grid_for(n)), dim3(256), 0, c->s, n,
+                               c->gxd, c->red, 0, c->delta);
+            V2D_LAUNCH();
+        } else {
+            V2D_HIP(hipMemsetAsync(c->delta, 0, sizeof(double) * n, c->s));
+        }
+        ++c->since_factor;
+    }
+    V2D_HIP(hipEventRecord(c->ev[3], c->s));
+    hipLaunchKernelGGL(k_apply_update, dim3(grid_for(N)), dim3(256), 0, c->s, N, c->delta, c->psi, c->ome);
+    V2D_LAUNCH();
+    double dn = 0.0;
+    if (dnorm2(c, c->delta, n, &dn)) return -1;
+    float a_ms = 0.f, t_ms = 0.f;
+    V2D_HIP(hipEventElapsedTime(&a_ms, c->ev[0], c->ev[1]));
+    V2D_HIP(hipEventElapsedTime(&fac_ms, c->ev[1], c->ev[2]));
+    V2D_HIP(hipEventElapsedTime(&sol_ms, c->ev[2], c->ev[3]));
+    V2D_HIP(hipEventElapsedTime(&t_ms, c->ev[0], c->ev[3]));
+    c->t_asm += a_ms; c->t_fac += fac_ms; c->t_sol += sol_ms; c->t_tot += t_ms;
+    ++c->n_steps;
+    c->n_lin += its;
+    *res = dn / (2.0 * (double)N);
+    *lin_iters = its;
+    if (!std::isfinite(*res)) {
+        set_error("Newton step: non-finite update");
+        return -1;
+    }
+    return 0;
+}
+
+int v2d_ctx_bdf2_shift(v2d_ctx *ctx)
+{
+    Ctx *c = ctx->c.get();
+    V2D_HIP(hipMemcpyAsync(c->omeoldold, c->omeold, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->s));
+    V2D_HIP(hipMemcpyAsync(c->omeold, c->ome, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int v2d_ctx_forces(v2d_ctx *ctx, double re, double ref_length, double x0, double y0, double *out)
+{
+    Ctx *c = ctx->c.get();
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(1024), 0, c->s, c->imax, re, c->dksi, c->deta, c->m, c->N,
+                       c->ome, ref_length, x0, y0, c->hbuf);
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpyAsync(out, c->hbuf, sizeof(double) * 10, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int v2d_ctx_timers(v2d_ctx *ctx, double *ms4, int64_t *counts3, int reset)
+{
+    Ctx *c = ctx->c.get();
+    ms4[0] = c->t_asm; ms4[1] = c->t_fac; ms4[2] = c->t_sol; ms4[3] = c->t_tot;
+    counts3[0] = c->n_steps; counts3[1] = c->n_fac; counts3[2] = c->n_lin;
+    if (reset) {
+        c->t_asm = c->t_fac = c->t_sol = c->t_tot = 0;
+        c->n_steps = c->n_fac = c->n_lin = 0;
+    }
+    return 0;
+}
+
+int v2d_ctx_time_assembly(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot,
+                          int launches, double *ms_per_launch)
+{
+    Ctx *c = ctx->c.get();
+    if (launches < 1) { set_error("launches must be >= 1"); return -1; }
+    if (ctx_assemble(c, re, invdtau, inv2dt, urot)) return -1;   // warm-up
+    V2D_HIP(hipEventRecord(c->ev[0], c->s));
+    for (int q = 0; q < launches; ++q)
+        if (ctx_assemble(c, re, invdtau, inv2dt, urot)) return -1;
+    V2D_HIP(hipEventRecord(c->ev[1], c->s));
+    V2D_HIP(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    V2D_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    *ms_per_launch = ms / launches;
+    return 0;
+}
+
+int v2d_ctx_time_spmv(v2d_ctx *ctx, int launches, double *ms_per_launch)
+{
+    Ctx *c = ctx->c.get();
+    if (launches < 1) { set_error("launches must be >= 1"); return -1; }
+    const int64_t n = 2 * c->N;
+    DBuf dx, dy;
+    const size_t vs = c->prec == V2D_FP64 ? sizeof(double) : sizeof(float);
+    V2D_ALLOC(dx, vs * n);
+    V2D_ALLOC(dy, vs * n);
+    auto launch = [&]() {
+        if (c->prec == V2D_FP64)
+            hipLaunchKernelGGL((k_spmv<double, double>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
+                               c->jmax, c->flags, static_cast<const double *>(c->val), dx.as<double>(),
+                               dy.as<double>());
+        else
+            hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
+                               c->jmax, c->flags, static_cast<const float *>(c->val), dx.as<float>(),
+                               dy.as<float>());
+    };
+    if (c->prec == V2D_FP64)
+        hipLaunchKernelGGL((k_fill_one<double>), dim3(grid_for(n)), dim3(256), 0, c->s, n, dx.as<double>());
+    else
+        hipLaunchKernelGGL((k_fill_one<float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, dx.as<float>());
+    launch();
+    V2D_LAUNCH();
+    V2D_HIP(hipEventRecord(c->ev[0], c->s));
+    for (int q = 0; q < launches; ++q) launch();
+    V2D_HIP(hipEventRecord(c->ev[1], c->s));
+    V2D_LAUNCH();
+    V2D_HIP(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    V2D_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    *ms_per_launch = ms / launches;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// v2d_internal.h -- declarations shared by the libvorti2d translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "v2d_common.h"
+
+namespace v2d {
+
+// Arguments of the residual / Jacobian builders: scalars of assemble_coo
+// (vorti2d_core.f90:219-234) plus device pointers to the 10 metric planes and
+// the four state planes.  m[] order: jac, alfa, beta, gama, pmet, qmet,
+// detadx, detady, xphys, yphys.
+struct AsmArgs {
+    int imax, jmax, ff_bc;
+    double re, invdtau, inv2dt, urot, dksi, deta, ca, sa;
+    const double *m[10];
+    const double *psi, *ome, *omeold, *omeoldold;
+};
+
+// ------------------------------------------------------------ kernels ------
+__global__ void k_metrics(int imax, int jmax, double dksi, double deta, const double *x,
+                          const double *y, double *out);
+__global__ void k_metrics_local(int gxm, int jmax, int il0, int il1, double dksi, double deta,
+                                const double *x, const double *y, double *out);
+template <typename T>
+__global__ void k_assemble(AsmArgs a, const uint8_t *flags, T *val, double *rhs);
+__global__ void k_assemble_coo(AsmArgs a, const uint8_t *flags, const int *pre_psi,
+                               const int *pre_ome, int npsi_out, int64_t r0, int64_t r1, int *ci,
+                               int *cj, double *cv, double *bvec);
+__global__ void k_assemble_coo_local(AsmArgs a, int il0, int ow, const int *lg,
+                                     const int64_t *node_off, int r0, int nloc, int *ci, int *cj,
+                                     double *cv, double *bvec);
+template <typename T>
+__global__ void k_slots_to_coo(int imax, int jmax, const uint8_t *flags, const int *pre_psi,
+                               const int *pre_ome, int npsi_out, const T *val, int *ci, int *cj,
+                               double *cv);
+template <typename T>
+__global__ void k_slots_to_csr(int imax, int jmax, const uint8_t *flags, const int *pre_psi,
+                               const int *pre_ome, int npsi_out, const T *val, int64_t *indptr,
+                               int *indices, double *data);
+template <typename T, typename V>
+__global__ void k_spmv(int imax, int jmax, const uint8_t *flags, const T *val, const V *x, V *y);
+template <typename V>
+__global__ void k_multidot(int64_t n, int m, const V *Vb, int64_t ld, const V *w, double *partial);
+__global__ void k_reduce_partials(int nb, int m, const double *partial, double *out);
+template <typename V>
+__global__ void k_multiaxpy(int64_t n, int m, const V *Vb, int64_t ld, const double *h, V *w);
+template <typename A, typename B>
+__global__ void k_scale_copy(int64_t n, const A *x, const double *s, int inv, B *y);
+template <typename Z, typename X>
+__global__ void k_update_sol(int64_t n, int m, const Z *Zb, int64_t ld, const double *y,
+                             double scale, X *x);
+template <typename T>
+__global__ void k_residual(int imax, int jmax, const uint8_t *flags, const T *val, const double *x,
+                           const double *b, double *r);
+__global__ void k_fill_zero_d(int64_t n, double *p);
+template <typename T>
+__global__ void k_fill_one(int64_t n, T *p);
+__global__ void k_apply_update(int64_t N, const double *d, double *psi, double *ome);
+__global__ void k_forces(int imax, double re, double dksi, double deta, const double *m, int64_t N,
+                         const double *ome, double ref_length, double x0, double y0, double *out);
+
+// ------------------------------------------------------- error state -------
+void set_error(const std::string &msg);
+#define V2D_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t _e = (call);                                                             \
+        if (_e != hipSuccess) {                                                             \
+            ::v2d::set_error(std::string(#call) + ": " + hipGetErrorString(_e));            \
+            return -1;                                                                      \
+        }                                                                                   \
+    } while (0)
+#define V2D_LAUNCH()                                                                        \
+    do {                                                                                    \
+        hipError_t _e = hipGetLastError();                                                  \
+        if (_e != hipSuccess) {                                                             \
+            ::v2d::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));      \
+            return -1;                                                                      \
+        }                                                                                   \
+    } while (0)
+
+inline unsigned grid_for(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------- block-tridiagonal direct solver
+// Exact LU of the Newton matrix in radial-level blocks: level l holds all
+// unknowns of its j rows (psi then omega, i fastest).  The factorisation keeps
+// the explicit inverse of every Schur complement S_l = D_l - L_l S_{l-1}^-1 U_{l-1}.
+struct DirectPattern {
+    Levels L;
+    int64_t N = 0;
+    std::vector<int> n;            // level sizes
+    std::vector<int64_t> voff;     // level offsets in a level-ordered 2N vector
+    std::vector<int64_t> soff;     // offsets of the dense inverses (elements)
+    int64_t sinv_elems = 0;        // total dense storage
+    int nmax = 0;
+    // device CSR (rows / columns indexed in level order), entries (local, vidx)
+    int *d_Dptr = nullptr, *d_Lptr = nullptr, *d_Uptr = nullptr, *d_UCptr = nullptr;
+    int2 *d_D = nullptr, *d_Lr = nullptr, *d_Ur = nullptr, *d_Uc = nullptr;
+};
+
+int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags_host);
+void direct_free_pattern(DirectPattern &P);
+
+template <typename T>
+struct DirectWork {
+    T *sinv = nullptr;   // all dense inverses
+    T *tmat = nullptr;   // Schur scratch (nmax x nmax)
+    T *colk = nullptr;   // Gauss-Jordan column panel (nmax x 64)
+    T *dinv = nullptr;   // 64 x 64 pivot-block inverse
+    T *z = nullptr;      // level-ordered 2N work vector
+};
+
+template <typename T>
+int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
+template <typename T>
+int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
+                 hipStream_t s);
+// Gauss-Jordan inverse of one dense n x n row-major matrix in place (exported
+// for tests of the dense kernel)
+template <typename T>
+int gj_invert(T *A, int n, T *colk, T *dinv, hipStream_t s);
+
+}  // namespace v2d

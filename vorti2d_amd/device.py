@@ -1,0 +1,163 @@
+"""Python handle on a libvorti2d device context (v2d_ctx_* in include/vorti2d.h).
+
+The context owns the mesh, metrics, state, Jacobian slot planes and the block-LU
+factorisation in HBM; methods move only scalars unless they are explicit
+uploads / downloads.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, f64c, lib, ptr
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+class DeviceContext:
+    def __init__(self, xg, yg, ff_bc: int = 0, ca: float = 1.0, sa: float = 0.0,
+                 precision: str | int = "fp64"):
+        xg = np.asarray(xg, dtype=np.float64)
+        yg = np.asarray(yg, dtype=np.float64)
+        if xg.ndim != 2 or xg.shape != yg.shape:
+            raise ValueError("xg, yg must be 2-D arrays of equal shape (imax, jmax)")
+        self.imax, self.jmax = xg.shape
+        self.N = self.imax * self.jmax
+        self.precision = precision if isinstance(precision, int) else PRECISIONS[precision]
+        h = C.c_void_p()
+        # (imax, jmax) arrays in i-fastest memory order
+        xf = np.ascontiguousarray(xg.T)
+        yf = np.ascontiguousarray(yg.T)
+        check(lib.v2d_ctx_create(C.byref(h), self.imax, self.jmax, ptr(xf), ptr(yf), int(ff_bc),
+                                 float(ca), float(sa), self.precision))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.v2d_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ state ----
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        check(lib.v2d_ctx_device_bytes(self._h, C.byref(b)))
+        return int(b.value)
+
+    def metrics(self):
+        out = np.empty((10, self.N))
+        check(lib.v2d_ctx_metrics(self._h, ptr(out)))
+        return tuple(np.ascontiguousarray(out[q]) for q in range(10))
+
+    def set_state(self, psi=None, ome=None, omeold=None, omeoldold=None):
+        arrs = [None if a is None else f64c(a) for a in (psi, ome, omeold, omeoldold)]
+        for a in arrs:
+            if a is not None and a.size != self.N:
+                raise ValueError("state arrays must have length imax*jmax")
+        check(lib.v2d_ctx_set_state(self._h, *[ptr(a) for a in arrs]))
+
+    def get_state(self, which=("psi", "ome", "omeold", "omeoldold")):
+        names = ("psi", "ome", "omeold", "omeoldold")
+        outs = {n: (np.empty(self.N) if n in which else None) for n in names}
+        check(lib.v2d_ctx_get_state(self._h, *[ptr(outs[n]) for n in names]))
+        return tuple(outs[n] for n in which)
+
+    def init_state(self):
+        check(lib.v2d_ctx_init_state(self._h))
+
+    def set_solver(self, refactor_every: int = 1, gmres_rtol: float = 1e-6,
+                   gmres_max_iter: int = 20):
+        check(lib.v2d_ctx_set_solver(self._h, int(refactor_every), float(gmres_rtol),
+                                     int(gmres_max_iter)))
+
+    # ----------------------------------------------------------- system ----
+    def assemble(self, re, invdtau=0.0, inv2dt=0.0, urot=0.0):
+        check(lib.v2d_ctx_assemble(self._h, float(re), float(invdtau), float(inv2dt), float(urot)))
+
+    def rhs_norm(self) -> float:
+        v = C.c_double()
+        check(lib.v2d_ctx_rhs_norm(self._h, C.byref(v)))
+        return float(v.value)
+
+    def nnz(self) -> int:
+        v = C.c_int64()
+        check(lib.v2d_ctx_nnz(self._h, C.byref(v)))
+        return int(v.value)
+
+    def export_coo(self):
+        nnz = self.nnz()
+        ci = np.empty(nnz, np.int32)
+        cj = np.empty(nnz, np.int32)
+        cv = np.empty(nnz, np.float64)
+        b = np.empty(2 * self.N, np.float64)
+        check(lib.v2d_ctx_export_coo(self._h, ptr(ci), ptr(cj), ptr(cv), ptr(b)))
+        return ci, cj, cv, b
+
+    def export_csr(self):
+        nnz = self.nnz()
+        ip = np.empty(2 * self.N + 1, np.int64)
+        ix = np.empty(nnz, np.int32)
+        dv = np.empty(nnz, np.float64)
+        check(lib.v2d_ctx_export_csr(self._h, ptr(ip), ptr(ix), ptr(dv)))
+        return ip, ix, dv
+
+    def spmv(self, x):
+        x = f64c(x)
+        if x.size != 2 * self.N:
+            raise ValueError("x must have length 2*imax*jmax")
+        y = np.empty(2 * self.N)
+        check(lib.v2d_ctx_spmv(self._h, ptr(x), ptr(y)))
+        return y
+
+    def factor(self):
+        check(lib.v2d_ctx_factor(self._h))
+
+    def solve(self, b):
+        b = f64c(b)
+        if b.size != 2 * self.N:
+            raise ValueError("b must have length 2*imax*jmax")
+        x = np.empty(2 * self.N)
+        check(lib.v2d_ctx_solve(self._h, ptr(b), ptr(x)))
+        return x
+
+    # ----------------------------------------------------------- Newton ----
+    def newton_step(self, re, invdtau=0.0, inv2dt=0.0, urot=0.0):
+        """One assemble + solve + update; returns (res, ||b||, lin_iters)."""
+        res, rn, its = C.c_double(), C.c_double(), C.c_int()
+        check(lib.v2d_ctx_newton_step(self._h, float(re), float(invdtau), float(inv2dt),
+                                      float(urot), C.byref(res), C.byref(rn), C.byref(its)))
+        return float(res.value), float(rn.value), int(its.value)
+
+    def bdf2_shift(self):
+        check(lib.v2d_ctx_bdf2_shift(self._h))
+
+    def forces(self, re, ref_length=None, moment_ref=(0.0, 0.0)):
+        out = np.zeros(10)
+        d = -1.0 if ref_length is None else float(ref_length)
+        check(lib.v2d_ctx_forces(self._h, float(re), d, float(moment_ref[0]),
+                                 float(moment_ref[1]), ptr(out)))
+        return out
+
+    def timers(self, reset: bool = False) -> dict:
+        ms = np.zeros(4)
+        cnt = np.zeros(3, np.int64)
+        check(lib.v2d_ctx_timers(self._h, ptr(ms), ptr(cnt), int(reset)))
+        return dict(assemble_ms=ms[0], factor_ms=ms[1], solve_ms=ms[2], step_ms=ms[3],
+                    steps=int(cnt[0]), factorizations=int(cnt[1]), gmres_iters=int(cnt[2]))
+
+    def time_assembly(self, re, invdtau=0.0, inv2dt=0.0, urot=0.0, launches=50) -> float:
+        v = C.c_double()
+        check(lib.v2d_ctx_time_assembly(self._h, float(re), float(invdtau), float(inv2dt),
+                                        float(urot), int(launches), C.byref(v)))
+        return float(v.value)
+
+    def time_spmv(self, launches=50) -> float:
+        v = C.c_double()
+        check(lib.v2d_ctx_time_spmv(self._h, int(launches), C.byref(v)))
+        return float(v.value)
