@@ -61,12 +61,14 @@ def newton_solve(ctx, re, max_iter=30):
     ctx.assemble(re)
     rn = [ctx.rhs_norm()]
     res_hist = []
+    lin = []
     t0 = time.perf_counter()
     it = 0
     while it < max_iter:
-        res, _, _ = ctx.newton_step(re)
+        res, _, li = ctx.newton_step(re)
         it += 1
         res_hist.append(res)
+        lin.append(li)
         ctx.assemble(re)
         rn.append(ctx.rhs_norm())
         if rn[-1] < RTOL_R:
@@ -74,6 +76,7 @@ def newton_solve(ctx, re, max_iter=30):
         if it >= 3 and rn[-1] > 0.5 * rn[-2] and res < 1e-12:
             break   # round-off floor reached
     dt = time.perf_counter() - t0
+    newton_solve.last_lin = lin
     return dt, it, rn, res_hist
 
 
@@ -120,6 +123,10 @@ def main():
     ap.add_argument("--re", type=float, default=40.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--refactor-every", type=int, default=1,
+                    help="mixed: refactor the FP32 block LU every N Newton steps "
+                         "(0 = only when GMRES misses its tolerance)")
+    ap.add_argument("--gmres-max", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -135,7 +142,8 @@ def main():
         t0 = time.perf_counter()
         ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, prec)
         if prec == "mixed":
-            ctx.set_solver(refactor_every=1, gmres_rtol=1e-6, gmres_max_iter=20)
+            ctx.set_solver(refactor_every=a.refactor_every, gmres_rtol=1e-6,
+                           gmres_max_iter=a.gmres_max)
         setup = time.perf_counter() - t0
         log(f"[{prec}] setup {setup:.2f} s, device memory {ctx.device_bytes() / 2**30:.1f} GiB")
         nsteps = a.steps if prec == "mixed" else max(1, a.steps // 2)
@@ -148,7 +156,8 @@ def main():
             dt, it, rn, rh = newton_solve(ctx, a.re)
             times.append(dt)
             log(f"[{prec}] solve {s}: {dt:.3f} s, {it} Newton iterations, ||R|| "
-                + " ".join(f"{x:.2e}" for x in rn))
+                + " ".join(f"{x:.2e}" for x in rn)
+                + f"; GMRES iterations {newton_solve.last_lin}")
         tm = ctx.timers()
         fc = ctx.forces(a.re)
         entry = dict(seconds=statistics.median(times), times=times, newton_iters=it,

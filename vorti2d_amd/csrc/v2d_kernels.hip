@@ -518,17 +518,22 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Block-wide sum returned to every thread (sh: one slot per wavefront + 1).
 __device__ __forceinline__ double block_sum(double v, double *sh)
 {
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) sh[wid] = v;
     __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x < (blockDim.x >> 6)) t = sh[threadIdx.x];
-    if (wid == 0) t = wave_sum(t);
+    if (wid == 0) {
+        double t = (threadIdx.x < (blockDim.x >> 6)) ? sh[threadIdx.x] : 0.0;
+        t = wave_sum(t);
+        if (lane == 0) sh[16] = t;
+    }
     __syncthreads();
-    return t;
+    const double total = sh[16];
+    __syncthreads();
+    return total;
 }
 
 // partial[b*m + c] = sum over the block's share of  V_c . w   (c < m), FP64
@@ -538,7 +543,7 @@ template <typename V>
 __global__ void __launch_bounds__(256) k_multidot(int64_t n, int m, const V *__restrict__ Vb, int64_t ld,
                                                   const V *__restrict__ w, double *__restrict__ partial)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[17];
     constexpr int kMax = 8;
     for (int c0 = 0; c0 < m; c0 += kMax) {
         const int mc = min(kMax, m - c0);
@@ -564,7 +569,7 @@ template __global__ void k_multidot<float>(int64_t, int, const float *, int64_t,
 // out[c] = sum_b partial[b*m + c]
 __global__ void k_reduce_partials(int nb, int m, const double *__restrict__ partial, double *__restrict__ out)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[17];
     for (int c = 0; c < m; ++c) {
         double s = 0.0;
         for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(int64_t)b * m + c];
@@ -693,7 +698,7 @@ __global__ void __launch_bounds__(1024) k_forces(int imax, double re, double dks
                                                  double ref_length, double x0, double y0,
                                                  double *__restrict__ out)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[17];
     const int P = imax - 1;
     const double *beta = m + 2 * N, *gama = m + 3 * N, *ex = m + 6 * N, *ey = m + 7 * N;
     const double *xp = m + 8 * N, *yp = m + 9 * N;
