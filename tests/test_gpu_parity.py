@@ -167,7 +167,8 @@ def test_ctx_export_matches_oracle(v, C, oracle_mod, prec, ff, ca, sa):
     assert abs(ctx.rhs_norm() - np.linalg.norm(ob)) <= 1e-13 * np.linalg.norm(ob)
 
 
-@pytest.mark.parametrize("imax,jmax,kind,ff", [(41, 41, "cyl", 0), (57, 33, "ann", 2),
+@pytest.mark.parametrize("imax,jmax,kind,ff", [(41, 41, "cyl", 0), (57, 33, "ann", 2), (20, 4, "cyl", 0),
+                                               (25, 5, "cyl", 1), (33, 8, "ann", 2),
                                                (181, 181, "ann", 1)])
 def test_direct_solve_fp64(v, C, oracle_mod, imax, jmax, kind, ff):
     say(f"direct fp64 {kind} {imax}x{jmax} ff={ff}")

@@ -529,15 +529,15 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     const int64_t nm = c->pat.nmax;
     if (precision == V2D_FP64) {
         c->wd.sinv = c->alloc<double>(c->pat.sinv_elems, rc);
-        c->wd.tmat = c->alloc<double>(nm * c->pat.ldmax, rc);
-        c->wd.colk = c->alloc<double>(nm * 64, rc);
-        c->wd.dinv = c->alloc<double>(64 * 64, rc);
+        c->wd.tmat = c->alloc<double>(2 * nm * c->pat.ldmax, rc);
+        c->wd.colk = c->alloc<double>(2 * nm * 64, rc);
+        c->wd.dinv = c->alloc<double>(2 * 64 * 64, rc);
         c->wd.z = c->alloc<double>(2 * N, rc);
     } else {
         c->wf.sinv = c->alloc<float>(c->pat.sinv_elems, rc);
-        c->wf.tmat = c->alloc<float>(nm * c->pat.ldmax, rc);
-        c->wf.colk = c->alloc<float>(nm * 64, rc);
-        c->wf.dinv = c->alloc<float>(64 * 64, rc);
+        c->wf.tmat = c->alloc<float>(2 * nm * c->pat.ldmax, rc);
+        c->wf.colk = c->alloc<float>(2 * nm * 64, rc);
+        c->wf.dinv = c->alloc<float>(2 * 64 * 64, rc);
         c->wf.z = c->alloc<float>(2 * N, rc);
         c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);
         c->gZ = c->alloc<float>((int64_t)c->gm_cap * 2 * N, rc);

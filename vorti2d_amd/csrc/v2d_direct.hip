@@ -3,15 +3,16 @@
 // Ordering the unknowns by radial level makes the Newton matrix block
 // tridiagonal: level l (one j row, or two at the wall / far field where the
 // Thom and outflow rows reach j +- 2) couples only to levels l-1 and l+1.  The
-// factorisation is block Gaussian elimination from the wall outward:
-//     S_0 = D_0,   S_l = D_l - L_l S_{l-1}^{-1} U_{l-1},
-// and keeps every S_l^{-1} explicitly, computed in place by blocked
-// Gauss-Jordan without pivoting.  L and U stay sparse (at most a handful of
+// factorisation is block Gaussian elimination from both ends at once (a
+// twisted factorisation), from the wall outward and from the far field inward:
+//     S_l = D_l - L_l S_{l-1}^{-1} U_{l-1}   (l < M),
+//     S_l = D_l - U_l S_{l+1}^{-1} L_{l+1}   (l > M),
+// meeting at level M, which takes both updates.  The two chains run in the same
+// launches, which halves the sequential depth.  Every S_l^{-1} is kept
+// explicitly, computed in place by blocked Gauss-Jordan without pivoting.  L and U stay sparse (at most a handful of
 // entries per row / column), so the Schur update is O(n^2) per level and the
 // dense inversion (2 n^3 flops, n = 2*imax) carries the work.  A solve is two
-// sweeps of one dense GEMV per level:
-//     forward  z_l = S_l^{-1} (b_l - L_l z_{l-1})
-//     backward x_l = z_l - S_l^{-1} U_l x_{l+1}.
+// sweeps of one dense GEMV per level (see direct_solve).
 // With T = double this is the exact Newton step of the reference's MUMPS LU
 // (vorti2d/petsc_solver.py:55-62); with T = float it is the mixed-precision
 // preconditioner.
@@ -55,18 +56,21 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
         return -1;
     }
     const int64_t n2 = 2 * N;
-    std::vector<int> cD(n2 + 1, 0), cL(n2 + 1, 0), cU(n2 + 1, 0), cUC(n2 + 1, 0);
+    std::vector<int> cD(n2 + 1, 0), cL(n2 + 1, 0), cU(n2 + 1, 0), cUC(n2 + 1, 0), cLC(n2 + 1, 0);
     // two passes: count, then fill
-    std::vector<int2> eD, eL, eU, eUC;
+    std::vector<int2> eD, eL, eU, eUC, eLC;
     for (int pass = 0; pass < 2; ++pass) {
-        std::vector<int> fD, fL, fU, fUC;
+        std::vector<int> fD, fL, fU, fUC, fLC;
         if (pass == 1) {
             for (int64_t r = 0; r < n2; ++r) {
                 cD[r + 1] += cD[r]; cL[r + 1] += cL[r]; cU[r + 1] += cU[r]; cUC[r + 1] += cUC[r];
+                cLC[r + 1] += cLC[r];
             }
             eD.resize(cD[n2]); eL.resize(cL[n2]); eU.resize(cU[n2]); eUC.resize(cUC[n2]);
+            eLC.resize(cLC[n2]);
             fD.assign(cD.begin(), cD.end() - 1); fL.assign(cL.begin(), cL.end() - 1);
             fU.assign(cU.begin(), cU.end() - 1); fUC.assign(cUC.begin(), cUC.end() - 1);
+            fLC.assign(cLC.begin(), cLC.end() - 1);
         }
         for (int64_t g = 0; g < n2; ++g) {
             const int comp = g >= N;
@@ -87,7 +91,9 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
                 if (lc == lr) {
                     if (pass == 0) ++cD[R + 1]; else eD[fD[R]++] = make_int2(c, vidx);
                 } else if (lc == lr - 1) {
-                    if (pass == 0) ++cL[R + 1]; else eL[fL[R]++] = make_int2(c, vidx);
+                    const int64_t C = P.voff[lc] + c;
+                    if (pass == 0) { ++cL[R + 1]; ++cLC[C + 1]; }
+                    else { eL[fL[R]++] = make_int2(c, vidx); eLC[fLC[C]++] = make_int2(r, vidx); }
                 } else if (lc == lr + 1) {
                     const int64_t C = P.voff[lc] + c;
                     if (pass == 0) { ++cU[R + 1]; ++cUC[C + 1]; }
@@ -109,81 +115,115 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
         return 0;
     };
     if (up(&P.d_Dptr, &P.d_D, cD, eD) || up(&P.d_Lptr, &P.d_Lr, cL, eL) ||
-        up(&P.d_Uptr, &P.d_Ur, cU, eU) || up(&P.d_UCptr, &P.d_Uc, cUC, eUC))
+        up(&P.d_Uptr, &P.d_Ur, cU, eU) || up(&P.d_UCptr, &P.d_Uc, cUC, eUC) ||
+        up(&P.d_LCptr, &P.d_Lc, cLC, eLC))
         return -1;
     return 0;
 }
 
 void direct_free_pattern(DirectPattern &P)
 {
-    int *ptrs[] = {P.d_Dptr, P.d_Lptr, P.d_Uptr, P.d_UCptr};
-    int2 *ents[] = {P.d_D, P.d_Lr, P.d_Ur, P.d_Uc};
+    int *ptrs[] = {P.d_Dptr, P.d_Lptr, P.d_Uptr, P.d_UCptr, P.d_LCptr};
+    int2 *ents[] = {P.d_D, P.d_Lr, P.d_Ur, P.d_Uc, P.d_Lc};
     for (int *p : ptrs) if (p) (void)hipFree(p);
     for (int2 *p : ents) if (p) (void)hipFree(p);
-    P.d_Dptr = P.d_Lptr = P.d_Uptr = P.d_UCptr = nullptr;
-    P.d_D = P.d_Lr = P.d_Ur = P.d_Uc = nullptr;
+    P.d_Dptr = P.d_Lptr = P.d_Uptr = P.d_UCptr = P.d_LCptr = nullptr;
+    P.d_D = P.d_Lr = P.d_Ur = P.d_Uc = P.d_Lc = nullptr;
 }
+
+
+// ------------------------------------------------------------ batching ----
+// Both elimination chains advance together: every launch below carries up to
+// two level jobs and selects its job with blockIdx.z.
+template <typename T>
+struct Job {
+    T *S;                  // dense Schur complement -> its inverse (n x ld, row-major)
+    int n, ld, l;          // size, row stride, level
+    int64_t voff;          // offset of the level in level-ordered vectors / CSR rows
+    T *colk, *dinv, *tmat; // per-job scratch
+};
+
+// One Schur-update source: the neighbouring level's inverse on the side the
+// chain comes from, the column lists of the coupling into this level
+// (entries: row in the neighbour, value index) and the row lists of this level
+// into the neighbour (entries: column in the neighbour, value index).
+template <typename T>
+struct Src {
+    const T *Sprev;
+    int np, ldp;
+    const int *cptr;
+    const int2 *cent;
+    const int *rptr;
+    const int2 *rent;
+};
+
+template <typename T>
+struct Batch {
+    Job<T> j[2];
+    Src<T> src[2];
+    int nj;
+};
 
 // ------------------------------------------------------- level build ------
 template <typename T>
-__global__ void __launch_bounds__(256) k_fill(T *__restrict__ p, int64_t n, T v)
+__global__ void __launch_bounds__(256) k_fill_b(Batch<T> B)
 {
+    const Job<T> &J = B.j[blockIdx.z];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = v;
+    if (t < (int64_t)J.n * J.ld) J.S[t] = T(0);
 }
 
-// S[r][c] = D entries of level rows
+// S[r][c] = D entries of the level rows
 template <typename T>
-__global__ void __launch_bounds__(256) k_scatter_D(int n, int ld, const int *__restrict__ ptr,
-                                                   const int2 *__restrict__ ent,
-                                                   const T *__restrict__ val, T *__restrict__ S)
+__global__ void __launch_bounds__(256) k_scatter_D_b(Batch<T> B, const int *__restrict__ dptr,
+                                                     const int2 *__restrict__ dent,
+                                                     const T *__restrict__ val)
 {
+    const Job<T> &J = B.j[blockIdx.z];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    if (r >= J.n) return;
+    const int *ptr = dptr + J.voff;
     for (int e = ptr[r]; e < ptr[r + 1]; ++e) {
-        const int2 x = ent[e];
-        S[(int64_t)r * ld + x.x] = val[x.y];
+        const int2 x = dent[e];
+        J.S[(int64_t)r * J.ld + x.x] = val[x.y];
     }
 }
 
-// Tm[c][q] = sum_{(p,u) in Ucol(q)} Sprev[c][p] * u      (np x nl)
+// Tm[c][q] = sum_{(p,u) in col(q)} Sprev[c][p] * u      (np x n, stride ld)
 template <typename T>
-__global__ void __launch_bounds__(256) k_schur_T(int np, int ldp, int nl, int ldl,
-                                                 const int *__restrict__ ucptr,
-                                                 const int2 *__restrict__ uc,
-                                                 const T *__restrict__ val,
-                                                 const T *__restrict__ Sprev, T *__restrict__ Tm)
+__global__ void __launch_bounds__(256) k_schur_T_b(Batch<T> B, const T *__restrict__ val)
 {
+    const Job<T> &J = B.j[blockIdx.z];
+    const Src<T> &S = B.src[blockIdx.z];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
-    if (q >= nl) return;
+    if (q >= J.n || c >= S.np) return;
     double acc = 0.0;
-    const T *row = Sprev + (int64_t)c * ldp;
-    for (int e = ucptr[q]; e < ucptr[q + 1]; ++e) {
-        const int2 x = uc[e];
+    const T *row = S.Sprev + (int64_t)c * S.ldp;
+    for (int e = S.cptr[q]; e < S.cptr[q + 1]; ++e) {
+        const int2 x = S.cent[e];
         acc += (double)row[x.x] * (double)val[x.y];
     }
-    Tm[(int64_t)c * ldl + q] = (T)acc;
+    J.tmat[(int64_t)c * J.ld + q] = (T)acc;
 }
 
-// S[r][q] -= sum_{(c,v) in Lrow(r)} v * Tm[c][q]
+// S[r][q] -= sum_{(c,v) in row(r)} v * Tm[c][q]
 template <typename T>
-__global__ void __launch_bounds__(256) k_schur_S(int nl, int ldl, const int *__restrict__ lptr,
-                                                 const int2 *__restrict__ lr,
-                                                 const T *__restrict__ val,
-                                                 const T *__restrict__ Tm, T *__restrict__ S)
+__global__ void __launch_bounds__(256) k_schur_S_b(Batch<T> B, const T *__restrict__ val)
 {
+    const Job<T> &J = B.j[blockIdx.z];
+    const Src<T> &S = B.src[blockIdx.z];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
-    if (q >= nl) return;
-    const int e0 = lptr[r], e1 = lptr[r + 1];
+    if (q >= J.n || r >= J.n) return;
+    const int e0 = S.rptr[r], e1 = S.rptr[r + 1];
     if (e0 == e1) return;
-    double acc = (double)S[(int64_t)r * ldl + q];
+    double acc = (double)J.S[(int64_t)r * J.ld + q];
     for (int e = e0; e < e1; ++e) {
-        const int2 x = lr[e];
-        acc -= (double)val[x.y] * (double)Tm[(int64_t)x.x * ldl + q];
+        const int2 x = S.rent[e];
+        acc -= (double)val[x.y] * (double)J.tmat[(int64_t)x.x * J.ld + q];
     }
-    S[(int64_t)r * ldl + q] = (T)acc;
+    J.S[(int64_t)r * J.ld + q] = (T)acc;
 }
 
 // --------------------------------------------------- Gauss-Jordan ---------
@@ -241,19 +281,21 @@ __device__ __forceinline__ void gj_regs(T (&a)[16], int kb, T *prow, T *pcol, T 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_gj_diag(const T *__restrict__ A, int n, int ld, int k0,
-                                                 T *__restrict__ dinv)
+__global__ void __launch_bounds__(256) k_gj_diag_b(Batch<T> B, int k0)
 {
     __shared__ T prow[2 * NB], pcol[2 * NB], scale[NB];
+    const Job<T> &J = B.j[blockIdx.z];
+    const int n = J.n, ld = J.ld;
+    if (k0 >= n) return;
     const int kb = min(NB, n - k0);
     const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
     T a[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int r = rg * 16 + q;
-        a[q] = (r < kb && c < kb) ? A[(int64_t)(k0 + r) * ld + k0 + c] : T(0);
+        a[q] = (r < kb && c < kb) ? J.S[(int64_t)(k0 + r) * ld + k0 + c] : T(0);
     }
-    gj_regs(a, kb, prow, pcol, scale, dinv);
+    gj_regs(a, kb, prow, pcol, scale, J.dinv);
 }
 
 // blockIdx.y == 0: pivot rows  A[K, c] <- Dinv A[K, c]  (A[K, K] <- Dinv) for a
@@ -262,26 +304,29 @@ __global__ void __launch_bounds__(256) k_gj_diag(const T *__restrict__ A, int n,
 // blockIdx.y == 1: colk[r][t] <- A[r][k0+t] for a 16-row tile of rows outside K
 constexpr int RP = 16;
 template <typename T>
-__global__ void __launch_bounds__(256) k_gj_rowpanel(T *__restrict__ A, int n, int ld, int k0,
-                                                     const T *__restrict__ dinv,
-                                                     T *__restrict__ colk)
+__global__ void __launch_bounds__(256) k_gj_rowpanel_b(Batch<T> B, int k0)
 {
     __shared__ T sdT[NB * NB];   // sdT[s][t] = Dinv[t][s]
     __shared__ T sp[NB * RP];    // sp[s][c]  = A[k0+s][base+c]
+    const Job<T> &J = B.j[blockIdx.z];
+    T *__restrict__ A = J.S;
+    const int n = J.n, ld = J.ld;
+    if (k0 >= n) return;
     const int kb = min(NB, n - k0);
     const int base = blockIdx.x * RP;
+    if (base >= n) return;
     if (blockIdx.y == 1) {
         for (int idx = threadIdx.x; idx < RP * NB; idx += blockDim.x) {
             const int r = base + idx / NB, t = idx % NB;
             if (r < n && t < kb && (r < k0 || r >= k0 + kb))
-                colk[(int64_t)r * NB + t] = A[(int64_t)r * ld + k0 + t];
+                J.colk[(int64_t)r * NB + t] = A[(int64_t)r * ld + k0 + t];
         }
         return;
     }
     const bool cin = (base >= k0 && base < k0 + NB);
     for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
         const int t = idx / NB, s2 = idx % NB;
-        sdT[s2 * NB + t] = dinv[idx];
+        sdT[s2 * NB + t] = J.dinv[idx];
     }
     for (int idx = threadIdx.x; idx < NB * RP; idx += blockDim.x) {
         const int t = idx / RP, c = idx % RP;
@@ -313,16 +358,19 @@ __global__ void __launch_bounds__(256) k_gj_rowpanel(T *__restrict__ A, int n, i
 // A[i][c] = (c in K ? 0 : A[i][c]) - sum_t colk[i][t] A[k0+t][c]   for i outside K.
 // The workgroup owning the next pivot tile also inverts it into dinv.
 template <typename T>
-__global__ void __launch_bounds__(256) k_gj_update(T *__restrict__ A, int n, int ld, int k0,
-                                                   const T *__restrict__ colk,
-                                                   T *__restrict__ dinv)
+__global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
 {
     __shared__ T sc[NB * NB];   // sc[t][r]: colk tile, transposed
     __shared__ T sr[NB * NB];   // sr[t][c]: pivot-row tile
+    const Job<T> &J = B.j[blockIdx.z];
+    T *__restrict__ A = J.S;
+    const int n = J.n, ld = J.ld;
+    if (k0 >= n) return;
     const int kb = min(NB, n - k0);
     // Tile order: the next pivot tile (k0+NB, k0+NB) is block 0, so its update
     // and inversion start first and overlap the bulk of the rank-NB update.
     const int nt = (n + NB - 1) / NB;
+    if ((int)blockIdx.x >= nt || (int)blockIdx.y >= nt) return;
     const int kt1 = k0 / NB + 1;
     int lin = blockIdx.y * nt + blockIdx.x;
     if (kt1 < nt) {
@@ -334,7 +382,7 @@ __global__ void __launch_bounds__(256) k_gj_update(T *__restrict__ A, int n, int
     if (i0 == k0) return;
     for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
         const int r = idx / NB, t = idx % NB;
-        sc[t * NB + r] = (i0 + r < n && t < kb) ? colk[(int64_t)(i0 + r) * NB + t] : T(0);
+        sc[t * NB + r] = (i0 + r < n && t < kb) ? J.colk[(int64_t)(i0 + r) * NB + t] : T(0);
         sr[idx] = (r < kb && c0 + t < n) ? A[(int64_t)(k0 + r) * ld + c0 + t] : T(0);
     }
     __syncthreads();
@@ -383,49 +431,140 @@ __global__ void __launch_bounds__(256) k_gj_update(T *__restrict__ A, int n, int
         T g[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) g[q] = sc[(rg * 16 + q) * NB + c];
-        gj_regs(g, kb2, sr, sr + 2 * NB, sr + 4 * NB, dinv);
+        gj_regs(g, kb2, sr, sr + 2 * NB, sr + 4 * NB, J.dinv);
     }
+}
+
+// In-place Gauss-Jordan inverse of the batch's dense matrices.
+template <typename T>
+static int gj_invert_b(const Batch<T> &B, hipStream_t s)
+{
+    int nmax = 0;
+    for (int z = 0; z < B.nj; ++z) nmax = std::max(nmax, B.j[z].n);
+    const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
+    const unsigned np = (unsigned)((nmax + RP - 1) / RP);
+    const unsigned nz = (unsigned)B.nj;
+    hipLaunchKernelGGL(k_gj_diag_b<T>, dim3(1, 1, nz), dim3(256), 0, s, B, 0);
+    for (int k0 = 0; k0 < nmax; k0 += NB) {
+        hipLaunchKernelGGL(k_gj_rowpanel_b<T>, dim3(np, 2, nz), dim3(256), 0, s, B, k0);
+        hipLaunchKernelGGL(k_gj_update_b<T>, dim3(nt, nt, nz), dim3(256), 0, s, B, k0);
+    }
+    V2D_LAUNCH();
+    return 0;
 }
 
 template <typename T>
 int gj_invert(T *A, int n, int ld, T *colk, T *dinv, hipStream_t s)
 {
-    const unsigned nt = (unsigned)((n + NB - 1) / NB);
-    const unsigned np = (unsigned)((n + RP - 1) / RP);
-    hipLaunchKernelGGL(k_gj_diag<T>, dim3(1), dim3(256), 0, s, A, n, ld, 0, dinv);
-    V2D_LAUNCH();
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        hipLaunchKernelGGL(k_gj_rowpanel<T>, dim3(np, 2), dim3(256), 0, s, A, n, ld, k0, dinv, colk);
-        hipLaunchKernelGGL(k_gj_update<T>, dim3(nt, nt), dim3(256), 0, s, A, n, ld, k0, colk, dinv);
-    }
-    V2D_LAUNCH();
-    return 0;
+    Batch<T> B{};
+    B.nj = 1;
+    B.j[0] = Job<T>{A, n, ld, 0, 0, colk, dinv, nullptr};
+    return gj_invert_b(B, s);
 }
 template int gj_invert<double>(double *, int, int, double *, double *, hipStream_t);
 template int gj_invert<float>(float *, int, int, float *, float *, hipStream_t);
+
+// ------------------------------------------------------------ factor ------
+// Twisted block factorisation: the bottom chain eliminates levels 0..M-1 from
+// the wall outward (S_l = D_l - L_l S_{l-1}^-1 U_{l-1}), the top chain levels
+// nlev-1..M+1 from the far field inward (S_l = D_l - U_l S_{l+1}^-1 L_{l+1}),
+// both in the same launches; the middle level M takes both Schur updates.
+template <typename T>
+static Job<T> make_job(const DirectPattern &P, DirectWork<T> &W, int l, int slot)
+{
+    Job<T> J;
+    J.S = W.sinv + P.soff[l];
+    J.n = P.n[l];
+    J.ld = P.ld[l];
+    J.l = l;
+    J.voff = P.voff[l];
+    J.colk = W.colk + (int64_t)slot * P.nmax * NB;
+    J.dinv = W.dinv + slot * NB * NB;
+    J.tmat = W.tmat + (int64_t)slot * P.nmax * P.ldmax;
+    return J;
+}
+
+// Schur source of level l coming from level p = l-1 (bottom) or l+1 (top).
+template <typename T>
+static Src<T> make_src(const DirectPattern &P, const DirectWork<T> &W, int l, int p)
+{
+    Src<T> S;
+    S.Sprev = W.sinv + P.soff[p];
+    S.np = P.n[p];
+    S.ldp = P.ld[p];
+    if (p == l - 1) {   // coupling U_{l-1} (columns in l), rows L_l
+        S.cptr = P.d_UCptr + P.voff[l]; S.cent = P.d_Uc;
+        S.rptr = P.d_Lptr + P.voff[l];  S.rent = P.d_Lr;
+    } else {            // coupling L_{l+1} (columns in l), rows U_l
+        S.cptr = P.d_LCptr + P.voff[l]; S.cent = P.d_Lc;
+        S.rptr = P.d_Uptr + P.voff[l];  S.rent = P.d_Ur;
+    }
+    return S;
+}
+
+template <typename T>
+static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, bool schur[2][2],
+                            int srcp[2][2], DirectWork<T> &W, hipStream_t s)
+{
+    int nmax = 0, npmax = 0;
+    for (int z = 0; z < B.nj; ++z) nmax = std::max(nmax, B.j[z].n);
+    const unsigned nz = (unsigned)B.nj;
+    hipLaunchKernelGGL(k_fill_b<T>, dim3(grid_for((int64_t)nmax * P.ldmax), 1, nz), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(k_scatter_D_b<T>, dim3(grid_for(nmax), 1, nz), dim3(256), 0, s, B, P.d_Dptr,
+                       P.d_D, val);
+    for (int k = 0; k < 2; ++k) {   // up to two Schur sources (the middle level has two)
+        Batch<T> Bk = B;
+        int nj = 0;
+        for (int z = 0; z < B.nj; ++z)
+            if (schur[z][k]) {
+                Bk.j[nj] = B.j[z];
+                Bk.src[nj] = make_src(P, W, B.j[z].l, srcp[z][k]);
+                npmax = std::max(npmax, Bk.src[nj].np);
+                ++nj;
+            }
+        if (!nj) continue;
+        Bk.nj = nj;
+        hipLaunchKernelGGL(k_schur_T_b<T>, dim3(grid_for(nmax), npmax, nj), dim3(256), 0, s, Bk, val);
+        hipLaunchKernelGGL(k_schur_S_b<T>, dim3(grid_for(nmax), nmax, nj), dim3(256), 0, s, Bk, val);
+    }
+    V2D_LAUNCH();
+    return gj_invert_b(B, s);
+}
 
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
 {
     const int nlev = P.L.nlev;
-    for (int l = 0; l < nlev; ++l) {
-        const int n = P.n[l], ld = P.ld[l];
-        T *S = W.sinv + P.soff[l];
-        const int64_t nn = (int64_t)n * ld;
-        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(nn)), dim3(256), 0, s, S, nn, T(0));
-        hipLaunchKernelGGL(k_scatter_D<T>, dim3(grid_for(n)), dim3(256), 0, s, n, ld,
-                           P.d_Dptr + P.voff[l], P.d_D, val, S);
-        if (l > 0) {
-            const int np = P.n[l - 1], ldp = P.ld[l - 1];
-            hipLaunchKernelGGL(k_schur_T<T>, dim3(grid_for(n), np), dim3(256), 0, s, np, ldp, n, ld,
-                               P.d_UCptr + P.voff[l], P.d_Uc, val, W.sinv + P.soff[l - 1], W.tmat);
-            hipLaunchKernelGGL(k_schur_S<T>, dim3(grid_for(n), n), dim3(256), 0, s, n, ld,
-                               P.d_Lptr + P.voff[l], P.d_Lr, val, W.tmat, S);
+    const int M = (nlev - 1) / 2;            // meeting level
+    const int nbot = M, ntop = nlev - 1 - M;
+    const int steps = std::max(nbot, ntop);
+    for (int st = 0; st < steps; ++st) {
+        Batch<T> B{};
+        bool schur[2][2] = {{false, false}, {false, false}};
+        int srcp[2][2] = {{0, 0}, {0, 0}};
+        int nj = 0;
+        if (st < nbot) {
+            const int l = st;
+            B.j[nj] = make_job(P, W, l, nj);
+            if (l > 0) { schur[nj][0] = true; srcp[nj][0] = l - 1; }
+            ++nj;
         }
-        V2D_LAUNCH();
-        if (gj_invert<T>(S, n, ld, W.colk, W.dinv, s)) return -1;
+        if (st < ntop) {
+            const int l = nlev - 1 - st;
+            B.j[nj] = make_job(P, W, l, nj);
+            if (st > 0) { schur[nj][0] = true; srcp[nj][0] = l + 1; }
+            ++nj;
+        }
+        B.nj = nj;
+        if (build_and_invert(P, B, val, schur, srcp, W, s)) return -1;
     }
-    return 0;
+    Batch<T> B{};
+    bool schur[2][2] = {{M > 0, M < nlev - 1}, {false, false}};
+    int srcp[2][2] = {{M - 1, M + 1}, {0, 0}};
+    if (!schur[0][0]) { schur[0][0] = schur[0][1]; srcp[0][0] = srcp[0][1]; schur[0][1] = false; }
+    B.nj = 1;
+    B.j[0] = make_job(P, W, M, 0);
+    return build_and_invert(P, B, val, schur, srcp, W, s);
 }
 template int direct_factor<double>(const DirectPattern &, DirectWork<double> &, const double *, hipStream_t);
 template int direct_factor<float>(const DirectPattern &, DirectWork<float> &, const float *, hipStream_t);
@@ -438,51 +577,52 @@ __device__ __forceinline__ double wsum(double v)
     return v;
 }
 
-constexpr int kRowsPerWave = 2;
-constexpr int kRowsPerBlock = 4 * kRowsPerWave;
-
-// forward right-hand side of level l:  y = b_l - L_l z_{l-1}
+// Sparse right-hand side of a level:  y = [b_l] - sum_t list_t(r) . vec_t
+// (a term's vector is level-ordered, or the global field-blocked vector
+// addressed through the level map when glob is set).
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_rhs(Levels L, int64_t N, int l, int n,
-                                                 const int *__restrict__ lptr,
-                                                 const int2 *__restrict__ lr,
-                                                 const T *__restrict__ val, const T *__restrict__ b,
-                                                 const T *__restrict__ zprev, T *__restrict__ y)
+struct RhsTerm {
+    const int *ptr;
+    const int2 *ent;
+    const T *vec;
+    int vlev, glob;
+};
+template <typename T>
+struct RhsJob {
+    int l, n, nt;
+    const T *b;        // global right-hand side or null
+    RhsTerm<T> t[2];
+    T *y;
+};
+template <typename T>
+struct RhsBatch {
+    RhsJob<T> j[2];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_rhs_b(RhsBatch<T> B, Levels L, int64_t N,
+                                               const T *__restrict__ val, double sgn)
 {
+    const RhsJob<T> &J = B.j[blockIdx.z];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double acc = (double)b[L.global(r, N, l)];
-    if (l > 0)
-        for (int e = lptr[r]; e < lptr[r + 1]; ++e) {
-            const int2 x = lr[e];
-            acc -= (double)val[x.y] * (double)zprev[x.x];
+    if (r >= J.n) return;
+    double sum = 0.0;
+    for (int k = 0; k < J.nt; ++k) {
+        const RhsTerm<T> &t = J.t[k];
+        for (int e = t.ptr[r]; e < t.ptr[r + 1]; ++e) {
+            const int2 x = t.ent[e];
+            const double v = t.glob ? (double)t.vec[L.global(x.x, N, t.vlev)] : (double)t.vec[x.x];
+            sum += (double)val[x.y] * v;
         }
-    y[r] = (T)acc;
-}
-
-// backward right-hand side of level l:  w = U_l x_{l+1}
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_rhs(Levels L, int64_t N, int l, int n,
-                                                 const int *__restrict__ uptr,
-                                                 const int2 *__restrict__ ur,
-                                                 const T *__restrict__ val, const T *__restrict__ x,
-                                                 T *__restrict__ w)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double acc = 0.0;
-    for (int e = uptr[r]; e < uptr[r + 1]; ++e) {
-        const int2 u = ur[e];
-        acc += (double)val[u.y] * (double)x[L.global(u.x, N, l + 1)];
     }
-    w[r] = (T)acc;
+    const double b = J.b ? (double)J.b[L.global(r, N, J.l)] : 0.0;
+    J.y[r] = (T)(b + sgn * sum);
 }
 
-// out = base - S v  (mode 1: scatter to the global vector, base = z_l)
-//        or  S v    (mode 0: level-ordered z_l)
-// One wavefront per row; rows are padded to ld (a multiple of 64 elements), so
-// every lane reads 16-byte vectors and the padding (zeros in S) is skipped by
-// the bound on the vector length.
+// out = base + sgn * S v for up to two levels: level-ordered output (glob = 0)
+// or scattered to the global vector (glob = 1).  One wavefront per row; rows
+// are padded to ld (a multiple of 64 elements) so every lane reads 16-byte
+// vectors, and the padding (zeros) is bounded by the vector length.
 template <typename T> struct Vec16;
 template <> struct Vec16<float> { using type = float4; static constexpr int w = 4; };
 template <> struct Vec16<double> { using type = double2; static constexpr int w = 2; };
@@ -501,22 +641,33 @@ __device__ __forceinline__ double dot16<double>(const double2 &a, const double *
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_level_gemv(Levels L, int64_t N, int l, int n, int ld,
-                                                    int mode, const T *__restrict__ S,
-                                                    const T *__restrict__ v,
-                                                    const T *__restrict__ base,
-                                                    T *__restrict__ out)
+struct GemvJob {
+    const T *S;
+    int n, ld, l, glob;
+    const T *v, *base;
+    T *out;
+};
+template <typename T>
+struct GemvBatch {
+    GemvJob<T> j[2];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemv_b(GemvBatch<T> B, Levels L, int64_t N, double sgn)
 {
     using V = typename Vec16<T>::type;
     constexpr int W = Vec16<T>::w;
+    const GemvJob<T> &J = B.j[blockIdx.z];
+    const int n = J.n, ld = J.ld;
+    if ((int)blockIdx.x * 4 >= n) return;
     extern __shared__ unsigned char smem[];
     T *vs = reinterpret_cast<T *>(smem);
-    for (int r = threadIdx.x; r < ld; r += blockDim.x) vs[r] = r < n ? v[r] : T(0);
+    for (int r = threadIdx.x; r < ld; r += blockDim.x) vs[r] = r < n ? J.v[r] : T(0);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + wave;
     if (r >= n) return;
-    const V *row = reinterpret_cast<const V *>(S + (int64_t)r * ld);
+    const V *row = reinterpret_cast<const V *>(J.S + (int64_t)r * ld);
     const int nv = (n + W - 1) / W;
     double acc0 = 0.0, acc1 = 0.0;
     int c = lane;
@@ -528,54 +679,106 @@ __global__ void __launch_bounds__(256) k_level_gemv(Levels L, int64_t N, int l, 
     if (c < nv) acc0 += dot16<T>(row[c], vs + c * W);
     const double acc = wsum(acc0 + acc1);
     if (lane == 0) {
-        if (mode == 0) out[r] = (T)acc;
-        else out[L.global(r, N, l)] = (T)((double)base[r] - acc);
+        const double b = J.base ? (double)J.base[r] : 0.0;
+        const T o = (T)(b + sgn * acc);
+        if (J.glob) J.out[L.global(r, N, J.l)] = o;
+        else J.out[r] = o;
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_last(Levels L, int64_t N, int l, int n,
-                                                  const T *__restrict__ zl, T *__restrict__ x)
+static int launch_rhs(const RhsBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
+                      double sgn, hipStream_t s)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) x[L.global(r, N, l)] = zl[r];
+    hipLaunchKernelGGL(k_rhs_b<T>, dim3(grid_for(nmax), 1, nj), dim3(256), 0, s, B, P.L, P.N, val, sgn);
+    V2D_LAUNCH();
+    return 0;
 }
 
+template <typename T>
+static int launch_gemv(const GemvBatch<T> &B, int nj, int nmax, const DirectPattern &P, double sgn,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gemv_b<T>, dim3((nmax + 3) / 4, 1, nj), dim3(256), sizeof(T) * P.ldmax, s, B,
+                       P.L, P.N, sgn);
+    V2D_LAUNCH();
+    return 0;
+}
+
+// Two sweeps over the twisted factorisation:
+//   forward  bottom: z_l = S_l^-1 (b_l - L_l z_{l-1}),  top: z_l = S_l^-1 (b_l - U_l z_{l+1})
+//   middle        : x_M = S_M^-1 (b_M - L_M z_{M-1} - U_M z_{M+1})
+//   backward bottom: x_l = z_l - S_l^-1 U_l x_{l+1},   top: x_l = z_l - S_l^-1 L_l x_{l-1}
 template <typename T>
 int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
                  hipStream_t s)
 {
     const int nlev = P.L.nlev;
-    const size_t lds = sizeof(T) * (size_t)P.ldmax;
-    if (lds > 65536) {
+    if (sizeof(T) * (size_t)P.ldmax > 65536) {
         set_error("direct solver: a level of " + std::to_string(P.nmax) +
                   " unknowns exceeds the 64 KiB LDS staging of the solve sweeps");
         return -1;
     }
-    T *y = W.colk;   // level right-hand side scratch (nmax <= nmax*64)
-    for (int l = 0; l < nlev; ++l) {
-        const int n = P.n[l];
-        const T *zprev = l > 0 ? W.z + P.voff[l - 1] : W.z;
-        hipLaunchKernelGGL(k_fwd_rhs<T>, dim3(grid_for(n)), dim3(256), 0, s, P.L, P.N, l, n,
-                           P.d_Lptr + P.voff[l], P.d_Lr, val, b, zprev, y);
-        hipLaunchKernelGGL(k_level_gemv<T>, dim3((n + 3) / 4), dim3(256), sizeof(T) * P.ld[l], s,
-                           P.L, P.N, l, n, P.ld[l], 0, W.sinv + P.soff[l], y, (const T *)nullptr,
-                           W.z + P.voff[l]);
+    const int M = (nlev - 1) / 2;
+    const int nbot = M, ntop = nlev - 1 - M;
+    const int steps = std::max(nbot, ntop);
+    T *ybuf[2] = {W.colk, W.colk + (int64_t)P.nmax * NB};
+    auto zl = [&](int l) { return W.z + P.voff[l]; };
+    auto S = [&](int l) { return (const T *)(W.sinv + P.soff[l]); };
+    // forward sweeps
+    for (int st = 0; st < steps; ++st) {
+        RhsBatch<T> R{};
+        GemvBatch<T> G{};
+        int nj = 0, nmax = 0;
+        for (int side = 0; side < 2; ++side) {
+            if ((side == 0 && st >= nbot) || (side == 1 && st >= ntop)) continue;
+            const int l = side == 0 ? st : nlev - 1 - st;
+            RhsJob<T> &r = R.j[nj];
+            r.l = l; r.n = P.n[l]; r.b = b; r.y = ybuf[nj]; r.nt = 0;
+            if (st > 0) {
+                const int p = side == 0 ? l - 1 : l + 1;
+                r.t[0] = RhsTerm<T>{side == 0 ? P.d_Lptr + P.voff[l] : P.d_Uptr + P.voff[l],
+                                    side == 0 ? P.d_Lr : P.d_Ur, zl(p), p, 0};
+                r.nt = 1;
+            }
+            G.j[nj] = GemvJob<T>{S(l), P.n[l], P.ld[l], l, 0, ybuf[nj], nullptr, zl(l)};
+            nmax = std::max(nmax, P.n[l]);
+            ++nj;
+        }
+        if (launch_rhs(R, nj, nmax, P, val, -1.0, s) || launch_gemv(G, nj, nmax, P, 1.0, s)) return -1;
     }
-    V2D_LAUNCH();
-    {
-        const int l = nlev - 1, n = P.n[l];
-        hipLaunchKernelGGL(k_bwd_last<T>, dim3(grid_for(n)), dim3(256), 0, s, P.L, P.N, l, n,
-                           W.z + P.voff[l], x);
+    {   // middle level
+        RhsBatch<T> R{};
+        GemvBatch<T> G{};
+        RhsJob<T> &r = R.j[0];
+        r.l = M; r.n = P.n[M]; r.b = b; r.y = ybuf[0]; r.nt = 0;
+        if (M > 0)
+            r.t[r.nt++] = RhsTerm<T>{P.d_Lptr + P.voff[M], P.d_Lr, zl(M - 1), M - 1, 0};
+        if (M < nlev - 1)
+            r.t[r.nt++] = RhsTerm<T>{P.d_Uptr + P.voff[M], P.d_Ur, zl(M + 1), M + 1, 0};
+        G.j[0] = GemvJob<T>{S(M), P.n[M], P.ld[M], M, 1, ybuf[0], nullptr, x};
+        if (launch_rhs(R, 1, P.n[M], P, val, -1.0, s) || launch_gemv(G, 1, P.n[M], P, 1.0, s)) return -1;
     }
-    for (int l = nlev - 2; l >= 0; --l) {
-        const int n = P.n[l];
-        hipLaunchKernelGGL(k_bwd_rhs<T>, dim3(grid_for(n)), dim3(256), 0, s, P.L, P.N, l, n,
-                           P.d_Uptr + P.voff[l], P.d_Ur, val, x, y);
-        hipLaunchKernelGGL(k_level_gemv<T>, dim3((n + 3) / 4), dim3(256), sizeof(T) * P.ld[l], s,
-                           P.L, P.N, l, n, P.ld[l], 1, W.sinv + P.soff[l], y, W.z + P.voff[l], x);
+    // backward sweeps, outward from the middle
+    for (int st = 0; st < steps; ++st) {
+        RhsBatch<T> R{};
+        GemvBatch<T> G{};
+        int nj = 0, nmax = 0;
+        for (int side = 0; side < 2; ++side) {
+            const int l = side == 0 ? M - 1 - st : M + 1 + st;
+            if (l < 0 || l >= nlev) continue;
+            const int p = side == 0 ? l + 1 : l - 1;
+            RhsJob<T> &r = R.j[nj];
+            r.l = l; r.n = P.n[l]; r.b = nullptr; r.y = ybuf[nj]; r.nt = 1;
+            r.t[0] = RhsTerm<T>{side == 0 ? P.d_Uptr + P.voff[l] : P.d_Lptr + P.voff[l],
+                                side == 0 ? P.d_Ur : P.d_Lr, x, p, 1};
+            G.j[nj] = GemvJob<T>{S(l), P.n[l], P.ld[l], l, 1, ybuf[nj], zl(l), x};
+            nmax = std::max(nmax, P.n[l]);
+            ++nj;
+        }
+        if (!nj) break;
+        if (launch_rhs(R, nj, nmax, P, val, 1.0, s) || launch_gemv(G, nj, nmax, P, -1.0, s)) return -1;
     }
-    V2D_LAUNCH();
     return 0;
 }
 template int direct_solve<double>(const DirectPattern &, const DirectWork<double> &, const double *,

@@ -100,8 +100,8 @@ struct DirectPattern {
     int64_t sinv_elems = 0;        // total dense storage
     int nmax = 0, ldmax = 0;
     // device CSR (rows / columns indexed in level order), entries (local, vidx)
-    int *d_Dptr = nullptr, *d_Lptr = nullptr, *d_Uptr = nullptr, *d_UCptr = nullptr;
-    int2 *d_D = nullptr, *d_Lr = nullptr, *d_Ur = nullptr, *d_Uc = nullptr;
+    int *d_Dptr = nullptr, *d_Lptr = nullptr, *d_Uptr = nullptr, *d_UCptr = nullptr, *d_LCptr = nullptr;
+    int2 *d_D = nullptr, *d_Lr = nullptr, *d_Ur = nullptr, *d_Uc = nullptr, *d_Lc = nullptr;
 };
 
 int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags_host);
@@ -110,9 +110,9 @@ void direct_free_pattern(DirectPattern &P);
 template <typename T>
 struct DirectWork {
     T *sinv = nullptr;   // all dense inverses
-    T *tmat = nullptr;   // Schur scratch (nmax x ldmax)
-    T *colk = nullptr;   // Gauss-Jordan column panel (nmax x 64)
-    T *dinv = nullptr;   // 64 x 64 pivot-block inverse
+    T *tmat = nullptr;   // Schur scratch, 2 x (nmax x ldmax) (one per elimination chain)
+    T *colk = nullptr;   // Gauss-Jordan column panels, 2 x (nmax x 64)
+    T *dinv = nullptr;   // pivot-block inverses, 2 x (64 x 64)
     T *z = nullptr;      // level-ordered 2N work vector
 };
 
