@@ -144,8 +144,12 @@ def main():
         if prec == "mixed":
             ctx.set_solver(refactor_every=a.refactor_every, gmres_rtol=1e-6,
                            gmres_max_iter=a.gmres_max)
+        ctx.init_state()
+        tiles = ctx.autotune_update_tile(a.re)
         setup = time.perf_counter() - t0
-        log(f"[{prec}] setup {setup:.2f} s, device memory {ctx.device_bytes() / 2**30:.1f} GiB")
+        log(f"[{prec}] setup {setup:.2f} s (update-tile autotune "
+            + ", ".join(f"{k}: {v:.3f} s" for k, v in tiles.items())
+            + f"), device memory {ctx.device_bytes() / 2**30:.1f} GiB")
         nsteps = a.steps if prec == "mixed" else max(1, a.steps // 2)
         for w in range(a.warmup):
             dt, it, rn, rh = newton_solve(ctx, a.re)

@@ -113,6 +113,10 @@ int v2d_ctx_init_state(v2d_ctx *ctx);
  * refactor_every Newton steps (0 = only when GMRES needs more than
  * gmres_max_iter iterations), inner relative tolerance, iteration cap. */
 int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int gmres_max_iter);
+/* Tile edge (64 or 128) of the block-LU Gauss-Jordan update kernel: a tuning
+ * knob with no counterpart in the reference (the results do not depend on it
+ * beyond floating-point rounding order). */
+int v2d_ctx_set_update_tile(v2d_ctx *ctx, int tile);
 
 /* Device assembly of the Jacobian values and the FP64 residual-form RHS for
  * the current state (the scalars of assemble_coo). */

@@ -1,7 +1,9 @@
 """Build step of libvorti2d.so: hipcc for gfx950, in-tree.
 
-``python -m vorti2d_amd._build`` compiles ``vorti2d_amd/csrc/*.hip`` into
-``vorti2d_amd/libvorti2d.so``.  Called by ``__graft_entry__.build()``.
+``python vorti2d_amd/_build.py`` compiles ``vorti2d_amd/csrc/*.hip`` into
+``vorti2d_amd/libvorti2d.so``.  Called by ``__graft_entry__.build()``.  Run it as
+a script (not ``-m``): importing the package first would load the library that
+is about to be rebuilt.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 """ctypes binding of libvorti2d.so (the C ABI declared in include/vorti2d.h).
 
-The library is built in-tree by ``python -m vorti2d_amd._build``.  There is no
+The library is built in-tree by ``python vorti2d_amd/_build.py``.  There is no
 fallback: if the shared library is missing, importing this module raises.
 """
 from __future__ import annotations
@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libvorti2d.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"{LIB_PATH} is missing: build the HIP library first "
-        "(python -m vorti2d_amd._build, or __graft_entry__.build())")
+        "(python vorti2d_amd/_build.py, or __graft_entry__.build())")
 
 lib = C.CDLL(LIB_PATH)
 
@@ -44,6 +44,7 @@ _SIGS = {
     "v2d_ctx_get_state": ([vp, vp, vp, vp, vp], i32),
     "v2d_ctx_init_state": ([vp], i32),
     "v2d_ctx_set_solver": ([vp, i32, f64, i32], i32),
+    "v2d_ctx_set_update_tile": ([vp, i32], i32),
     "v2d_ctx_assemble": ([vp, f64, f64, f64, f64], i32),
     "v2d_ctx_rhs_norm": ([vp, P(f64)], i32),
     "v2d_ctx_nnz": ([vp, P(i64)], i32),

@@ -637,6 +637,17 @@ static int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double
     return 0;
 }
 
+int v2d_ctx_set_update_tile(v2d_ctx *ctx, int tile)
+{
+    if (tile != 64 && tile != 128) {
+        set_error("update tile must be 64 or 128");
+        return -1;
+    }
+    ctx->c->wd.update_tile = tile;
+    ctx->c->wf.update_tile = tile;
+    return 0;
+}
+
 int v2d_ctx_assemble(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot)
 {
     return ctx_assemble(ctx->c.get(), re, invdtau, inv2dt, urot);

@@ -162,6 +162,7 @@ struct Batch {
     Job<T> j[2];
     Src<T> src[2];
     int nj;
+    int ut;   // Gauss-Jordan update tile edge: 64 or 128
 };
 
 // ------------------------------------------------------- level build ------
@@ -355,76 +356,121 @@ __global__ void __launch_bounds__(256) k_gj_rowpanel_b(Batch<T> B, int k0)
     }
 }
 
-// A[i][c] = (c in K ? 0 : A[i][c]) - sum_t colk[i][t] A[k0+t][c]   for i outside K.
-// The workgroup owning the next pivot tile also inverts it into dinv.
-template <typename T>
+// A[i][c] = (c in K ? 0 : A[i][c]) - sum_t colk[i][t] A[k0+t][c]   for i outside K,
+// on UT x UT output tiles (UT = 64 or 128), MT x MT accumulators per thread
+// (MT = UT / 16), the K = 64 operand tiles staged in LDS.  Blocks form one 1-D
+// grid with the jobs interleaved: blocks 0 and 1 are the tiles holding each
+// job's next pivot block (k0+64, k0+64); they are dispatched first and also
+// invert that block into dinv, so the 64-pivot chain overlaps the bulk update.
+template <typename T> struct Vec8;
+template <> struct Vec8<float> { using type = float4; static constexpr int w = 4; };
+template <> struct Vec8<double> { using type = double2; static constexpr int w = 2; };
+
+template <typename T, int UT>
 __global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
 {
-    __shared__ T sc[NB * NB];   // sc[t][r]: colk tile, transposed
-    __shared__ T sr[NB * NB];   // sr[t][c]: pivot-row tile
-    const Job<T> &J = B.j[blockIdx.z];
+    constexpr int MT = UT / 16;
+    __shared__ T sc[NB * UT];   // sc[t][r]: colk rows of the tile, transposed
+    __shared__ T sr[NB * UT];   // sr[t][c]: pivot-row tile
+    using V = typename Vec8<T>::type;
+    constexpr int VW = Vec8<T>::w;
+    const int z = blockIdx.x % B.nj;
+    const Job<T> &J = B.j[z];
     T *__restrict__ A = J.S;
     const int n = J.n, ld = J.ld;
     if (k0 >= n) return;
     const int kb = min(NB, n - k0);
-    // Tile order: the next pivot tile (k0+NB, k0+NB) is block 0, so its update
-    // and inversion start first and overlap the bulk of the rank-NB update.
-    const int nt = (n + NB - 1) / NB;
-    if ((int)blockIdx.x >= nt || (int)blockIdx.y >= nt) return;
-    const int kt1 = k0 / NB + 1;
-    int lin = blockIdx.y * nt + blockIdx.x;
-    if (kt1 < nt) {
-        const int piv = kt1 * nt + kt1;
+    const int nt = (n + UT - 1) / UT;
+    int lin = blockIdx.x / B.nj;
+    if (lin >= nt * nt) return;
+    const int k1 = k0 + NB;
+    if (k1 < n) {
+        const int piv = (k1 / UT) * nt + k1 / UT;
         if (lin == 0) lin = piv;
         else if (lin <= piv) lin -= 1;
     }
-    const int i0 = (lin / nt) * NB, c0 = (lin % nt) * NB;
-    if (i0 == k0) return;
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+    const int i0 = (lin / nt) * UT, c0 = (lin % nt) * UT;
+    if (UT == NB && i0 == k0) return;   // a 64-row tile inside K has nothing to update
+    for (int idx = threadIdx.x; idx < UT * NB; idx += blockDim.x) {
         const int r = idx / NB, t = idx % NB;
-        sc[t * NB + r] = (i0 + r < n && t < kb) ? J.colk[(int64_t)(i0 + r) * NB + t] : T(0);
-        sr[idx] = (r < kb && c0 + t < n) ? A[(int64_t)(k0 + r) * ld + c0 + t] : T(0);
+        const int i = i0 + r;
+        const bool ok = i < n && t < kb && (i < k0 || i >= k0 + kb);
+        sc[t * UT + r] = ok ? J.colk[(int64_t)i * NB + t] : T(0);
+    }
+    for (int idx = threadIdx.x; idx < NB * UT; idx += blockDim.x) {
+        const int t = idx / UT, c = idx % UT;
+        sr[idx] = (t < kb && c0 + c < n) ? A[(int64_t)(k0 + t) * ld + c0 + c] : T(0);
     }
     __syncthreads();
     const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-    T acc[4][4];
-    const bool cin = (c0 == k0);   // column tile == pivot columns
+    const int cb = c0 + tc * MT;                 // first column of this thread
+    const bool cfull = cb + MT <= n;
+    T acc[MT][MT];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < MT; ++a) {
+        const int i = i0 + tr * MT + a;
+        const bool rowok = i < n && (i < k0 || i >= k0 + kb);
+        if (rowok && cfull) {
+            const V *src = reinterpret_cast<const V *>(A + (int64_t)i * ld + cb);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int i = i0 + tr * 4 + a, c = c0 + tc * 4 + b;
-            acc[a][b] = (!cin && i < n && c < n) ? A[(int64_t)i * ld + c] : T(0);
+            for (int v = 0; v < MT / VW; ++v) {
+                const V x = src[v];
+                const T *xs = reinterpret_cast<const T *>(&x);
+#pragma unroll
+                for (int w = 0; w < VW; ++w) acc[a][v * VW + w] = xs[w];
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < MT; ++b) acc[a][b] = (rowok && cb + b < n) ? A[(int64_t)i * ld + cb + b] : T(0);
         }
+#pragma unroll
+        for (int b = 0; b < MT; ++b)
+            if (cb + b >= k0 && cb + b < k0 + kb) acc[a][b] = T(0);
+    }
+#pragma unroll 2
     for (int t = 0; t < kb; ++t) {
-        T av[4], bv[4];
+        T av[MT], bv[MT];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) av[a] = sc[t * NB + tr * 4 + a];
+        for (int a = 0; a < MT; ++a) av[a] = sc[t * UT + tr * MT + a];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) bv[b] = sr[t * NB + tc * 4 + b];
+        for (int b = 0; b < MT; ++b) bv[b] = sr[t * UT + tc * MT + b];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < MT; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(-av[a], bv[b], acc[a][b]);
+            for (int b = 0; b < MT; ++b) acc[a][b] = fma(-av[a], bv[b], acc[a][b]);
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < MT; ++a) {
+        const int i = i0 + tr * MT + a;
+        if (i >= n || (i >= k0 && i < k0 + kb)) continue;
+        if (cfull) {
+            V *dst = reinterpret_cast<V *>(A + (int64_t)i * ld + cb);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int i = i0 + tr * 4 + a, c = c0 + tc * 4 + b;
-            if (i < n && c < n) A[(int64_t)i * ld + c] = acc[a][b];
+            for (int v = 0; v < MT / VW; ++v) {
+                V x;
+                T *xs = reinterpret_cast<T *>(&x);
+#pragma unroll
+                for (int w = 0; w < VW; ++w) xs[w] = acc[a][v * VW + w];
+                dst[v] = x;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < MT; ++b)
+                if (cb + b < n) A[(int64_t)i * ld + cb + b] = acc[a][b];
         }
+    }
     // fused inversion of the next pivot block
-    const int k1 = k0 + NB;
-    if (k1 < n && i0 == k1 && c0 == k1) {
+    if (k1 < n && i0 == (k1 / UT) * UT && c0 == (k1 / UT) * UT) {
         const int kb2 = min(NB, n - k1);
+        const int off = k1 - i0;   // 0, or 64 inside a 128 tile
         __syncthreads();
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < MT; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int r = tr * 4 + a, c = tc * 4 + b;
-                sc[r * NB + c] = (r < kb2 && c < kb2) ? acc[a][b] : T(0);
+            for (int b = 0; b < MT; ++b) {
+                const int r = tr * MT + a - off, c = tc * MT + b - off;
+                if (r >= 0 && r < NB && c >= 0 && c < NB)
+                    sc[r * NB + c] = (r < kb2 && c < kb2) ? acc[a][b] : T(0);
             }
         __syncthreads();
         const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
@@ -441,13 +487,17 @@ static int gj_invert_b(const Batch<T> &B, hipStream_t s)
 {
     int nmax = 0;
     for (int z = 0; z < B.nj; ++z) nmax = std::max(nmax, B.j[z].n);
-    const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
+    const int ut = B.ut == 128 ? 128 : 64;
+    const unsigned nt = (unsigned)((nmax + ut - 1) / ut);
     const unsigned np = (unsigned)((nmax + RP - 1) / RP);
     const unsigned nz = (unsigned)B.nj;
     hipLaunchKernelGGL(k_gj_diag_b<T>, dim3(1, 1, nz), dim3(256), 0, s, B, 0);
     for (int k0 = 0; k0 < nmax; k0 += NB) {
         hipLaunchKernelGGL(k_gj_rowpanel_b<T>, dim3(np, 2, nz), dim3(256), 0, s, B, k0);
-        hipLaunchKernelGGL(k_gj_update_b<T>, dim3(nt, nt, nz), dim3(256), 0, s, B, k0);
+        if (ut == 128)
+            hipLaunchKernelGGL((k_gj_update_b<T, 128>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
+        else
+            hipLaunchKernelGGL((k_gj_update_b<T, 64>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
     }
     V2D_LAUNCH();
     return 0;
@@ -458,6 +508,7 @@ int gj_invert(T *A, int n, int ld, T *colk, T *dinv, hipStream_t s)
 {
     Batch<T> B{};
     B.nj = 1;
+    B.ut = 64;
     B.j[0] = Job<T>{A, n, ld, 0, 0, colk, dinv, nullptr};
     return gj_invert_b(B, s);
 }
@@ -534,6 +585,7 @@ static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, b
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
 {
+    const int ut = W.update_tile;
     const int nlev = P.L.nlev;
     const int M = (nlev - 1) / 2;            // meeting level
     const int nbot = M, ntop = nlev - 1 - M;
@@ -556,6 +608,7 @@ int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStr
             ++nj;
         }
         B.nj = nj;
+        B.ut = ut;
         if (build_and_invert(P, B, val, schur, srcp, W, s)) return -1;
     }
     Batch<T> B{};
@@ -563,6 +616,7 @@ int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStr
     int srcp[2][2] = {{M - 1, M + 1}, {0, 0}};
     if (!schur[0][0]) { schur[0][0] = schur[0][1]; srcp[0][0] = srcp[0][1]; schur[0][1] = false; }
     B.nj = 1;
+    B.ut = ut;
     B.j[0] = make_job(P, W, M, 0);
     return build_and_invert(P, B, val, schur, srcp, W, s);
 }

@@ -114,6 +114,7 @@ struct DirectWork {
     T *colk = nullptr;   // Gauss-Jordan column panels, 2 x (nmax x 64)
     T *dinv = nullptr;   // pivot-block inverses, 2 x (64 x 64)
     T *z = nullptr;      // level-ordered 2N work vector
+    int update_tile = 64;   // Gauss-Jordan update tile edge (64 or 128)
 };
 
 template <typename T>

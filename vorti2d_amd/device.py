@@ -76,6 +76,24 @@ class DeviceContext:
         check(lib.v2d_ctx_set_solver(self._h, int(refactor_every), float(gmres_rtol),
                                      int(gmres_max_iter)))
 
+    def set_update_tile(self, tile: int):
+        """Tile edge (64 or 128) of the block-LU update kernel."""
+        check(lib.v2d_ctx_set_update_tile(self._h, int(tile)))
+
+    def autotune_update_tile(self, re: float, tiles=(64, 128)) -> dict:
+        """Time one block-LU factorisation of the current state's Jacobian with
+        each update tile edge and keep the fastest; returns {tile: seconds}."""
+        import time
+        self.assemble(re)
+        times = {}
+        for t in tiles:
+            self.set_update_tile(t)
+            t0 = time.perf_counter()
+            self.factor()
+            times[t] = time.perf_counter() - t0
+        self.set_update_tile(min(times, key=times.get))
+        return times
+
     # ----------------------------------------------------------- system ----
     def assemble(self, re, invdtau=0.0, inv2dt=0.0, urot=0.0):
         check(lib.v2d_ctx_assemble(self._h, float(re), float(invdtau), float(inv2dt), float(urot)))
