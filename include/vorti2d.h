@@ -113,10 +113,19 @@ int v2d_ctx_init_state(v2d_ctx *ctx);
  * refactor_every Newton steps (0 = only when GMRES needs more than
  * gmres_max_iter iterations), inner relative tolerance, iteration cap. */
 int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int gmres_max_iter);
-/* Tile edge (64 or 128) of the block-LU Gauss-Jordan update kernel: a tuning
- * knob with no counterpart in the reference (the results do not depend on it
- * beyond floating-point rounding order). */
-int v2d_ctx_set_update_tile(v2d_ctx *ctx, int tile);
+/* Kernel of the block-LU Gauss-Jordan update: VALU on 64- or 128-wide tiles,
+ * or the matrix cores (MFMA 16x16x4, FP32 or FP64) on 64-wide tiles (the
+ * default).  A tuning knob with no counterpart in the reference: results do not
+ * depend on it beyond floating-point rounding order.  Drops a stored
+ * factorisation. */
+enum { V2D_UPDATE_VALU64 = 0, V2D_UPDATE_VALU128 = 1, V2D_UPDATE_MFMA64 = 2 };
+int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind);
+/* Factorise the last assembled Jacobian once with every Gauss-Jordan update
+ * launch bracketed by HIP events on the context's stream (no counterpart in the
+ * reference; measurement only).  out[5] = {update launches, summed update
+ * kernel milliseconds, algorithmic flops of those launches, algorithmic bytes
+ * of those launches, wall milliseconds of the whole factorisation}. */
+int v2d_ctx_profile_factor(v2d_ctx *ctx, double *out);
 
 /* Device assembly of the Jacobian values and the FP64 residual-form RHS for
  * the current state (the scalars of assemble_coo). */

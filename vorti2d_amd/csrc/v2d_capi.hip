@@ -2,6 +2,7 @@
 // solver context, the mixed-precision GMRES driver and the Newton step.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -104,8 +105,10 @@ struct Ctx {
     bool factored = false;
     int since_factor = 0;
     double last_re = 0, last_invdtau = 0, last_inv2dt = 0, last_urot = 0;
-    // timers
-    hipEvent_t ev[4];
+    // timers: ev[0..3] bracket a Newton step (start, assembled, preconditioner
+    // ready, solved); ev[4..5] bracket the factorisation made inside it, if any
+    hipEvent_t ev[6];
+    bool fac_in_step = false;
     double t_asm = 0, t_fac = 0, t_sol = 0, t_tot = 0;
     int64_t n_steps = 0, n_fac = 0, n_lin = 0;
     std::vector<void *> allocs;
@@ -531,12 +534,14 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->wd.sinv = c->alloc<double>(c->pat.sinv_elems, rc);
         c->wd.tmat = c->alloc<double>(2 * nm * c->pat.ldmax, rc);
         c->wd.colk = c->alloc<double>(2 * nm * 64, rc);
+        c->wd.rowk = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
         c->wd.dinv = c->alloc<double>(2 * 64 * 64, rc);
         c->wd.z = c->alloc<double>(2 * N, rc);
     } else {
         c->wf.sinv = c->alloc<float>(c->pat.sinv_elems, rc);
         c->wf.tmat = c->alloc<float>(2 * nm * c->pat.ldmax, rc);
         c->wf.colk = c->alloc<float>(2 * nm * 64, rc);
+        c->wf.rowk = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
         c->wf.dinv = c->alloc<float>(2 * 64 * 64, rc);
         c->wf.z = c->alloc<float>(2 * N, rc);
         c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);
@@ -579,6 +584,8 @@ int v2d_ctx_set_state(v2d_ctx *ctx, const double *psi, const double *ome, const 
     V2D_HIP(hipStreamSynchronize(c->s));
     for (int q = 0; q < 4; ++q)
         if (src[q]) V2D_HIP(hipMemcpy(dst[q], src[q], sizeof(double) * c->N, hipMemcpyHostToDevice));
+    // a new state makes the stored block LU (of an earlier Jacobian) stale
+    if (psi || ome) c->factored = false;
     return 0;
 }
 
@@ -627,24 +634,48 @@ static int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double
 {
     const AsmArgs a = c->args(re, invdtau, inv2dt, urot);
     c->last_re = re; c->last_invdtau = invdtau; c->last_inv2dt = inv2dt; c->last_urot = urot;
+    const AsmConsts k = asm_consts(re, c->dksi, c->deta);
     if (c->prec == V2D_FP64)
-        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_for(c->N)), dim3(256), 0, c->s, a, c->flags,
+        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_for(c->N)), dim3(256), 0, c->s, a, k, c->flags,
                            static_cast<double *>(c->val), c->rhs);
     else
-        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_for(c->N)), dim3(256), 0, c->s, a, c->flags,
+        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_for(c->N)), dim3(256), 0, c->s, a, k, c->flags,
                            static_cast<float *>(c->val), c->rhs);
     V2D_LAUNCH();
     return 0;
 }
 
-int v2d_ctx_set_update_tile(v2d_ctx *ctx, int tile)
+int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind)
 {
-    if (tile != 64 && tile != 128) {
-        set_error("update tile must be 64 or 128");
+    if (kind != V2D_UPDATE_VALU64 && kind != V2D_UPDATE_VALU128 && kind != V2D_UPDATE_MFMA64) {
+        set_error("update kernel must be V2D_UPDATE_VALU64, V2D_UPDATE_VALU128 or V2D_UPDATE_MFMA64");
         return -1;
     }
-    ctx->c->wd.update_tile = tile;
-    ctx->c->wf.update_tile = tile;
+    ctx->c->wd.update_kind = kind;
+    ctx->c->wf.update_kind = kind;
+    ctx->c->factored = false;   // the panels of a stored factorisation are kernel-specific
+    return 0;
+}
+
+static int ctx_factor(Ctx *c);
+
+int v2d_ctx_profile_factor(v2d_ctx *ctx, double *out)
+{
+    Ctx *c = ctx->c.get();
+    LaunchProbe probe;
+    c->wd.probe = c->wf.probe = &probe;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = ctx_factor(c);
+    c->wd.probe = c->wf.probe = nullptr;
+    if (rc) return -1;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    out[0] = (double)probe.launches;
+    out[1] = probe.ms;
+    out[2] = probe.flops;
+    out[3] = probe.bytes;
+    out[4] = wall;
     return 0;
 }
 
@@ -759,11 +790,14 @@ int v2d_ctx_spmv(v2d_ctx *ctx, const double *x, double *y)
 static int ctx_factor(Ctx *c)
 {
     int rc;
+    V2D_HIP(hipEventRecord(c->ev[4], c->s));
     if (c->prec == V2D_FP64)
         rc = direct_factor<double>(c->pat, c->wd, static_cast<const double *>(c->val), c->s);
     else
         rc = direct_factor<float>(c->pat, c->wf, static_cast<const float *>(c->val), c->s);
     if (rc) return rc;
+    V2D_HIP(hipEventRecord(c->ev[5], c->s));
+    c->fac_in_step = true;
     c->factored = true;
     c->since_factor = 0;
     ++c->n_fac;
@@ -819,6 +853,7 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
 {
     Ctx *c = ctx->c.get();
     const int64_t N = c->N, n = 2 * N;
+    c->fac_in_step = false;
     V2D_HIP(hipEventRecord(c->ev[0], c->s));
     if (ctx_assemble(c, re, invdtau, inv2dt, urot)) return -1;
     V2D_HIP(hipEventRecord(c->ev[1], c->s));
@@ -882,8 +917,9 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
     if (dnorm2(c, c->delta, n, &dn)) return -1;
     float a_ms = 0.f, t_ms = 0.f;
     V2D_HIP(hipEventElapsedTime(&a_ms, c->ev[0], c->ev[1]));
-    V2D_HIP(hipEventElapsedTime(&fac_ms, c->ev[1], c->ev[2]));
-    V2D_HIP(hipEventElapsedTime(&sol_ms, c->ev[2], c->ev[3]));
+    if (c->fac_in_step) V2D_HIP(hipEventElapsedTime(&fac_ms, c->ev[4], c->ev[5]));
+    V2D_HIP(hipEventElapsedTime(&sol_ms, c->ev[1], c->ev[3]));
+    sol_ms -= fac_ms;   // the factorisation may come before or after a GMRES attempt
     V2D_HIP(hipEventElapsedTime(&t_ms, c->ev[0], c->ev[3]));
     c->t_asm += a_ms; c->t_fac += fac_ms; c->t_sol += sol_ms; c->t_tot += t_ms;
     ++c->n_steps;

@@ -141,6 +141,7 @@ struct Job {
     int n, ld, l;          // size, row stride, level
     int64_t voff;          // offset of the level in level-ordered vectors / CSR rows
     T *colk, *dinv, *tmat; // per-job scratch
+    T *rowk;               // transposed pivot-row panel (MFMA update only, else null)
 };
 
 // One Schur-update source: the neighbouring level's inverse on the side the
@@ -162,7 +163,7 @@ struct Batch {
     Job<T> j[2];
     Src<T> src[2];
     int nj;
-    int ut;   // Gauss-Jordan update tile edge: 64 or 128
+    int kind;   // Gauss-Jordan update kernel (UpdateKind)
 };
 
 // ------------------------------------------------------- level build ------
@@ -299,6 +300,11 @@ __global__ void __launch_bounds__(256) k_gj_diag_b(Batch<T> B, int k0)
     gj_regs(a, kb, prow, pcol, scale, J.dinv);
 }
 
+// 16-byte vectors of T
+template <typename T> struct Vec8;
+template <> struct Vec8<float> { using type = float4; static constexpr int w = 4; };
+template <> struct Vec8<double> { using type = double2; static constexpr int w = 2; };
+
 // blockIdx.y == 0: pivot rows  A[K, c] <- Dinv A[K, c]  (A[K, K] <- Dinv) for a
 //                  16-column tile (short per-block work: this is on the
 //                  critical path of every Gauss-Jordan step)
@@ -354,6 +360,19 @@ __global__ void __launch_bounds__(256) k_gj_rowpanel_b(Batch<T> B, int k0)
         const int t = tr * 4 + a;
         if (t < kb && col < n) A[(int64_t)(k0 + t) * ld + col] = acc[a];
     }
+    if (J.rowk && col < n) {   // rowk[c][t] = A[k0+t][c], zero past kb
+        typename Vec8<T>::type *dst =
+            reinterpret_cast<typename Vec8<T>::type *>(J.rowk + (int64_t)col * NB + tr * 4);
+        constexpr int VW = Vec8<T>::w;
+#pragma unroll
+        for (int v = 0; v < 4 / VW; ++v) {
+            typename Vec8<T>::type x;
+            T *xs = reinterpret_cast<T *>(&x);
+#pragma unroll
+            for (int w = 0; w < VW; ++w) xs[w] = (tr * 4 + v * VW + w < kb) ? acc[v * VW + w] : T(0);
+            dst[v] = x;
+        }
+    }
 }
 
 // A[i][c] = (c in K ? 0 : A[i][c]) - sum_t colk[i][t] A[k0+t][c]   for i outside K,
@@ -362,9 +381,6 @@ __global__ void __launch_bounds__(256) k_gj_rowpanel_b(Batch<T> B, int k0)
 // grid with the jobs interleaved: blocks 0 and 1 are the tiles holding each
 // job's next pivot block (k0+64, k0+64); they are dispatched first and also
 // invert that block into dinv, so the 64-pivot chain overlaps the bulk update.
-template <typename T> struct Vec8;
-template <> struct Vec8<float> { using type = float4; static constexpr int w = 4; };
-template <> struct Vec8<double> { using type = double2; static constexpr int w = 2; };
 
 template <typename T, int UT>
 __global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
@@ -481,39 +497,252 @@ __global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
     }
 }
 
+// MFMA form of the same update for 64 x 64 output tiles: 2 x 2 waves of
+// 32 x 32, each a 2 x 2 array of 16x16x4 matrix-core blocks.  The product
+// colk (rows x 64) * pivot rows (64 x cols) needs no LDS staging: lane
+// (li = lane % 16, lk = lane / 16) supplies A[li][t] and B[t][li] for
+// t = 16 lk + s in the s-th instruction, so the four lane groups carry four
+// distinct t per instruction and 16 instructions cover K; both operands are
+// then contiguous 16-element runs, colk[r][16 lk ..] and rowk[c][16 lk ..]
+// (rowk = the pivot rows transposed, written by the row-panel kernel).
+// FP32 issues the product transposed (pivot-row operand first), which puts
+// four consecutive columns of one row in each lane's accumulator: C moves as
+// 16-byte vectors.  FP64 keeps the direct form, where accumulator element v of
+// lane l holds row 4 v + l / 16, column l % 16 of its block.  The product
+// starts from zero and the old tile is added at the end, so its load overlaps
+// the matrix-core chain.  Rows of K form exactly the tile i0 == k0 (64-aligned),
+// which is skipped, and columns of K exactly the tile c0 == k0, whose old
+// values are taken as zero; operand entries with t >= kb are zeroed in
+// registers; rows past n read a valid row and are not stored; columns past n
+// only feed unstored outputs.
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+    using C = float __attribute__((ext_vector_type(4)));
+    static constexpr bool kSwap = true;
+    static __device__ __forceinline__ C op(float a, float b, C c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, c, 0, 0, 0);
+    }
+    // (row, column) of accumulator element v inside a 16 x 16 block
+    static __device__ __forceinline__ int row(int li, int lk, int v) { return li; }
+    static __device__ __forceinline__ int col(int li, int lk, int v) { return 4 * lk + v; }
+};
+template <> struct Mfma<double> {
+    using C = double __attribute__((ext_vector_type(4)));
+    static constexpr bool kSwap = false;
+    static __device__ __forceinline__ C op(double a, double b, C c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int li, int lk, int v) { return 4 * v + lk; }
+    static __device__ __forceinline__ int col(int li, int lk, int v) { return li; }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
+{
+    using M = Mfma<T>;
+    using C = typename M::C;
+    using V = typename Vec8<T>::type;
+    constexpr int VW = Vec8<T>::w;
+    __shared__ T sc[NB * NB];   // next pivot block (fused inversion only)
+    __shared__ T sx[5 * NB];    // gj_regs scratch
+    const int z = blockIdx.x % B.nj;
+    const Job<T> &J = B.j[z];
+    T *__restrict__ A = J.S;
+    const int n = J.n, ld = J.ld;
+    if (k0 >= n) return;
+    const int kb = min(NB, n - k0);
+    const int nt = (n + NB - 1) / NB;
+    int lin = blockIdx.x / B.nj;
+    if (lin >= nt * nt) return;
+    const int k1 = k0 + NB;
+    if (k1 < n) {
+        const int piv = (k1 / NB) * nt + k1 / NB;
+        if (lin == 0) lin = piv;
+        else if (lin <= piv) lin -= 1;
+    }
+    const int i0 = (lin / nt) * NB, c0 = (lin % nt) * NB;
+    if (i0 == k0) return;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+    const int rw = i0 + (w >> 1) * 32, cw = c0 + (w & 1) * 32;
+    const bool colK = c0 == k0;            // the pivot columns: old values are zero
+    const bool cfull = c0 + NB <= n;       // no column of the tile past n
+    // operands
+    const T *pa[2], *pb[2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) pa[bi] = J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 16 * lk;
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) pb[bj] = J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 16 * lk;
+    // old tile (consumed after the product)
+    C old[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            if (M::kSwap && cfull && !colK) {
+                const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                if (r < n) {
+                    const float4 x = *reinterpret_cast<const float4 *>(A + (int64_t)r * ld + c);
+                    old[bi][bj][0] = x.x; old[bi][bj][1] = x.y; old[bi][bj][2] = x.z; old[bi][bj][3] = x.w;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) old[bi][bj][v] = T(0);
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
+                    old[bi][bj][v] = (!colK && r < n && c < n) ? A[(int64_t)r * ld + c] : T(0);
+                }
+            }
+        }
+    C acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
+    const bool tail = kb < NB;
+#pragma unroll
+    for (int h = 0; h < 16 / VW; ++h) {
+        T av[2][VW], bv[2][VW];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const V x = reinterpret_cast<const V *>(pa[bi])[h];
+#pragma unroll
+            for (int e = 0; e < VW; ++e) av[bi][e] = -reinterpret_cast<const T *>(&x)[e];
+        }
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const V x = reinterpret_cast<const V *>(pb[bj])[h];
+#pragma unroll
+            for (int e = 0; e < VW; ++e) bv[bj][e] = reinterpret_cast<const T *>(&x)[e];
+        }
+        if (tail) {
+#pragma unroll
+            for (int e = 0; e < VW; ++e)
+                if (16 * lk + h * VW + e >= kb) av[0][e] = av[1][e] = T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < VW; ++e)
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
+            if (M::kSwap && cfull) {
+                const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                if (r < n) {
+                    float4 x;
+                    x.x = acc[bi][bj][0]; x.y = acc[bi][bj][1]; x.z = acc[bi][bj][2]; x.w = acc[bi][bj][3];
+                    *reinterpret_cast<float4 *>(A + (int64_t)r * ld + c) = x;
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
+                    if (r < n && c < n) A[(int64_t)r * ld + c] = acc[bi][bj][v];
+                }
+            }
+        }
+    // fused inversion of the next pivot block
+    if (k1 < n && i0 == k1 && c0 == k1) {
+        const int kb2 = min(NB, n - k1);
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (w >> 1) * 32 + bi * 16 + M::row(li, lk, v);
+                    const int c = (w & 1) * 32 + bj * 16 + M::col(li, lk, v);
+                    sc[r * NB + c] = (r < kb2 && c < kb2) ? acc[bi][bj][v] : T(0);
+                }
+        __syncthreads();
+        const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+        T g[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) g[q] = sc[(rg * 16 + q) * NB + c];
+        gj_regs(g, kb2, sx, sx + 2 * NB, sx + 4 * NB, J.dinv);
+    }
+}
+
+// ------------------------------------------------------ launch probe ------
+LaunchProbe::~LaunchProbe()
+{
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+}
+
+static int probe_flush(LaunchProbe &p)
+{
+    if (!p.used) return 0;
+    V2D_HIP(hipEventSynchronize(p.ev[2 * p.used - 1]));
+    for (int i = 0; i < p.used; ++i) {
+        float ms = 0.f;
+        V2D_HIP(hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]));
+        p.ms += ms;
+    }
+    p.used = 0;
+    return 0;
+}
+
+template <typename T>
+static int probe_mark(LaunchProbe *p, const Batch<T> &B, int k0, bool begin, hipStream_t s)
+{
+    if (!p) return 0;
+    if (begin) {
+        if (p->used == LaunchProbe::kPairs && probe_flush(*p)) return -1;
+        for (int q = 0; q < 2; ++q)
+            if (!p->ev[2 * p->used + q]) V2D_HIP(hipEventCreate(&p->ev[2 * p->used + q]));
+        V2D_HIP(hipEventRecord(p->ev[2 * p->used], s));
+        return 0;
+    }
+    V2D_HIP(hipEventRecord(p->ev[2 * p->used + 1], s));
+    ++p->used;
+    ++p->launches;
+    for (int z = 0; z < B.nj; ++z) {
+        const double n = B.j[z].n;
+        if (k0 >= B.j[z].n) continue;
+        const double kb = std::min(NB, B.j[z].n - k0);
+        p->flops += 2.0 * (n - kb) * n * kb;
+        p->bytes += sizeof(T) * (2.0 * (n - kb) * n + (n - kb) * kb + kb * n);
+    }
+    return 0;
+}
+
 // In-place Gauss-Jordan inverse of the batch's dense matrices.
 template <typename T>
-static int gj_invert_b(const Batch<T> &B, hipStream_t s)
+static int gj_invert_b(const Batch<T> &B, LaunchProbe *probe, hipStream_t s)
 {
     int nmax = 0;
     for (int z = 0; z < B.nj; ++z) nmax = std::max(nmax, B.j[z].n);
-    const int ut = B.ut == 128 ? 128 : 64;
+    const int ut = B.kind == kUpdateValu128 ? 128 : 64;
     const unsigned nt = (unsigned)((nmax + ut - 1) / ut);
     const unsigned np = (unsigned)((nmax + RP - 1) / RP);
     const unsigned nz = (unsigned)B.nj;
     hipLaunchKernelGGL(k_gj_diag_b<T>, dim3(1, 1, nz), dim3(256), 0, s, B, 0);
     for (int k0 = 0; k0 < nmax; k0 += NB) {
         hipLaunchKernelGGL(k_gj_rowpanel_b<T>, dim3(np, 2, nz), dim3(256), 0, s, B, k0);
-        if (ut == 128)
+        if (probe_mark(probe, B, k0, true, s)) return -1;
+        if (B.kind == kUpdateMfma64)
+            hipLaunchKernelGGL(k_gj_update_mfma_b<T>, dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
+        else if (ut == 128)
             hipLaunchKernelGGL((k_gj_update_b<T, 128>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
         else
             hipLaunchKernelGGL((k_gj_update_b<T, 64>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
+        if (probe_mark(probe, B, k0, false, s)) return -1;
     }
     V2D_LAUNCH();
     return 0;
 }
-
-template <typename T>
-int gj_invert(T *A, int n, int ld, T *colk, T *dinv, hipStream_t s)
-{
-    Batch<T> B{};
-    B.nj = 1;
-    B.ut = 64;
-    B.j[0] = Job<T>{A, n, ld, 0, 0, colk, dinv, nullptr};
-    return gj_invert_b(B, s);
-}
-template int gj_invert<double>(double *, int, int, double *, double *, hipStream_t);
-template int gj_invert<float>(float *, int, int, float *, float *, hipStream_t);
 
 // ------------------------------------------------------------ factor ------
 // Twisted block factorisation: the bottom chain eliminates levels 0..M-1 from
@@ -532,6 +761,7 @@ static Job<T> make_job(const DirectPattern &P, DirectWork<T> &W, int l, int slot
     J.colk = W.colk + (int64_t)slot * P.nmax * NB;
     J.dinv = W.dinv + slot * NB * NB;
     J.tmat = W.tmat + (int64_t)slot * P.nmax * P.ldmax;
+    J.rowk = W.update_kind == kUpdateMfma64 ? W.rowk + (int64_t)slot * P.ldmax * NB : nullptr;
     return J;
 }
 
@@ -579,13 +809,13 @@ static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, b
         hipLaunchKernelGGL(k_schur_S_b<T>, dim3(grid_for(nmax), nmax, nj), dim3(256), 0, s, Bk, val);
     }
     V2D_LAUNCH();
-    return gj_invert_b(B, s);
+    return gj_invert_b(B, W.probe, s);
 }
 
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
 {
-    const int ut = W.update_tile;
+    const int kind = W.update_kind;
     const int nlev = P.L.nlev;
     const int M = (nlev - 1) / 2;            // meeting level
     const int nbot = M, ntop = nlev - 1 - M;
@@ -608,7 +838,7 @@ int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStr
             ++nj;
         }
         B.nj = nj;
-        B.ut = ut;
+        B.kind = kind;
         if (build_and_invert(P, B, val, schur, srcp, W, s)) return -1;
     }
     Batch<T> B{};
@@ -616,9 +846,10 @@ int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStr
     int srcp[2][2] = {{M - 1, M + 1}, {0, 0}};
     if (!schur[0][0]) { schur[0][0] = schur[0][1]; srcp[0][0] = srcp[0][1]; schur[0][1] = false; }
     B.nj = 1;
-    B.ut = ut;
+    B.kind = kind;
     B.j[0] = make_job(P, W, M, 0);
-    return build_and_invert(P, B, val, schur, srcp, W, s);
+    if (build_and_invert(P, B, val, schur, srcp, W, s)) return -1;
+    return W.probe ? probe_flush(*W.probe) : 0;
 }
 template int direct_factor<double>(const DirectPattern &, DirectWork<double> &, const double *, hipStream_t);
 template int direct_factor<float>(const DirectPattern &, DirectWork<float> &, const float *, hipStream_t);
@@ -723,15 +954,18 @@ __global__ void __launch_bounds__(256) k_gemv_b(GemvBatch<T> B, Levels L, int64_
     if (r >= n) return;
     const V *row = reinterpret_cast<const V *>(J.S + (int64_t)r * ld);
     const int nv = (n + W - 1) / W;
-    double acc0 = 0.0, acc1 = 0.0;
-    int c = lane;
-    for (; c + 64 < nv; c += 128) {
-        const V x0 = row[c], x1 = row[c + 64];
-        acc0 += dot16<T>(x0, vs + c * W);
-        acc1 += dot16<T>(x1, vs + (c + 64) * W);
+    // four independent 16-byte loads in flight per lane
+    double acc4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = lane; c < nv; c += 256) {
+        V x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c + 64 * u < nv) x[u] = row[c + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c + 64 * u < nv) acc4[u] += dot16<T>(x[u], vs + (c + 64 * u) * W);
     }
-    if (c < nv) acc0 += dot16<T>(row[c], vs + c * W);
-    const double acc = wsum(acc0 + acc1);
+    const double acc = wsum((acc4[0] + acc4[1]) + (acc4[2] + acc4[3]));
     if (lane == 0) {
         const double b = J.base ? (double)J.base[r] : 0.0;
         const T o = (T)(b + sgn * acc);

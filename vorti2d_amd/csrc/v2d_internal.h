@@ -22,13 +22,36 @@ struct AsmArgs {
     const double *psi, *ome, *omeold, *omeoldold;
 };
 
+// Reciprocal launch constants of the production assembly.
+struct AsmConsts {
+    double ire, idk2, ide2, i2dk, i2de, i4dkde, ide;
+    double c_al, c_bt, c_p, c_q, c_g;   // 1/(re dk^2), 1/(re de^2), 1/(2 re dk), 1/(2 re de), 2/(4 re dk de)
+};
+inline AsmConsts asm_consts(double re, double dk, double de)
+{
+    AsmConsts c;
+    c.ire = 1.0 / re;
+    c.idk2 = 1.0 / (dk * dk);
+    c.ide2 = 1.0 / (de * de);
+    c.i2dk = 1.0 / (2.0 * dk);
+    c.i2de = 1.0 / (2.0 * de);
+    c.i4dkde = 1.0 / (4.0 * de * dk);
+    c.ide = 1.0 / de;
+    c.c_al = 1.0 / (re * (dk * dk));
+    c.c_bt = 1.0 / (re * (de * de));
+    c.c_p = 1.0 / (2.0 * re * dk);
+    c.c_q = 1.0 / (2.0 * re * de);
+    c.c_g = 2.0 / (4.0 * re * dk * de);
+    return c;
+}
+
 // ------------------------------------------------------------ kernels ------
 __global__ void k_metrics(int imax, int jmax, double dksi, double deta, const double *x,
                           const double *y, double *out);
 __global__ void k_metrics_local(int gxm, int jmax, int il0, int il1, double dksi, double deta,
                                 const double *x, const double *y, double *out);
 template <typename T>
-__global__ void k_assemble(AsmArgs a, const uint8_t *flags, T *val, double *rhs);
+__global__ void k_assemble(AsmArgs a, AsmConsts c, const uint8_t *flags, T *val, double *rhs);
 __global__ void k_assemble_coo(AsmArgs a, const uint8_t *flags, const int *pre_psi,
                                const int *pre_ome, int npsi_out, int64_t r0, int64_t r1, int *ci,
                                int *cj, double *cv, double *bvec);
@@ -107,14 +130,34 @@ struct DirectPattern {
 int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags_host);
 void direct_free_pattern(DirectPattern &P);
 
+// Gauss-Jordan update kernels: VALU with 64- or 128-wide output tiles, or
+// MFMA 16x16x4 (FP32 / FP64 matrix cores) on 64-wide tiles.
+enum UpdateKind { kUpdateValu64 = 0, kUpdateValu128 = 1, kUpdateMfma64 = 2 };
+
+// Optional per-launch timing of the Gauss-Jordan update kernel: an event pair
+// brackets every launch on the factorisation stream; pairs are read back in
+// groups of kPairs.  flops / bytes are the algorithmic counts of the timed
+// launches (2 (n-kb) n kb flops per job; the updated rows read and written
+// once, both panels read once).
+struct LaunchProbe {
+    static constexpr int kPairs = 64;
+    hipEvent_t ev[2 * kPairs] = {};
+    int used = 0;
+    int64_t launches = 0;
+    double ms = 0.0, flops = 0.0, bytes = 0.0;
+    ~LaunchProbe();
+};
+
 template <typename T>
 struct DirectWork {
     T *sinv = nullptr;   // all dense inverses
     T *tmat = nullptr;   // Schur scratch, 2 x (nmax x ldmax) (one per elimination chain)
     T *colk = nullptr;   // Gauss-Jordan column panels, 2 x (nmax x 64)
+    T *rowk = nullptr;   // Gauss-Jordan pivot-row panels, transposed, 2 x (ldmax x 64)
     T *dinv = nullptr;   // pivot-block inverses, 2 x (64 x 64)
     T *z = nullptr;      // level-ordered 2N work vector
-    int update_tile = 64;   // Gauss-Jordan update tile edge (64 or 128)
+    int update_kind = kUpdateMfma64;
+    LaunchProbe *probe = nullptr;   // set to time the update launches
 };
 
 template <typename T>
@@ -122,9 +165,5 @@ int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStr
 template <typename T>
 int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
                  hipStream_t s);
-// Gauss-Jordan inverse of one dense n x n row-major matrix in place (exported
-// for tests of the dense kernel)
-template <typename T>
-int gj_invert(T *A, int n, int ld, T *colk, T *dinv, hipStream_t s);
 
 }  // namespace v2d

@@ -264,9 +264,14 @@ __device__ __forceinline__ NodeIx global_ix(int imax, int i, int j)
 }
 
 // Production assembly: both rows of every node, values into the slot planes
-// (cast to T), residual-form RHS in FP64 (field-blocked, 2N).
+// (cast to T), residual-form RHS in FP64 (field-blocked, 2N).  Same stencils as
+// psi_row / ome_row, with every division by a launch constant replaced by a
+// product with its reciprocal (AsmConsts, computed once on the host): a few
+// ulp from the reference's rounding, and no FP64 divide sequences in the
+// interior.  The parity COO kernels keep the reference's exact operations.
 template <typename T>
-__global__ void __launch_bounds__(256) k_assemble(AsmArgs a, const uint8_t *__restrict__ flags,
+__global__ void __launch_bounds__(256) k_assemble(AsmArgs a, AsmConsts c,
+                                                  const uint8_t *__restrict__ flags,
                                                   T *__restrict__ val, double *__restrict__ rhs)
 {
     const int imax = a.imax, jmax = a.jmax;
@@ -274,19 +279,98 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs a, const uint8_t *__re
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
     const int j = (int)(k / imax), i = (int)(k - (int64_t)j * imax);
-    const int jt = j == 0 ? 0 : (j == jmax - 1 ? 2 : 1);
-    const uint8_t fl = jt == 2 ? flags[i] : 0;
     const NodeIx s = global_ix(imax, i, j);
-    double v[13], b;
-    const int np = psi_row(a, jt, fl, s, v, b);
-    for (int q = 0; q < np; ++q) val[q * N + k] = (T)v[q];
-    rhs[k] = b;
-    const int no = ome_row(a, jt, fl, s, v, b);
-    for (int q = 0; q < no; ++q) val[(kPsiSlots + q) * N + k] = (T)v[q];
-    rhs[N + k] = b;
+    const double *P = a.psi, *O = a.ome;
+    if (j > 0 && j < jmax - 1) {
+        const double J = a.m[0][k], al = a.m[1][k], bt = a.m[2][k], g = a.m[3][k];
+        const double p = a.m[4][k], q = a.m[5][k];
+        const double Pk = P[k], Pe = P[s.e], Pw = P[s.w], Pn = P[s.n], Ps = P[s.s];
+        const double Ok = O[k], Oe = O[s.e], Ow = O[s.w], On = O[s.n], Os = O[s.s];
+        const double Px = P[s.ne] - P[s.nw] - P[s.se] + P[s.sw];
+        const double Ox = O[s.ne] - O[s.nw] - O[s.se] + O[s.sw];
+        const double ca = al * c.c_al, cb = bt * c.c_bt, cp = p * c.c_p, cq = q * c.c_q;
+        const double cg = g * c.c_g;
+        // psi row (vorti2d_core.f90:249-301)
+        val[0 * N + k] = (T)(a.invdtau + (2.0 * al * c.idk2 + 2.0 * bt * c.ide2) * c.ire);
+        val[1 * N + k] = (T)(-(ca + cp));
+        val[2 * N + k] = (T)(-ca + cp);
+        val[3 * N + k] = (T)(-(cb + cq));
+        val[4 * N + k] = (T)(-cb + cq);
+        val[5 * N + k] = (T)(-cg);
+        val[6 * N + k] = (T)cg;
+        val[7 * N + k] = (T)cg;
+        val[8 * N + k] = (T)(-cg);
+        val[9 * N + k] = (T)(-c.ire);
+        rhs[k] = c.ire * (al * ((Pe - 2.0 * Pk + Pw) * c.idk2) + 2.0 * g * (Px * c.i4dkde)
+                          + bt * ((Pn - 2.0 * Pk + Ps) * c.ide2) + p * ((Pe - Pw) * c.i2dk)
+                          + q * ((Pn - Ps) * c.i2de) + Ok);
+        // omega row (vorti2d_core.f90:302-379)
+        const double Aa = (Pn - Ps) * c.i2de, Bb = (Oe - Ow) * c.i2dk;
+        const double Cc = (Pe - Pw) * c.i2dk, Dd = (On - Os) * c.i2de;
+        const double ja = J * Aa * c.i2dk, jc = J * Cc * c.i2de;
+        const double jb = J * Bb * c.i2de, jd = J * Dd * c.i2dk;
+        T *vo = val + kPsiSlots * N;
+        vo[0 * N + k] = (T)(a.invdtau + 2.0 * ca + 2.0 * cb + 3.0 * a.inv2dt);
+        vo[1 * N + k] = (T)(ja - ca - cp);
+        vo[2 * N + k] = (T)(-ja - ca + cp);
+        vo[3 * N + k] = (T)(-jc - cb - cq);
+        vo[4 * N + k] = (T)(jc - cb + cq);
+        vo[5 * N + k] = (T)(-cg);
+        vo[6 * N + k] = (T)cg;
+        vo[7 * N + k] = (T)cg;
+        vo[8 * N + k] = (T)(-cg);
+        vo[9 * N + k] = (T)jb;
+        vo[10 * N + k] = (T)(-jb);
+        vo[11 * N + k] = (T)(-jd);
+        vo[12 * N + k] = (T)jd;
+        double hist = 0.0;   // the BDF2 history is read only for an unsteady step
+        if (a.inv2dt != 0.0) hist = (3.0 * Ok - 4.0 * a.omeold[k] + a.omeoldold[k]) * a.inv2dt;
+        rhs[N + k] = -(J * Aa * Bb) + (J * Cc * Dd)
+                     + c.ire * (al * ((Oe - 2.0 * Ok + Ow) * c.idk2) + 2.0 * g * (Ox * c.i4dkde)
+                                + bt * ((On - 2.0 * Ok + Os) * c.ide2) + p * ((Oe - Ow) * c.i2dk)
+                                + q * ((On - Os) * c.i2de))
+                     - hist;
+        return;
+    }
+    T *vo = val + kPsiSlots * N;
+    if (j == 0) {   // wall: psi Dirichlet, Thom vorticity
+        val[k] = (T)1.0;
+        rhs[k] = -P[k];
+        const double bt = a.m[2][k];
+        const double bw = bt * c.ide2;
+        vo[0 * N + k] = (T)(-1.0);
+        vo[1 * N + k] = (T)(3.5 * bw);
+        vo[2 * N + k] = (T)(-4.0 * bw);
+        vo[3 * N + k] = (T)(0.5 * bw);
+        const double xk = a.m[8][k], yk = a.m[9][k];
+        const double rad = sqrt(xk * xk + yk * yk);
+        const double ct = xk / rad, st = yk / rad;
+        rhs[N + k] = O[k] - 3.5 * bw * P[k] + 4.0 * bw * P[s.n2] - 0.5 * bw * P[s.n3]
+                     - ((a.m[5][k] - 3.0 * (bt * c.ide)) * (a.urot / (a.m[6][k] * ct + a.m[7][k] * st)));
+        return;
+    }
+    const uint8_t fl = flags[i];   // far field
+    if (fl & kOutPsi) {
+        val[0 * N + k] = (T)1.0;
+        val[1 * N + k] = (T)(-2.0);
+        val[2 * N + k] = (T)1.0;
+        rhs[k] = -(P[k] - 2.0 * P[s.s1] + P[s.s2]);
+    } else {
+        val[k] = (T)1.0;
+        rhs[k] = (a.ca * a.m[9][k] - a.sa * a.m[8][k]) - P[k];
+    }
+    if (fl & kOutOme) {
+        vo[0 * N + k] = (T)1.0;
+        vo[1 * N + k] = (T)(-4.0 / 3.0);
+        vo[2 * N + k] = (T)(1.0 / 3.0);
+        rhs[N + k] = (4.0 * O[s.s1] - O[s.s2]) * (1.0 / 3.0) - O[k];
+    } else {
+        vo[k] = (T)1.0;
+        rhs[N + k] = -O[k];
+    }
 }
-template __global__ void k_assemble<double>(AsmArgs, const uint8_t *, double *, double *);
-template __global__ void k_assemble<float>(AsmArgs, const uint8_t *, float *, double *);
+template __global__ void k_assemble<double>(AsmArgs, AsmConsts, const uint8_t *, double *, double *);
+template __global__ void k_assemble<float>(AsmArgs, AsmConsts, const uint8_t *, float *, double *);
 
 // Parity emission: rows [r0, r1) of the field-blocked system written in the
 // reference COO order (vorti2d_core.f90:247-381) with FP64 values.

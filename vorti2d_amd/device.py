@@ -13,6 +13,7 @@ import numpy as np
 from ._lib import check, f64c, lib, ptr
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
+UPDATE_KERNELS = {"valu64": 0, "valu128": 1, "mfma64": 2}
 
 
 class DeviceContext:
@@ -76,23 +77,33 @@ class DeviceContext:
         check(lib.v2d_ctx_set_solver(self._h, int(refactor_every), float(gmres_rtol),
                                      int(gmres_max_iter)))
 
-    def set_update_tile(self, tile: int):
-        """Tile edge (64 or 128) of the block-LU update kernel."""
-        check(lib.v2d_ctx_set_update_tile(self._h, int(tile)))
+    def set_update_kernel(self, kind: str | int):
+        """Gauss-Jordan update kernel of the block LU: "valu64", "valu128" or
+        "mfma64" (the default)."""
+        check(lib.v2d_ctx_set_update_kernel(self._h, UPDATE_KERNELS.get(kind, kind)))
 
-    def autotune_update_tile(self, re: float, tiles=(64, 128)) -> dict:
+    def autotune_update_kernel(self, re: float, kinds=("mfma64", "valu64")) -> dict:
         """Time one block-LU factorisation of the current state's Jacobian with
-        each update tile edge and keep the fastest; returns {tile: seconds}."""
+        each update kernel and keep the fastest; returns {kind: seconds}."""
         import time
         self.assemble(re)
         times = {}
-        for t in tiles:
-            self.set_update_tile(t)
+        for k in kinds:
+            self.set_update_kernel(k)
+            self.factor()   # first use of a kernel variant pays its load
             t0 = time.perf_counter()
             self.factor()
-            times[t] = time.perf_counter() - t0
-        self.set_update_tile(min(times, key=times.get))
+            times[k] = time.perf_counter() - t0
+        self.set_update_kernel(min(times, key=times.get))
         return times
+
+    def profile_factor(self) -> dict:
+        """One factorisation of the last assembly with the update launches
+        timed by HIP events (see v2d_ctx_profile_factor)."""
+        out = np.zeros(5)
+        check(lib.v2d_ctx_profile_factor(self._h, ptr(out)))
+        return dict(launches=int(out[0]), update_ms=float(out[1]), flops=float(out[2]),
+                    bytes=float(out[3]), factor_ms=float(out[4]))
 
     # ----------------------------------------------------------- system ----
     def assemble(self, re, invdtau=0.0, inv2dt=0.0, urot=0.0):
