@@ -221,6 +221,26 @@ def test_direct_solve_fp32_update_kernels(v, C, oracle_mod, imax, jmax, kind, ff
         assert err[upd] <= max(4.0 * err["valu64"], 1e-6), (upd, err)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_stored_factorisation_is_self_contained(v, C, oracle_mod, prec):
+    """A stored block LU keeps solving the system it was built from after later
+    assemblies overwrite the Jacobian (its solve sweeps must not read the new
+    inter-level couplings)."""
+    imax, jmax = 73, 41
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, "cyl", prec, 1)
+    ctx.assemble(40.0, 0.0, 2.5, 0.1)
+    ci, cj, cv, b = ctx.export_coo()
+    ctx.factor()
+    x1 = ctx.solve(b)
+    ctx.assemble(75.0, 0.3, 0.0, 0.0)          # different Jacobian, same state
+    x2 = ctx.solve(b)
+    assert np.array_equal(x1, x2)
+    if prec == "fp64":
+        N = imax * jmax
+        A = sp.csc_matrix((cv, (ci, cj)), shape=(2 * N, 2 * N))
+        assert np.linalg.norm(b - A @ x2) <= 1e-11 * np.linalg.norm(b)
+
+
 def test_mixed_newton_step_matches_fp64(v, C, oracle_mod):
     """The FP32 block LU alone is a preconditioner (FP32 accuracy); FP32 GMRES
     on it must reproduce the FP64 exact Newton update to the GMRES tolerance."""
@@ -410,3 +430,76 @@ def test_full_size_assembly_1025(v, C, oracle_mod):
     assert nnz == 24124400
     assert np.array_equal(ci, oi[:nnz]) and np.array_equal(cj, oj[:nnz])
     _check_production_values(cv, ov[:nnz], b, ob, "fp64")
+
+
+# ------------------------------------------------- unsteady shedding (C2) --
+def _unsteady_c2(v, prec, xg, yg, tmp, t_end):
+    cfg = v.Config(re=100.0, steady=False, dt_phys=0.2, t_start=0.0, t_end=t_end,
+                   rot_speed=0.5, rot_until=2.0, farfield_bc="outflow", pseudo_tol=1e-10,
+                   max_pseudo_iter=100, out_dir=os.path.join(tmp, prec), verbose=False,
+                   write_csv=False, write_xdmf=False, compute_forces=True, precision=prec)
+    s = v.Solver(cfg, mesh=(xg, yg)).run()
+    d = np.genfromtxt(os.path.join(tmp, prec, "forces.csv"), delimiter=",", names=True)
+    return s, d
+
+
+def test_unsteady_first_steps_match_oracle(v, C, oracle_mod):
+    """181x181, Re 100, BDF2 dt 0.2, rotation kick 0.5, outflow far field: the
+    first three physical steps of the device dual-time loop (FP64 and mixed)
+    equal the oracle's FP64 loop with exact LU solves."""
+    imax = jmax = 181
+    xg, yg = oracle_mod.generate_annulus(imax, jmax)
+    m = C.compute_metrics(1 / 180, 1 / 180, xg, yg)
+    N = imax * jmax
+    psi, ome = oracle_mod.initial_state(imax, jmax, m[8], m[9])
+    o1 = np.zeros(N)
+    o2 = np.zeros(N)
+    for step in range(3):                     # vorti2d/solver.py:169-190
+        r = oracle_mod.newton_solve(C, imax, jmax, m, 100.0, inv2dt=2.5, urot=0.5, ff_bc=1,
+                                    psi=psi, ome=ome, omeold=o1, omeoldold=o2, tol=1e-10)
+        psi, ome = r.psi, r.ome
+        o2, o1 = o1, ome.copy()
+    with tempfile.TemporaryDirectory() as tmp:
+        for prec in ("fp64", "mixed"):
+            say(f"unsteady first steps {prec}")
+            s, _ = _unsteady_c2(v, prec, xg, yg, tmp, t_end=0.4)
+            assert s.step == 3
+            assert np.abs(s.psi - psi).max() < 1e-8, prec
+            assert np.abs(s.ome - ome).max() < 1e-8, prec
+
+
+def test_unsteady_shedding_mixed_vs_fp64(v):
+    """181x181 cylinder O-grid, Re 100, BDF2 dual time with dt 0.2 to t = 80,
+    rotation kick 0.5 until t = 2, outflow far field.  The mixed path (FP32
+    Jacobian / block LU / GMRES under FP64 Newton refinement) against the plain
+    FP64 path: Cl(t), Cd(t) and the Strouhal number of the saturated cycle.
+    Shedding is an instability, so round-off level differences between the two
+    paths grow during the transient; the saturated cycle (amplitude, mean drag,
+    frequency) must agree closely, and St must sit near the reference's own
+    Re = 100 value 0.1535 (docs/verification.rst:181-204)."""
+    from lco import strouhal
+    xg, yg = v.generate_annulus(181, 181, 0.5, 50.0)
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for prec in ("fp64", "mixed"):
+            say(f"unsteady C2 {prec}")
+            s, d = _unsteady_c2(v, prec, xg, yg, tmp, t_end=80.0)
+            st = strouhal(d["t"], d["cl"])
+            i0 = st["i0"]
+            out[prec] = dict(t=d["t"], cl=d["cl"], cd=d["cd"], st=st,
+                             cd_mean=float(d["cd"][i0:].mean()))
+            say(f"  {prec}: St {st['st']:.4f} over {st.get('n_periods')} periods from "
+                f"t={d['t'][i0]:.1f}, Cl amplitude {st['amp']:.4f}, mean Cd "
+                f"{out[prec]['cd_mean']:.4f}")
+    f, mx = out["fp64"], out["mixed"]
+    assert len(f["t"]) == len(mx["t"]) == 401
+    dcl = np.abs(f["cl"] - mx["cl"])
+    dcd = np.abs(f["cd"] - mx["cd"])
+    say(f"  max |dCl| {dcl.max():.2e} (t <= 20: {dcl[f['t'] <= 20].max():.2e}), "
+        f"max |dCd| {dcd.max():.2e}, dSt {mx['st']['st'] - f['st']['st']:+.2e}")
+    assert dcl[f["t"] <= 20].max() < 1e-6            # before the instability has grown
+    assert f["st"]["n_periods"] >= 3
+    assert abs(mx["st"]["st"] - f["st"]["st"]) < 2e-3
+    assert abs(mx["st"]["amp"] - f["st"]["amp"]) < 0.02 * f["st"]["amp"]
+    assert abs(mx["cd_mean"] - f["cd_mean"]) < 1e-3 * f["cd_mean"]
+    assert 0.14 < f["st"]["st"] < 0.17

@@ -91,7 +91,8 @@ struct Ctx {
     int npsi_out = 0;
     int64_t nnz = 0;
     std::vector<uint8_t> hflags;
-    void *val = nullptr;   // kSlots * N of T
+    void *val = nullptr;      // kSlots * N of T: the last assembled Jacobian
+    void *val_pc = nullptr;   // the values the stored block LU was built from
     DirectPattern pat;
     DirectWork<double> wd;
     DirectWork<float> wf;
@@ -172,7 +173,8 @@ static int gmres_f32(Ctx *c, int cap, int *iters, double *relres)
 {
     const int64_t n = 2 * c->N;
     const int m = cap;
-    const float *val = static_cast<const float *>(c->val);
+    const float *val = static_cast<const float *>(c->val);        // operator
+    const float *vpc = static_cast<const float *>(c->val_pc);     // preconditioner
     const unsigned nb = std::min<unsigned>(1024, grid_for(n));
     std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1);
     // v0 = b (unit norm already) -> gV[0]
@@ -182,7 +184,7 @@ static int gmres_f32(Ctx *c, int cap, int *iters, double *relres)
     double est = 1.0;
     for (; j < m; ++j) {
         float *vj = c->gV + j * n, *zj = c->gZ + j * n, *w = c->gV + (j + 1) * n;
-        if (direct_solve<float>(c->pat, c->wf, val, vj, zj, c->s)) return -1;
+        if (direct_solve<float>(c->pat, c->wf, vpc, vj, zj, c->s)) return -1;
         hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
                            c->jmax, c->flags, val, zj, w);
         V2D_LAUNCH();
@@ -529,6 +531,7 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     if (ctx_bc(c, ff_bc, ca, sa)) return -1;
     const size_t tsz = precision == V2D_FP64 ? sizeof(double) : sizeof(float);
     c->val = c->alloc<unsigned char>((int64_t)kSlots * N * tsz, rc);
+    c->val_pc = c->alloc<unsigned char>((int64_t)kSlots * N * tsz, rc);
     const int64_t nm = c->pat.nmax;
     if (precision == V2D_FP64) {
         c->wd.sinv = c->alloc<double>(c->pat.sinv_elems, rc);
@@ -791,10 +794,15 @@ static int ctx_factor(Ctx *c)
 {
     int rc;
     V2D_HIP(hipEventRecord(c->ev[4], c->s));
+    // The block LU is a self-contained operator: its solve sweeps use the
+    // inter-level couplings of the same Jacobian as its dense Schur inverses,
+    // so a stored factorisation stays consistent after later assemblies.
+    const size_t tsz = c->prec == V2D_FP64 ? sizeof(double) : sizeof(float);
+    V2D_HIP(hipMemcpyAsync(c->val_pc, c->val, (size_t)kSlots * c->N * tsz, hipMemcpyDeviceToDevice, c->s));
     if (c->prec == V2D_FP64)
-        rc = direct_factor<double>(c->pat, c->wd, static_cast<const double *>(c->val), c->s);
+        rc = direct_factor<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->s);
     else
-        rc = direct_factor<float>(c->pat, c->wf, static_cast<const float *>(c->val), c->s);
+        rc = direct_factor<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), c->s);
     if (rc) return rc;
     V2D_HIP(hipEventRecord(c->ev[5], c->s));
     c->fac_in_step = true;
@@ -826,7 +834,7 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
         V2D_ALLOC(db, sizeof(double) * n);
         V2D_ALLOC(dx, sizeof(double) * n);
         V2D_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
-        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val), db.as<double>(),
+        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), db.as<double>(),
                                  dx.as<double>(), c->s))
             return -1;
         V2D_HIP(hipStreamSynchronize(c->s));
@@ -839,7 +847,7 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
     V2D_ALLOC(db, sizeof(float) * n);
     V2D_ALLOC(dx, sizeof(float) * n);
     V2D_HIP(hipMemcpy(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    if (direct_solve<float>(c->pat, c->wf, static_cast<const float *>(c->val), db.as<float>(),
+    if (direct_solve<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), db.as<float>(),
                             dx.as<float>(), c->s))
         return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
@@ -869,7 +877,7 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
     if (c->prec == V2D_FP64) {
         if (ctx_factor(c)) return -1;
         V2D_HIP(hipEventRecord(c->ev[2], c->s));
-        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val), c->rhs, c->delta, c->s))
+        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->rhs, c->delta, c->s))
             return -1;
     } else {
         // Factorisation policy: refactor_every N > 0 refactors every N steps;

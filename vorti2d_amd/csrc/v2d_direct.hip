@@ -501,10 +501,12 @@ __global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
 // 32 x 32, each a 2 x 2 array of 16x16x4 matrix-core blocks.  The product
 // colk (rows x 64) * pivot rows (64 x cols) needs no LDS staging: lane
 // (li = lane % 16, lk = lane / 16) supplies A[li][t] and B[t][li] for
-// t = 16 lk + s in the s-th instruction, so the four lane groups carry four
-// distinct t per instruction and 16 instructions cover K; both operands are
-// then contiguous 16-element runs, colk[r][16 lk ..] and rowk[c][16 lk ..]
-// (rowk = the pivot rows transposed, written by the row-panel kernel).
+// t = 16 h + 4 lk + e in instruction (h, e), h, e = 0..3, so the four lane
+// groups carry four distinct t per instruction and the 16 instructions cover
+// K.  Each lane reads 4-element runs colk[r][16 h + 4 lk ..] and
+// rowk[c][16 h + 4 lk ..] (rowk = the pivot rows transposed, written by the
+// row-panel kernel): the four lanes of a row read 4 consecutive runs, so one
+// load instruction touches 16 contiguous 64-byte row segments.
 // FP32 issues the product transposed (pivot-row operand first), which puts
 // four consecutive columns of one row in each lane's accumulator: C moves as
 // 16-byte vectors.  FP64 keeps the direct form, where accumulator element v of
@@ -571,9 +573,9 @@ __global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
     // operands
     const T *pa[2], *pb[2];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi) pa[bi] = J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 16 * lk;
+    for (int bi = 0; bi < 2; ++bi) pa[bi] = J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk;
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) pb[bj] = J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 16 * lk;
+    for (int bj = 0; bj < 2; ++bj) pb[bj] = J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk;
     // old tile (consumed after the product)
     C old[2][2];
 #pragma unroll
@@ -606,27 +608,31 @@ __global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
             for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
     const bool tail = kb < NB;
 #pragma unroll
-    for (int h = 0; h < 16 / VW; ++h) {
-        T av[2][VW], bv[2][VW];
+    for (int h = 0; h < 4; ++h) {
+        T av[2][4], bv[2][4];
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-            const V x = reinterpret_cast<const V *>(pa[bi])[h];
+        for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-            for (int e = 0; e < VW; ++e) av[bi][e] = -reinterpret_cast<const T *>(&x)[e];
-        }
+            for (int u = 0; u < 4 / VW; ++u) {
+                const V x = reinterpret_cast<const V *>(pa[bi] + 16 * h)[u];
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const V x = reinterpret_cast<const V *>(pb[bj])[h];
+                for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = -reinterpret_cast<const T *>(&x)[e];
+            }
 #pragma unroll
-            for (int e = 0; e < VW; ++e) bv[bj][e] = reinterpret_cast<const T *>(&x)[e];
-        }
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int u = 0; u < 4 / VW; ++u) {
+                const V x = reinterpret_cast<const V *>(pb[bj] + 16 * h)[u];
+#pragma unroll
+                for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = reinterpret_cast<const T *>(&x)[e];
+            }
         if (tail) {
 #pragma unroll
-            for (int e = 0; e < VW; ++e)
-                if (16 * lk + h * VW + e >= kb) av[0][e] = av[1][e] = T(0);
+            for (int e = 0; e < 4; ++e)
+                if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
         }
 #pragma unroll
-        for (int e = 0; e < VW; ++e)
+        for (int e = 0; e < 4; ++e)
 #pragma unroll
             for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -862,7 +868,7 @@ __device__ __forceinline__ double wsum(double v)
     return v;
 }
 
-// Sparse right-hand side of a level:  y = [b_l] - sum_t list_t(r) . vec_t
+// Sparse right-hand side of a level:  y = [b_l] + sgn * sum_t list_t(r) . vec_t
 // (a term's vector is level-ordered, or the global field-blocked vector
 // addressed through the level map when glob is set).
 template <typename T>
@@ -877,37 +883,9 @@ struct RhsJob {
     int l, n, nt;
     const T *b;        // global right-hand side or null
     RhsTerm<T> t[2];
-    T *y;
-};
-template <typename T>
-struct RhsBatch {
-    RhsJob<T> j[2];
 };
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_rhs_b(RhsBatch<T> B, Levels L, int64_t N,
-                                               const T *__restrict__ val, double sgn)
-{
-    const RhsJob<T> &J = B.j[blockIdx.z];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= J.n) return;
-    double sum = 0.0;
-    for (int k = 0; k < J.nt; ++k) {
-        const RhsTerm<T> &t = J.t[k];
-        for (int e = t.ptr[r]; e < t.ptr[r + 1]; ++e) {
-            const int2 x = t.ent[e];
-            const double v = t.glob ? (double)t.vec[L.global(x.x, N, t.vlev)] : (double)t.vec[x.x];
-            sum += (double)val[x.y] * v;
-        }
-    }
-    const double b = J.b ? (double)J.b[L.global(r, N, J.l)] : 0.0;
-    J.y[r] = (T)(b + sgn * sum);
-}
-
-// out = base + sgn * S v for up to two levels: level-ordered output (glob = 0)
-// or scattered to the global vector (glob = 1).  One wavefront per row; rows
-// are padded to ld (a multiple of 64 elements) so every lane reads 16-byte
-// vectors, and the padding (zeros) is bounded by the vector length.
+// 16-byte vectors of T and their FP64 dot with an LDS vector.
 template <typename T> struct Vec16;
 template <> struct Vec16<float> { using type = float4; static constexpr int w = 4; };
 template <> struct Vec16<double> { using type = double2; static constexpr int w = 2; };
@@ -925,70 +903,110 @@ __device__ __forceinline__ double dot16<double>(const double2 &a, const double *
     return a.x * b[0] + a.y * b[1];
 }
 
+// One solve step of up to two levels, fused: every block first issues the
+// loads of its own S rows (two rows per wavefront, up to kSweepPre 16-byte
+// vectors per lane held in registers), then builds the whole sparse right-hand
+// side y of its level in LDS while those loads are in flight,
+//     y = [b_l] + sgn_rhs * sum_t list_t(r) . vec_t,
+// and finally forms out = [base] + sgn_mv * S y (rows past the prefetch are
+// streamed).  Output level-ordered (glob = 0) or scattered to the global vector.
+constexpr int kSweepRowsPerWave = 2;
+constexpr int kSweepPre = 9;
+
 template <typename T>
-struct GemvJob {
+struct SweepJob {
+    RhsJob<T> rhs;   // y (kept in LDS)
     const T *S;
-    int n, ld, l, glob;
-    const T *v, *base;
+    int ld, glob;
+    const T *base;
     T *out;
 };
 template <typename T>
-struct GemvBatch {
-    GemvJob<T> j[2];
+struct SweepBatch {
+    SweepJob<T> j[2];
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_gemv_b(GemvBatch<T> B, Levels L, int64_t N, double sgn)
+__global__ void __launch_bounds__(256) k_sweep_b(SweepBatch<T> B, Levels L, int64_t N,
+                                                 const T *__restrict__ val, double sgn_rhs,
+                                                 double sgn_mv)
 {
     using V = typename Vec16<T>::type;
     constexpr int W = Vec16<T>::w;
-    const GemvJob<T> &J = B.j[blockIdx.z];
-    const int n = J.n, ld = J.ld;
-    if ((int)blockIdx.x * 4 >= n) return;
+    constexpr int RW = kSweepRowsPerWave;
+    const SweepJob<T> &J = B.j[blockIdx.y];
+    const RhsJob<T> &R = J.rhs;
+    const int n = R.n, ld = J.ld;
+    const int row0 = blockIdx.x * (4 * RW);
+    if (row0 >= n) return;
     extern __shared__ unsigned char smem[];
     T *vs = reinterpret_cast<T *>(smem);
-    for (int r = threadIdx.x; r < ld; r += blockDim.x) vs[r] = r < n ? J.v[r] : T(0);
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + wave;
-    if (r >= n) return;
-    const V *row = reinterpret_cast<const V *>(J.S + (int64_t)r * ld);
     const int nv = (n + W - 1) / W;
-    // four independent 16-byte loads in flight per lane
-    double acc4[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int c = lane; c < nv; c += 256) {
-        V x[4];
+    V pre[RW][kSweepPre];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (c + 64 * u < nv) x[u] = row[c + 64 * u];
+    for (int q = 0; q < RW; ++q) {
+        const int r = min(row0 + wave * RW + q, n - 1);
+        const V *row = reinterpret_cast<const V *>(J.S + (int64_t)r * ld);
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (c + 64 * u < nv) acc4[u] += dot16<T>(x[u], vs + (c + 64 * u) * W);
+        for (int u = 0; u < kSweepPre; ++u) {
+            const int c = lane + 64 * u;
+            if (c < nv) pre[q][u] = row[c];
+            else pre[q][u] = V{};
+        }
     }
-    const double acc = wsum((acc4[0] + acc4[1]) + (acc4[2] + acc4[3]));
-    if (lane == 0) {
-        const double b = J.base ? (double)J.base[r] : 0.0;
-        const T o = (T)(b + sgn * acc);
-        if (J.glob) J.out[L.global(r, N, J.l)] = o;
-        else J.out[r] = o;
+    for (int r = threadIdx.x; r < ld; r += blockDim.x) {
+        double y = 0.0;
+        if (r < n) {
+            double sum = 0.0;
+            for (int k = 0; k < R.nt; ++k) {
+                const RhsTerm<T> &t = R.t[k];
+                for (int e = t.ptr[r]; e < t.ptr[r + 1]; ++e) {
+                    const int2 x = t.ent[e];
+                    const double v = t.glob ? (double)t.vec[L.global(x.x, N, t.vlev)] : (double)t.vec[x.x];
+                    sum += (double)val[x.y] * v;
+                }
+            }
+            y = (R.b ? (double)R.b[L.global(r, N, R.l)] : 0.0) + sgn_rhs * sum;
+        }
+        vs[r] = (T)y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+        const int r = row0 + wave * RW + q;
+        const V *row = reinterpret_cast<const V *>(J.S + (int64_t)min(r, n - 1) * ld);
+        double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+        for (int u = 0; u < kSweepPre; ++u) {
+            const int c = lane + 64 * u;
+            if (c < nv) {
+                if (u & 1) acc1 += dot16<T>(pre[q][u], vs + c * W);
+                else acc0 += dot16<T>(pre[q][u], vs + c * W);
+            }
+        }
+        for (int c = lane + 64 * kSweepPre; c < nv; c += 128) {
+            const V x0 = row[c];
+            acc0 += dot16<T>(x0, vs + c * W);
+            if (c + 64 < nv) acc1 += dot16<T>(row[c + 64], vs + (c + 64) * W);
+        }
+        const double acc = wsum(acc0 + acc1);
+        if (lane == 0 && r < n) {
+            const double b = J.base ? (double)J.base[r] : 0.0;
+            const T o = (T)(b + sgn_mv * acc);
+            if (J.glob) J.out[L.global(r, N, R.l)] = o;
+            else J.out[r] = o;
+        }
     }
 }
 
 template <typename T>
-static int launch_rhs(const RhsBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
-                      double sgn, hipStream_t s)
+static int launch_sweep(const SweepBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
+                        double sgn_rhs, double sgn_mv, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rhs_b<T>, dim3(grid_for(nmax), 1, nj), dim3(256), 0, s, B, P.L, P.N, val, sgn);
-    V2D_LAUNCH();
-    return 0;
-}
-
-template <typename T>
-static int launch_gemv(const GemvBatch<T> &B, int nj, int nmax, const DirectPattern &P, double sgn,
-                       hipStream_t s)
-{
-    hipLaunchKernelGGL(k_gemv_b<T>, dim3((nmax + 3) / 4, 1, nj), dim3(256), sizeof(T) * P.ldmax, s, B,
-                       P.L, P.N, sgn);
+    const unsigned rows_per_block = 4 * kSweepRowsPerWave;
+    hipLaunchKernelGGL(k_sweep_b<T>, dim3((nmax + rows_per_block - 1) / rows_per_block, nj, 1), dim3(256),
+                       sizeof(T) * P.ldmax, s, B, P.L, P.N, val, sgn_rhs, sgn_mv);
     V2D_LAUNCH();
     return 0;
 }
@@ -1010,62 +1028,61 @@ int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, c
     const int M = (nlev - 1) / 2;
     const int nbot = M, ntop = nlev - 1 - M;
     const int steps = std::max(nbot, ntop);
-    T *ybuf[2] = {W.colk, W.colk + (int64_t)P.nmax * NB};
     auto zl = [&](int l) { return W.z + P.voff[l]; };
     auto S = [&](int l) { return (const T *)(W.sinv + P.soff[l]); };
     // forward sweeps
     for (int st = 0; st < steps; ++st) {
-        RhsBatch<T> R{};
-        GemvBatch<T> G{};
+        SweepBatch<T> G{};
         int nj = 0, nmax = 0;
         for (int side = 0; side < 2; ++side) {
             if ((side == 0 && st >= nbot) || (side == 1 && st >= ntop)) continue;
             const int l = side == 0 ? st : nlev - 1 - st;
-            RhsJob<T> &r = R.j[nj];
-            r.l = l; r.n = P.n[l]; r.b = b; r.y = ybuf[nj]; r.nt = 0;
+            RhsJob<T> &r = G.j[nj].rhs;
+            r.l = l; r.n = P.n[l]; r.b = b; r.nt = 0;
             if (st > 0) {
                 const int p = side == 0 ? l - 1 : l + 1;
                 r.t[0] = RhsTerm<T>{side == 0 ? P.d_Lptr + P.voff[l] : P.d_Uptr + P.voff[l],
                                     side == 0 ? P.d_Lr : P.d_Ur, zl(p), p, 0};
                 r.nt = 1;
             }
-            G.j[nj] = GemvJob<T>{S(l), P.n[l], P.ld[l], l, 0, ybuf[nj], nullptr, zl(l)};
+            G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 0;
+            G.j[nj].base = nullptr; G.j[nj].out = zl(l);
             nmax = std::max(nmax, P.n[l]);
             ++nj;
         }
-        if (launch_rhs(R, nj, nmax, P, val, -1.0, s) || launch_gemv(G, nj, nmax, P, 1.0, s)) return -1;
+        if (launch_sweep(G, nj, nmax, P, val, -1.0, 1.0, s)) return -1;
     }
     {   // middle level
-        RhsBatch<T> R{};
-        GemvBatch<T> G{};
-        RhsJob<T> &r = R.j[0];
-        r.l = M; r.n = P.n[M]; r.b = b; r.y = ybuf[0]; r.nt = 0;
+        SweepBatch<T> G{};
+        RhsJob<T> &r = G.j[0].rhs;
+        r.l = M; r.n = P.n[M]; r.b = b; r.nt = 0;
         if (M > 0)
             r.t[r.nt++] = RhsTerm<T>{P.d_Lptr + P.voff[M], P.d_Lr, zl(M - 1), M - 1, 0};
         if (M < nlev - 1)
             r.t[r.nt++] = RhsTerm<T>{P.d_Uptr + P.voff[M], P.d_Ur, zl(M + 1), M + 1, 0};
-        G.j[0] = GemvJob<T>{S(M), P.n[M], P.ld[M], M, 1, ybuf[0], nullptr, x};
-        if (launch_rhs(R, 1, P.n[M], P, val, -1.0, s) || launch_gemv(G, 1, P.n[M], P, 1.0, s)) return -1;
+        G.j[0].S = S(M); G.j[0].ld = P.ld[M]; G.j[0].glob = 1;
+        G.j[0].base = nullptr; G.j[0].out = x;
+        if (launch_sweep(G, 1, P.n[M], P, val, -1.0, 1.0, s)) return -1;
     }
     // backward sweeps, outward from the middle
     for (int st = 0; st < steps; ++st) {
-        RhsBatch<T> R{};
-        GemvBatch<T> G{};
+        SweepBatch<T> G{};
         int nj = 0, nmax = 0;
         for (int side = 0; side < 2; ++side) {
             const int l = side == 0 ? M - 1 - st : M + 1 + st;
             if (l < 0 || l >= nlev) continue;
             const int p = side == 0 ? l + 1 : l - 1;
-            RhsJob<T> &r = R.j[nj];
-            r.l = l; r.n = P.n[l]; r.b = nullptr; r.y = ybuf[nj]; r.nt = 1;
+            RhsJob<T> &r = G.j[nj].rhs;
+            r.l = l; r.n = P.n[l]; r.b = nullptr; r.nt = 1;
             r.t[0] = RhsTerm<T>{side == 0 ? P.d_Uptr + P.voff[l] : P.d_Lptr + P.voff[l],
                                 side == 0 ? P.d_Ur : P.d_Lr, x, p, 1};
-            G.j[nj] = GemvJob<T>{S(l), P.n[l], P.ld[l], l, 1, ybuf[nj], zl(l), x};
+            G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 1;
+            G.j[nj].base = zl(l); G.j[nj].out = x;
             nmax = std::max(nmax, P.n[l]);
             ++nj;
         }
         if (!nj) break;
-        if (launch_rhs(R, nj, nmax, P, val, 1.0, s) || launch_gemv(G, nj, nmax, P, -1.0, s)) return -1;
+        if (launch_sweep(G, nj, nmax, P, val, 1.0, -1.0, s)) return -1;
     }
     return 0;
 }
