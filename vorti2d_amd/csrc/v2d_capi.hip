@@ -105,6 +105,9 @@ struct Ctx {
     double *gxd = nullptr;
     bool factored = false;
     int since_factor = 0;
+    // measured costs for the refactorisation decision (milliseconds)
+    double fac_ms_last = 0.0, it_ms_avg = 0.0;
+    int fresh_its = 0;
     double last_re = 0, last_invdtau = 0, last_inv2dt = 0, last_urot = 0;
     // timers: ev[0..3] bracket a Newton step (start, assembled, preconditioner
     // ready, solved); ev[4..5] bracket the factorisation made inside it, if any
@@ -169,7 +172,9 @@ static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
 // one re-orthogonalisation (fused multi-dot / multi-axpy), Givens rotations on
 // the host.  Solves A x = b for unit-norm b (FP32 vectors, FP64 dot products);
 // Z_j = M V_j is kept so the solution is x = Z y (flexible form).
-static int gmres_f32(Ctx *c, int cap, int *iters, double *relres)
+// budget > 0 (a stored, older factorisation): give up (return 2) once the
+// convergence rate seen so far projects more than `budget` iterations.
+static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
 {
     const int64_t n = 2 * c->N;
     const int m = cap;
@@ -240,6 +245,16 @@ static int gmres_f32(Ctx *c, int cap, int *iters, double *relres)
             return 1;
         }
         if (est <= c->gm_rtol || hn == 0.0) { ++j; break; }
+        if (budget > 0 && j >= 2) {
+            // geometric projection of the iterations this preconditioner needs
+            const double rate = std::pow(est, 1.0 / (j + 1));
+            const double need = rate < 1.0 ? std::log(c->gm_rtol) / std::log(rate) : INFINITY;
+            if (need > budget) {
+                *iters = j + 1;
+                *relres = est;
+                return 2;
+            }
+        }
     }
     const int k = std::min(j, m);
     std::vector<double> yv(k, 0.0);
@@ -893,14 +908,22 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
                                c->rhs, c->red, 1, c->gb);
             V2D_LAUNCH();
             double rr = 1.0;
-            int st = gmres_f32(c, need ? c->gm_cap : c->gm_max, &its, &rr);
+            // Iterations on the stored factorisation are worth spending while
+            // they cost less than a refactorisation plus the iterations a
+            // fresh one needs (both measured on this context).
+            int budget = c->gm_max;
+            if (!need && c->fac_ms_last > 0.0 && c->it_ms_avg > 0.0)
+                budget = std::max(3, std::min(c->gm_max, (int)(c->fac_ms_last / c->it_ms_avg) + c->fresh_its));
+            int st = gmres_f32(c, need ? c->gm_cap : c->gm_max, need ? 0 : budget, &its, &rr);
             if (st < 0) return -1;
+            if (need) c->fresh_its = its;
             if (st != 0 || rr > c->gm_rtol) {
                 if (!need) {   // stale factorisation: refactor and solve again
                     if (ctx_factor(c)) return -1;
                     int its2 = 0;
-                    st = gmres_f32(c, c->gm_cap, &its2, &rr);
+                    st = gmres_f32(c, c->gm_cap, 0, &its2, &rr);
                     if (st < 0) return -1;
+                    c->fresh_its = its2;
                     its += its2;
                 }
                 if (st != 0) {
@@ -930,6 +953,11 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
     sol_ms -= fac_ms;   // the factorisation may come before or after a GMRES attempt
     V2D_HIP(hipEventElapsedTime(&t_ms, c->ev[0], c->ev[3]));
     c->t_asm += a_ms; c->t_fac += fac_ms; c->t_sol += sol_ms; c->t_tot += t_ms;
+    if (c->fac_in_step) c->fac_ms_last = fac_ms;
+    if (its > 0) {
+        const double per = sol_ms / its;
+        c->it_ms_avg = c->it_ms_avg > 0.0 ? 0.5 * (c->it_ms_avg + per) : per;
+    }
     ++c->n_steps;
     c->n_lin += its;
     *res = dn / (2.0 * (double)N);

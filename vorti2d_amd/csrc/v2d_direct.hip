@@ -231,14 +231,17 @@ __global__ void __launch_bounds__(256) k_schur_S_b(Batch<T> B, const T *__restri
 // --------------------------------------------------- Gauss-Jordan ---------
 // Inverse of a kb x kb (kb <= 64) block, unpivoted Gauss-Jordan, held in
 // registers: thread t owns column c = t % 64 of the 16 consecutive rows
-// r = 16*(t/64) + q.  Per pivot p the owners publish the pivot row and column
-// to LDS (double-buffered: one barrier per pivot) and every entry takes one FMA
+// r = 16*(t/64) + q.  Per pivot p the owners of row p and column p publish
+// them to LDS (double-buffered, so one barrier per pivot suffices), every
+// thread reads the pivot, its row entry and its 16 column entries in one
+// round trip, and spends one FMA per entry
 //     a[r][c] -= a[r][p] * (a[p][c] / a[p][p]).
 // The pivot column is left unscaled during the sweep (later pivots act on it
-// linearly) and multiplied by -1/a[p][p] once at the end, with its diagonal
-// entry set to -1 so that it ends as 1/a[p][p]; only the pivot-row entry of a
-// thread needs a select.  Entries outside kb x kb must be zero on entry; the
-// result is written to out (row stride NB).
+// linearly) and multiplied by -1/a[p][p] once at the end (each thread keeps
+// the factor of its own column), with its diagonal entry set to -1 so that it
+// ends as 1/a[p][p]; only the pivot-row entry of a thread needs a select.
+// Entries outside kb x kb must be zero on entry; the result is written to out
+// (row stride NB).
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x)
 {
@@ -248,36 +251,46 @@ __device__ __forceinline__ double recip(double x)
 }
 
 template <typename T>
-__device__ __forceinline__ void gj_regs(T (&a)[16], int kb, T *prow, T *pcol, T *scale, T *out)
+__device__ __forceinline__ void gj_regs(T (&a)[16], int kb, T *prow, T *pcol, T *out)
 {
     const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
-    if (rg == 0) scale[c] = T(0);
+    T sc = T(0);
+    if (rg == 0) prow[c] = a[0];
+    if (c == 0)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) pcol[rg * 16 + q] = a[q];
     for (int pw = 0; pw < 4; ++pw) {
 #pragma unroll
         for (int pq = 0; pq < 16; ++pq) {
             const int p = pw * 16 + pq;
             if (p < kb) {
-                T *pr = prow + (pq & 1) * NB, *pc = pcol + (pq & 1) * NB;
-                if (rg == pw) pr[c] = a[pq];
-                if (c == p)
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) pc[rg * 16 + q] = a[q];
+                const T *pr = prow + (p & 1) * NB, *pc = pcol + (p & 1) * NB;
                 __syncthreads();
-                const T ip = recip(pr[p]);
-                const T rp = (c == p) ? T(0) : pr[c] * ip;
-                if (c == p && rg == 0) scale[p] = -ip;
+                const T piv = pr[p];
+                const T prc = pr[c];
                 T cr[16];
 #pragma unroll
                 for (int q = 0; q < 16; ++q) cr[q] = pc[rg * 16 + q];
+                const T ip = recip(piv);
+                const bool isp = c == p;
+                const T rp = isp ? T(0) : prc * ip;
+                sc = isp ? -ip : sc;
 #pragma unroll
                 for (int q = 0; q < 16; ++q) a[q] = fma(-cr[q], rp, a[q]);
-                const T rowv = (c == p) ? T(-1) : rp;
+                const T rowv = isp ? T(-1) : rp;
                 a[pq] = (rg == pw) ? rowv : a[pq];
+                // publish pivot p + 1 into the other buffer
+                const int p1 = p + 1;
+                if (p1 < kb) {
+                    T *pr1 = prow + (p1 & 1) * NB, *pc1 = pcol + (p1 & 1) * NB;
+                    if (rg == (pq == 15 ? pw + 1 : pw)) pr1[c] = a[(pq + 1) & 15];
+                    if (c == p1)
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) pc1[rg * 16 + q] = a[q];
+                }
             }
         }
     }
-    __syncthreads();
-    const T sc = scale[c];
 #pragma unroll
     for (int q = 0; q < 16; ++q) out[(rg * 16 + q) * NB + c] = (c < kb) ? a[q] * sc : T(0);
 }
@@ -285,7 +298,7 @@ __device__ __forceinline__ void gj_regs(T (&a)[16], int kb, T *prow, T *pcol, T 
 template <typename T>
 __global__ void __launch_bounds__(256) k_gj_diag_b(Batch<T> B, int k0)
 {
-    __shared__ T prow[2 * NB], pcol[2 * NB], scale[NB];
+    __shared__ T prow[2 * NB], pcol[2 * NB];
     const Job<T> &J = B.j[blockIdx.z];
     const int n = J.n, ld = J.ld;
     if (k0 >= n) return;
@@ -297,7 +310,7 @@ __global__ void __launch_bounds__(256) k_gj_diag_b(Batch<T> B, int k0)
         const int r = rg * 16 + q;
         a[q] = (r < kb && c < kb) ? J.S[(int64_t)(k0 + r) * ld + k0 + c] : T(0);
     }
-    gj_regs(a, kb, prow, pcol, scale, J.dinv);
+    gj_regs(a, kb, prow, pcol, J.dinv);
 }
 
 // 16-byte vectors of T
@@ -493,7 +506,7 @@ __global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
         T g[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) g[q] = sc[(rg * 16 + q) * NB + c];
-        gj_regs(g, kb2, sr, sr + 2 * NB, sr + 4 * NB, J.dinv);
+        gj_regs(g, kb2, sr, sr + 2 * NB, J.dinv);
     }
 }
 
@@ -548,7 +561,7 @@ __global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
     using V = typename Vec8<T>::type;
     constexpr int VW = Vec8<T>::w;
     __shared__ T sc[NB * NB];   // next pivot block (fused inversion only)
-    __shared__ T sx[5 * NB];    // gj_regs scratch
+    __shared__ T sx[4 * NB];    // gj_regs scratch
     const int z = blockIdx.x % B.nj;
     const Job<T> &J = B.j[z];
     T *__restrict__ A = J.S;
@@ -677,7 +690,7 @@ __global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
         T g[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) g[q] = sc[(rg * 16 + q) * NB + c];
-        gj_regs(g, kb2, sx, sx + 2 * NB, sx + 4 * NB, J.dinv);
+        gj_regs(g, kb2, sx, sx + 2 * NB, J.dinv);
     }
 }
 
@@ -903,19 +916,21 @@ __device__ __forceinline__ double dot16<double>(const double2 &a, const double *
     return a.x * b[0] + a.y * b[1];
 }
 
-// One solve step of up to two levels, fused: every block first issues the
-// loads of its own S rows (two rows per wavefront, up to kSweepPre 16-byte
-// vectors per lane held in registers), then builds the whole sparse right-hand
-// side y of its level in LDS while those loads are in flight,
-//     y = [b_l] + sgn_rhs * sum_t list_t(r) . vec_t,
-// and finally forms out = [base] + sgn_mv * S y (rows past the prefetch are
-// streamed).  Output level-ordered (glob = 0) or scattered to the global vector.
+// One solve step of up to two levels, as two launches:
+//   k_level_rhs_b:  y = [b_l] + sgn_rhs * sum_t list_t(r) . vec_t   (one row per thread)
+//   k_level_gemv_b: out = [base] + sgn_mv * S y
+// The GEMV issues the loads of its own S rows first (two rows per wavefront,
+// up to kSweepPre 16-byte vectors per lane held in registers), stages y in
+// LDS while they are in flight, then forms the dot products (row parts past
+// the prefetch are streamed).  Output level-ordered (glob = 0) or scattered
+// to the global vector.
 constexpr int kSweepRowsPerWave = 2;
 constexpr int kSweepPre = 9;
 
 template <typename T>
 struct SweepJob {
-    RhsJob<T> rhs;   // y (kept in LDS)
+    RhsJob<T> rhs;
+    T *y;            // level-ordered scratch for the right-hand side
     const T *S;
     int ld, glob;
     const T *base;
@@ -927,16 +942,34 @@ struct SweepBatch {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_b(SweepBatch<T> B, Levels L, int64_t N,
-                                                 const T *__restrict__ val, double sgn_rhs,
-                                                 double sgn_mv)
+__global__ void __launch_bounds__(256) k_level_rhs_b(SweepBatch<T> B, Levels L, int64_t N,
+                                                     const T *__restrict__ val, double sgn_rhs)
+{
+    const SweepJob<T> &J = B.j[blockIdx.y];
+    const RhsJob<T> &R = J.rhs;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R.n) return;
+    double sum = 0.0;
+    for (int k = 0; k < R.nt; ++k) {
+        const RhsTerm<T> &t = R.t[k];
+        for (int e = t.ptr[r]; e < t.ptr[r + 1]; ++e) {
+            const int2 x = t.ent[e];
+            const double v = t.glob ? (double)t.vec[L.global(x.x, N, t.vlev)] : (double)t.vec[x.x];
+            sum += (double)val[x.y] * v;
+        }
+    }
+    J.y[r] = (T)((R.b ? (double)R.b[L.global(r, N, R.l)] : 0.0) + sgn_rhs * sum);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L, int64_t N,
+                                                      double sgn_mv)
 {
     using V = typename Vec16<T>::type;
     constexpr int W = Vec16<T>::w;
     constexpr int RW = kSweepRowsPerWave;
     const SweepJob<T> &J = B.j[blockIdx.y];
-    const RhsJob<T> &R = J.rhs;
-    const int n = R.n, ld = J.ld;
+    const int n = J.rhs.n, ld = J.ld;
     const int row0 = blockIdx.x * (4 * RW);
     if (row0 >= n) return;
     extern __shared__ unsigned char smem[];
@@ -955,22 +988,7 @@ __global__ void __launch_bounds__(256) k_sweep_b(SweepBatch<T> B, Levels L, int6
             else pre[q][u] = V{};
         }
     }
-    for (int r = threadIdx.x; r < ld; r += blockDim.x) {
-        double y = 0.0;
-        if (r < n) {
-            double sum = 0.0;
-            for (int k = 0; k < R.nt; ++k) {
-                const RhsTerm<T> &t = R.t[k];
-                for (int e = t.ptr[r]; e < t.ptr[r + 1]; ++e) {
-                    const int2 x = t.ent[e];
-                    const double v = t.glob ? (double)t.vec[L.global(x.x, N, t.vlev)] : (double)t.vec[x.x];
-                    sum += (double)val[x.y] * v;
-                }
-            }
-            y = (R.b ? (double)R.b[L.global(r, N, R.l)] : 0.0) + sgn_rhs * sum;
-        }
-        vs[r] = (T)y;
-    }
+    for (int r = threadIdx.x; r < ld; r += blockDim.x) vs[r] = r < n ? J.y[r] : T(0);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
@@ -986,15 +1004,14 @@ __global__ void __launch_bounds__(256) k_sweep_b(SweepBatch<T> B, Levels L, int6
             }
         }
         for (int c = lane + 64 * kSweepPre; c < nv; c += 128) {
-            const V x0 = row[c];
-            acc0 += dot16<T>(x0, vs + c * W);
+            acc0 += dot16<T>(row[c], vs + c * W);
             if (c + 64 < nv) acc1 += dot16<T>(row[c + 64], vs + (c + 64) * W);
         }
         const double acc = wsum(acc0 + acc1);
         if (lane == 0 && r < n) {
             const double b = J.base ? (double)J.base[r] : 0.0;
             const T o = (T)(b + sgn_mv * acc);
-            if (J.glob) J.out[L.global(r, N, R.l)] = o;
+            if (J.glob) J.out[L.global(r, N, J.rhs.l)] = o;
             else J.out[r] = o;
         }
     }
@@ -1005,8 +1022,10 @@ static int launch_sweep(const SweepBatch<T> &B, int nj, int nmax, const DirectPa
                         double sgn_rhs, double sgn_mv, hipStream_t s)
 {
     const unsigned rows_per_block = 4 * kSweepRowsPerWave;
-    hipLaunchKernelGGL(k_sweep_b<T>, dim3((nmax + rows_per_block - 1) / rows_per_block, nj, 1), dim3(256),
-                       sizeof(T) * P.ldmax, s, B, P.L, P.N, val, sgn_rhs, sgn_mv);
+    hipLaunchKernelGGL(k_level_rhs_b<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N, val,
+                       sgn_rhs);
+    hipLaunchKernelGGL(k_level_gemv_b<T>, dim3((nmax + rows_per_block - 1) / rows_per_block, nj, 1),
+                       dim3(256), sizeof(T) * P.ldmax, s, B, P.L, P.N, sgn_mv);
     V2D_LAUNCH();
     return 0;
 }
@@ -1030,6 +1049,7 @@ int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, c
     const int steps = std::max(nbot, ntop);
     auto zl = [&](int l) { return W.z + P.voff[l]; };
     auto S = [&](int l) { return (const T *)(W.sinv + P.soff[l]); };
+    T *ybuf[2] = {W.colk, W.colk + (int64_t)P.nmax * NB};   // the panels are free during a solve
     // forward sweeps
     for (int st = 0; st < steps; ++st) {
         SweepBatch<T> G{};
@@ -1045,6 +1065,7 @@ int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, c
                                     side == 0 ? P.d_Lr : P.d_Ur, zl(p), p, 0};
                 r.nt = 1;
             }
+            G.j[nj].y = ybuf[nj];
             G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 0;
             G.j[nj].base = nullptr; G.j[nj].out = zl(l);
             nmax = std::max(nmax, P.n[l]);
@@ -1060,6 +1081,7 @@ int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, c
             r.t[r.nt++] = RhsTerm<T>{P.d_Lptr + P.voff[M], P.d_Lr, zl(M - 1), M - 1, 0};
         if (M < nlev - 1)
             r.t[r.nt++] = RhsTerm<T>{P.d_Uptr + P.voff[M], P.d_Ur, zl(M + 1), M + 1, 0};
+        G.j[0].y = ybuf[0];
         G.j[0].S = S(M); G.j[0].ld = P.ld[M]; G.j[0].glob = 1;
         G.j[0].base = nullptr; G.j[0].out = x;
         if (launch_sweep(G, 1, P.n[M], P, val, -1.0, 1.0, s)) return -1;
@@ -1076,6 +1098,7 @@ int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, c
             r.l = l; r.n = P.n[l]; r.b = nullptr; r.nt = 1;
             r.t[0] = RhsTerm<T>{side == 0 ? P.d_Uptr + P.voff[l] : P.d_Lptr + P.voff[l],
                                 side == 0 ? P.d_Ur : P.d_Lr, x, p, 1};
+            G.j[nj].y = ybuf[nj];
             G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 1;
             G.j[nj].base = zl(l); G.j[nj].out = x;
             nmax = std::max(nmax, P.n[l]);
