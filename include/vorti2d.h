@@ -113,6 +113,12 @@ int v2d_ctx_init_state(v2d_ctx *ctx);
  * refactor_every Newton steps (0 = only when GMRES needs more than
  * gmres_max_iter iterations), inner relative tolerance, iteration cap. */
 int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int gmres_max_iter);
+/* Inexact-Newton forcing for V2D_MIXED (no counterpart in the reference, whose
+ * Krylov solves use a fixed ksp_rtol): the GMRES tolerance of Newton step k is
+ * max(gmres_rtol, min(eta_max, 0.9 (||R_k|| / ||R_k-1||)^2)) with the
+ * Eisenstat-Walker safeguard; eta_max <= 0 restores the fixed tolerance.  The
+ * history restarts with every new state. */
+int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max);
 /* Kernel of the block-LU Gauss-Jordan update: VALU on 64- or 128-wide tiles,
  * or the matrix cores (MFMA 16x16x4, FP32 or FP64) on 64-wide tiles (the
  * default).  A tuning knob with no counterpart in the reference: results do not

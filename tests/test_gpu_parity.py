@@ -310,6 +310,16 @@ def test_newton_41_matches_reference_state(v, prec):
     assert abs(psi.max() + psi.min()) < 1e-6
 
 
+def test_mixed_forcing_converges(v):
+    """Eisenstat-Walker GMRES tolerances (loose early, gmres_rtol near the
+    solution) reach the reference converged state."""
+    g = golden("newton_41x41_re40")
+    s = _run(v, g["xg"], g["yg"], re=40.0, steady=True, pseudo_tol=1e-10, max_pseudo_iter=50,
+             precision="mixed", refactor_every=0, gmres_max_iter=20, gmres_forcing=0.1)
+    assert np.abs(s.psi - g["psi"]).max() < 1e-8
+    assert np.abs(s.ome - g["ome"]).max() < 1e-8
+
+
 def test_mixed_factorisation_reuse_converges(v):
     """Adaptive policy (refactor only when GMRES on the stale FP32 block LU misses
     its tolerance) reaches the same converged state as exact FP64 Newton."""

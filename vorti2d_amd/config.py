@@ -71,6 +71,9 @@ class Config:
     GMRES misses gmres_rtol within gmres_max_iter iterations)."""
     gmres_rtol: float = 1.0e-6
     gmres_max_iter: int = 20
+    gmres_forcing: float = 0.0
+    """mixed only: > 0 enables the Eisenstat-Walker GMRES tolerance per Newton
+    step, capped at this value (gmres_rtol stays the floor)."""
 
     def inv_dtau(self) -> float:
         return 0.0 if math.isinf(self.dtau) else 1.0 / self.dtau

@@ -108,6 +108,9 @@ struct Ctx {
     // measured costs for the refactorisation decision (milliseconds)
     double fac_ms_last = 0.0, it_ms_avg = 0.0;
     int fresh_its = 0;
+    // inexact Newton forcing (Eisenstat-Walker choice 2); eta_max <= 0: off
+    double eta_max = 0.0, eta_prev = 0.0, rn_prev = 0.0;
+    double rtol_step = 1e-6;   // GMRES tolerance of the current Newton step
     double last_re = 0, last_invdtau = 0, last_inv2dt = 0, last_urot = 0;
     // timers: ev[0..3] bracket a Newton step (start, assembled, preconditioner
     // ready, solved); ev[4..5] bracket the factorisation made inside it, if any
@@ -244,11 +247,11 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
             *relres = INFINITY;
             return 1;
         }
-        if (est <= c->gm_rtol || hn == 0.0) { ++j; break; }
+        if (est <= c->rtol_step || hn == 0.0) { ++j; break; }
         if (budget > 0 && j >= 2) {
             // geometric projection of the iterations this preconditioner needs
             const double rate = std::pow(est, 1.0 / (j + 1));
-            const double need = rate < 1.0 ? std::log(c->gm_rtol) / std::log(rate) : INFINITY;
+            const double need = rate < 1.0 ? std::log(c->rtol_step) / std::log(rate) : INFINITY;
             if (need > budget) {
                 *iters = j + 1;
                 *relres = est;
@@ -603,7 +606,11 @@ int v2d_ctx_set_state(v2d_ctx *ctx, const double *psi, const double *ome, const 
     for (int q = 0; q < 4; ++q)
         if (src[q]) V2D_HIP(hipMemcpy(dst[q], src[q], sizeof(double) * c->N, hipMemcpyHostToDevice));
     // a new state makes the stored block LU (of an earlier Jacobian) stale
-    if (psi || ome) c->factored = false;
+    if (psi || ome) {
+        c->factored = false;
+        c->rn_prev = 0.0;
+        c->eta_prev = 0.0;
+    }
     return 0;
 }
 
@@ -645,6 +652,18 @@ int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int 
     c->refactor_every = refactor_every;
     c->gm_rtol = gmres_rtol;
     c->gm_max = gmres_max_iter;
+    return 0;
+}
+
+int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max)
+{
+    Ctx *c = ctx->c.get();
+    if (!(eta_max < 1.0)) {
+        set_error("forcing eta_max must be < 1 (<= 0 disables it)");
+        return -1;
+    }
+    c->eta_max = eta_max > 0.0 ? eta_max : 0.0;
+    c->rn_prev = c->eta_prev = 0.0;
     return 0;
 }
 
@@ -908,6 +927,22 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
                                c->rhs, c->red, 1, c->gb);
             V2D_LAUNCH();
             double rr = 1.0;
+            // GMRES tolerance: gmres_rtol, or the Eisenstat-Walker forcing term
+            // eta_k = 0.9 (||R_k|| / ||R_k-1||)^2 (safeguarded, capped at eta_max)
+            c->rtol_step = c->gm_rtol;
+            if (c->eta_max > 0.0) {
+                double eta = c->eta_max;
+                if (c->rn_prev > 0.0) {
+                    const double r = bn / c->rn_prev;
+                    eta = 0.9 * r * r;
+                    const double sg = 0.9 * c->eta_prev * c->eta_prev;
+                    if (sg > 0.1) eta = std::max(eta, sg);
+                }
+                eta = std::min(eta, c->eta_max);
+                c->rtol_step = std::max(eta, c->gm_rtol);
+                c->eta_prev = eta;
+            }
+            c->rn_prev = bn;
             // Iterations on the stored factorisation are worth spending while
             // they cost less than a refactorisation plus the iterations a
             // fresh one needs (both measured on this context).
@@ -917,7 +952,7 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
             int st = gmres_f32(c, need ? c->gm_cap : c->gm_max, need ? 0 : budget, &its, &rr);
             if (st < 0) return -1;
             if (need) c->fresh_its = its;
-            if (st != 0 || rr > c->gm_rtol) {
+            if (st != 0 || rr > c->rtol_step) {
                 if (!need) {   // stale factorisation: refactor and solve again
                     if (ctx_factor(c)) return -1;
                     int its2 = 0;

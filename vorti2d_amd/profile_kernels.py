@@ -52,8 +52,8 @@ def main(argv=None):
         ctx.factor()
         p = ctx.profile_factor()
         print(f"factor {a.size}x{jmax} {a.precision}: {p['factor_ms']:.1f} ms, "
-              f"{p['launches']} update launches, {p['update_ms'] / p['launches'] * 1e3:.2f} us avg",
-              flush=True)
+              f"{p['launches']} update launches, {p['update_ms'] / p['launches'] * 1e3:.2f} us avg, "
+              f"algorithmic_bytes_per_update_launch={p['bytes'] / p['launches']:.0f}", flush=True)
     else:
         for _ in range(a.launches):
             ctx.assemble(a.re)
