@@ -205,7 +205,7 @@ def main():
                          "(0 = only when GMRES on the stored one misses its tolerance)")
     ap.add_argument("--gmres-max", type=int, default=30)
     ap.add_argument("--gmres-rtol", type=float, default=1e-6)
-    ap.add_argument("--forcing", type=float, default=0.0,
+    ap.add_argument("--forcing", type=float, default=0.1,
                     help="mixed: Eisenstat-Walker GMRES tolerance capped at this value (0: off)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -233,10 +233,10 @@ def main():
     kname = {"mfma64": "k_gj_update_mfma_b<float>", "valu64": "k_gj_update_b<float, 64>",
              "valu128": "k_gj_update_b<float, 128>"}[min(mx["tune"], key=mx["tune"].get)]
     alg_upd = pr["bytes"] / max(pr["launches"], 1)
-    ent = traffic.get(kname)
-    tr_upd = ent["measured_over_algorithmic"] * alg_upd if ent else None
-    ent = traffic.get("k_assemble<float>")
-    tr_asm = ent["bytes_per_launch"] if ent else None
+    ent = traffic.get(kname) or {}
+    tr_upd = ent["measured_over_algorithmic"] * alg_upd if "measured_over_algorithmic" in ent else None
+    ent = traffic.get("k_assemble<float>") or {}
+    tr_asm = ent.get("bytes_per_launch")
     tm = mx["timers"]
     last = mx["last"]
     res = {
