@@ -556,6 +556,8 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->wd.tmat = c->alloc<double>(2 * nm * c->pat.ldmax, rc);
         c->wd.colk = c->alloc<double>(2 * nm * 64, rc);
         c->wd.rowk = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
+        c->wd.colk2 = c->alloc<double>(2 * nm * 64, rc);
+        c->wd.rowk2 = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
         c->wd.dinv = c->alloc<double>(2 * 64 * 64, rc);
         c->wd.z = c->alloc<double>(2 * N, rc);
     } else {
@@ -563,6 +565,8 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->wf.tmat = c->alloc<float>(2 * nm * c->pat.ldmax, rc);
         c->wf.colk = c->alloc<float>(2 * nm * 64, rc);
         c->wf.rowk = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
+        c->wf.colk2 = c->alloc<float>(2 * nm * 64, rc);
+        c->wf.rowk2 = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
         c->wf.dinv = c->alloc<float>(2 * 64 * 64, rc);
         c->wf.z = c->alloc<float>(2 * N, rc);
         c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);
@@ -571,7 +575,14 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->gx = c->alloc<float>(2 * N, rc);
         c->gxd = c->alloc<double>(2 * N, rc);
     }
+    // pivot-ready flags (2 per precision) and the error word of the dataflow steps
+    int *sync_words = c->alloc<int>(8, rc);
     if (rc) return -1;
+    V2D_HIP(hipMemset(sync_words, 0, sizeof(int) * 8));
+    c->wd.flags = sync_words;
+    c->wd.err = sync_words + 2;
+    c->wf.flags = sync_words + 4;
+    c->wf.err = sync_words + 6;
     *out = h.release();
     return 0;
 }

@@ -142,6 +142,7 @@ struct Job {
     int64_t voff;          // offset of the level in level-ordered vectors / CSR rows
     T *colk, *dinv, *tmat; // per-job scratch
     T *rowk;               // transposed pivot-row panel (MFMA update only, else null)
+    T *colk_w, *rowk_w;    // panels of the next step (MFMA dataflow step only)
 };
 
 // One Schur-update source: the neighbouring level's inverse on the side the
@@ -240,8 +241,8 @@ __global__ void __launch_bounds__(256) k_schur_S_b(Batch<T> B, const T *__restri
 // linearly) and multiplied by -1/a[p][p] once at the end (each thread keeps
 // the factor of its own column), with its diagonal entry set to -1 so that it
 // ends as 1/a[p][p]; only the pivot-row entry of a thread needs a select.
-// Entries outside kb x kb must be zero on entry; the result is written to out
-// (row stride NB).
+// Entries outside kb x kb must be zero on entry; the result is left in a[] and
+// written to out (row stride NB).
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x)
 {
@@ -292,7 +293,10 @@ __device__ __forceinline__ void gj_regs(T (&a)[16], int kb, T *prow, T *pcol, T 
         }
     }
 #pragma unroll
-    for (int q = 0; q < 16; ++q) out[(rg * 16 + q) * NB + c] = (c < kb) ? a[q] * sc : T(0);
+    for (int q = 0; q < 16; ++q) {
+        a[q] = (c < kb) ? a[q] * sc : T(0);
+        out[(rg * 16 + q) * NB + c] = a[q];
+    }
 }
 
 template <typename T>
@@ -541,6 +545,12 @@ template <> struct Mfma<float> {
     // (row, column) of accumulator element v inside a 16 x 16 block
     static __device__ __forceinline__ int row(int li, int lk, int v) { return li; }
     static __device__ __forceinline__ int col(int li, int lk, int v) { return 4 * lk + v; }
+    // direct form: D += A B with element v of lane l at row drow(l / 16, v), column l % 16
+    static __device__ __forceinline__ C opd(float a, float b, C c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lk, int v) { return 4 * lk + v; }
 };
 template <> struct Mfma<double> {
     using C = double __attribute__((ext_vector_type(4)));
@@ -551,17 +561,38 @@ template <> struct Mfma<double> {
     }
     static __device__ __forceinline__ int row(int li, int lk, int v) { return 4 * v + lk; }
     static __device__ __forceinline__ int col(int li, int lk, int v) { return li; }
+    static __device__ __forceinline__ C opd(double a, double b, C c) { return op(a, b, c); }
+    static __device__ __forceinline__ int drow(int lk, int v) { return 4 * v + lk; }
 };
 
+// One Gauss-Jordan step of the MFMA form as a single launch (dataflow): every
+// block updates its 64 x 64 tile with the step-k0 panels (read buffers), and
+// the blocks that touch the next pivot block k1 = k0 + 64 also prepare step k1
+// (write buffers), so no row-panel launch sits between two steps:
+//   * tile (k1, k1), dispatched first: inverts its updated block in registers
+//     (gj_regs), stores the inverse as the new pivot block of A, into dinv and
+//     into the next transposed row panel, then publishes flags[job] = epoch
+//     (release, agent scope);
+//   * tiles (k1, c), c != k1: stage their updated tile in LDS, wait for the
+//     flag (acquire; a bounded spin that sets *err and carries on if the flag
+//     never comes, so every wave exits), and store D^-1 * tile (matrix cores,
+//     direct form) as the new pivot rows of A and the next row panel;
+//   * tiles (i, k1), i != k1: also store their updated tile as the next column
+//     panel.
+// The pivot-row work thus overlaps the bulk update of the step.
+constexpr int kSpinLimit = 1 << 21;
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
+__global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int epoch,
+                                                        int *__restrict__ flags, int *__restrict__ err)
 {
     using M = Mfma<T>;
     using C = typename M::C;
     using V = typename Vec8<T>::type;
     constexpr int VW = Vec8<T>::w;
-    __shared__ T sc[NB * NB];   // next pivot block (fused inversion only)
-    __shared__ T sx[4 * NB];    // gj_regs scratch
+    constexpr int LT = NB + 4;    // padded LDS row stride
+    __shared__ T sT[NB * LT];     // next pivot block, or a tile stored transposed
+    __shared__ T sx[4 * NB];      // gj_regs scratch
     const int z = blockIdx.x % B.nj;
     const Job<T> &J = B.j[z];
     T *__restrict__ A = J.S;
@@ -578,18 +609,30 @@ __global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
         else if (lin <= piv) lin -= 1;
     }
     const int i0 = (lin / nt) * NB, c0 = (lin % nt) * NB;
-    if (i0 == k0) return;
+    if (i0 == k0) {
+        // the current pivot rows are not updated; their part of column tile
+        // k1 is still the next column panel's rows k0 .. k0 + 63
+        if (k1 < n && c0 == k1) {
+            for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+                const int r = idx / NB, t = idx % NB;
+                J.colk_w[(int64_t)(k0 + r) * NB + t] = (k1 + t < n) ? A[(int64_t)(k0 + r) * ld + k1 + t] : T(0);
+            }
+        }
+        return;
+    }
+    const bool is_piv = k1 < n && i0 == k1 && c0 == k1;
+    const bool is_prow = k1 < n && i0 == k1 && c0 != k1;
+    const bool is_pcol = k1 < n && c0 == k1 && i0 != k1;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
     const int rw = i0 + (w >> 1) * 32, cw = c0 + (w & 1) * 32;
     const bool colK = c0 == k0;            // the pivot columns: old values are zero
     const bool cfull = c0 + NB <= n;       // no column of the tile past n
-    // operands
+    // ---------------------------------------------------- tile update ----
     const T *pa[2], *pb[2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) pa[bi] = J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk;
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) pb[bj] = J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk;
-    // old tile (consumed after the product)
     C old[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
@@ -654,27 +697,39 @@ __global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
+        for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
-            if (M::kSwap && cfull) {
-                const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
-                if (r < n) {
-                    float4 x;
-                    x.x = acc[bi][bj][0]; x.y = acc[bi][bj][1]; x.z = acc[bi][bj][2]; x.w = acc[bi][bj][3];
-                    *reinterpret_cast<float4 *>(A + (int64_t)r * ld + c) = x;
-                }
-            } else {
+    // ------------------------------------------------ ordinary tiles ----
+    if (!is_piv && !is_prow) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
-                    if (r < n && c < n) A[(int64_t)r * ld + c] = acc[bi][bj][v];
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                if (M::kSwap && cfull) {
+                    const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                    if (r < n) {
+                        float4 x;
+                        x.x = acc[bi][bj][0]; x.y = acc[bi][bj][1]; x.z = acc[bi][bj][2]; x.w = acc[bi][bj][3];
+                        *reinterpret_cast<float4 *>(A + (int64_t)r * ld + c) = x;
+                        if (is_pcol) *reinterpret_cast<float4 *>(J.colk_w + (int64_t)r * NB + (c - k1)) = x;
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
+                        if (r < n) {
+                            if (c < n) A[(int64_t)r * ld + c] = acc[bi][bj][v];
+                            if (is_pcol) J.colk_w[(int64_t)r * NB + (c - k1)] = acc[bi][bj][v];
+                        }
+                    }
                 }
             }
-        }
-    // fused inversion of the next pivot block
-    if (k1 < n && i0 == k1 && c0 == k1) {
-        const int kb2 = min(NB, n - k1);
+        return;
+    }
+    const int kb2 = min(NB, n - k1);
+    if (is_piv) {
+        // ------------------------------------- next pivot block inverse ----
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -683,15 +738,91 @@ __global__ void __launch_bounds__(256) k_gj_update_mfma_b(Batch<T> B, int k0)
                 for (int v = 0; v < 4; ++v) {
                     const int r = (w >> 1) * 32 + bi * 16 + M::row(li, lk, v);
                     const int c = (w & 1) * 32 + bj * 16 + M::col(li, lk, v);
-                    sc[r * NB + c] = (r < kb2 && c < kb2) ? acc[bi][bj][v] : T(0);
+                    sT[r * LT + c] = (r < kb2 && c < kb2) ? acc[bi][bj][v] : T(0);
                 }
         __syncthreads();
         const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
         T g[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) g[q] = sc[(rg * 16 + q) * NB + c];
+        for (int q = 0; q < 16; ++q) g[q] = sT[(rg * 16 + q) * LT + c];
         gj_regs(g, kb2, sx, sx + 2 * NB, J.dinv);
+        T *rk = J.rowk_w + (int64_t)(k1 + c) * NB + rg * 16;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int r = rg * 16 + q;
+            if (r < kb2 && c < kb2) A[(int64_t)(k1 + r) * ld + k1 + c] = g[q];
+            rk[q] = g[q];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(flags + z, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
     }
+    // ------------------------------------------------ next pivot rows ----
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = (w >> 1) * 32 + bi * 16 + M::row(li, lk, v);
+                const int c = (w & 1) * 32 + bj * 16 + M::col(li, lk, v);
+                sT[c * LT + r] = acc[bi][bj][v];   // transposed: sT[c][t]
+            }
+    if (threadIdx.x == 0) {
+        int spins = 0;
+        while (__hip_atomic_load(flags + z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            if (++spins > kSpinLimit) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int rw2 = (w >> 1) * 32, cw2 = (w & 1) * 32;
+    C d[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) d[bi][bj][v] = T(0);
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        T av[2][4], bv[2][4];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int u = 0; u < 4 / VW; ++u) {
+                const V x = reinterpret_cast<const V *>(J.dinv + (rw2 + bi * 16 + li) * NB + 16 * h + 4 * lk)[u];
+#pragma unroll
+                for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = reinterpret_cast<const T *>(&x)[e];
+            }
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bv[bj][e] = sT[(cw2 + bj * 16 + li) * LT + 16 * h + 4 * lk + e];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) d[bi][bj] = M::opd(av[bi][e], bv[bj][e], d[bi][bj]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int c = cw2 + bj * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rw2 + bi * 16 + M::drow(lk, v);
+                if (r < kb2 && c0 + c < n) A[(int64_t)(k1 + r) * ld + c0 + c] = d[bi][bj][v];
+                J.rowk_w[(int64_t)(c0 + c) * NB + r] = d[bi][bj][v];
+            }
+        }
 }
 
 // ------------------------------------------------------ launch probe ------
@@ -739,7 +870,7 @@ static int probe_mark(LaunchProbe *p, const Batch<T> &B, int k0, bool begin, hip
 
 // In-place Gauss-Jordan inverse of the batch's dense matrices.
 template <typename T>
-static int gj_invert_b(const Batch<T> &B, LaunchProbe *probe, hipStream_t s)
+static int gj_invert_b(const Batch<T> &B, DirectWork<T> &W, LaunchProbe *probe, hipStream_t s)
 {
     int nmax = 0;
     for (int z = 0; z < B.nj; ++z) nmax = std::max(nmax, B.j[z].n);
@@ -748,12 +879,28 @@ static int gj_invert_b(const Batch<T> &B, LaunchProbe *probe, hipStream_t s)
     const unsigned np = (unsigned)((nmax + RP - 1) / RP);
     const unsigned nz = (unsigned)B.nj;
     hipLaunchKernelGGL(k_gj_diag_b<T>, dim3(1, 1, nz), dim3(256), 0, s, B, 0);
+    if (B.kind == kUpdateMfma64) {
+        // step 0's panels from the row-panel kernel, then one dataflow launch
+        // per step, the panel buffers alternating
+        hipLaunchKernelGGL(k_gj_rowpanel_b<T>, dim3(np, 2, nz), dim3(256), 0, s, B, 0);
+        Batch<T> Bs = B;
+        for (int k0 = 0; k0 < nmax; k0 += NB) {
+            if (probe_mark(probe, B, k0, true, s)) return -1;
+            hipLaunchKernelGGL(k_gj_step_mfma_b<T>, dim3(nt * nt * nz), dim3(256), 0, s, Bs, k0,
+                               ++W.epoch, W.flags, W.err);
+            if (probe_mark(probe, B, k0, false, s)) return -1;
+            for (int z = 0; z < B.nj; ++z) {
+                std::swap(Bs.j[z].colk, Bs.j[z].colk_w);
+                std::swap(Bs.j[z].rowk, Bs.j[z].rowk_w);
+            }
+        }
+        V2D_LAUNCH();
+        return 0;
+    }
     for (int k0 = 0; k0 < nmax; k0 += NB) {
         hipLaunchKernelGGL(k_gj_rowpanel_b<T>, dim3(np, 2, nz), dim3(256), 0, s, B, k0);
         if (probe_mark(probe, B, k0, true, s)) return -1;
-        if (B.kind == kUpdateMfma64)
-            hipLaunchKernelGGL(k_gj_update_mfma_b<T>, dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
-        else if (ut == 128)
+        if (ut == 128)
             hipLaunchKernelGGL((k_gj_update_b<T, 128>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
         else
             hipLaunchKernelGGL((k_gj_update_b<T, 64>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
@@ -780,7 +927,10 @@ static Job<T> make_job(const DirectPattern &P, DirectWork<T> &W, int l, int slot
     J.colk = W.colk + (int64_t)slot * P.nmax * NB;
     J.dinv = W.dinv + slot * NB * NB;
     J.tmat = W.tmat + (int64_t)slot * P.nmax * P.ldmax;
-    J.rowk = W.update_kind == kUpdateMfma64 ? W.rowk + (int64_t)slot * P.ldmax * NB : nullptr;
+    const bool df = W.update_kind == kUpdateMfma64;
+    J.rowk = df ? W.rowk + (int64_t)slot * P.ldmax * NB : nullptr;
+    J.colk_w = df ? W.colk2 + (int64_t)slot * P.nmax * NB : nullptr;
+    J.rowk_w = df ? W.rowk2 + (int64_t)slot * P.ldmax * NB : nullptr;
     return J;
 }
 
@@ -828,11 +978,40 @@ static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, b
         hipLaunchKernelGGL(k_schur_S_b<T>, dim3(grid_for(nmax), nmax, nj), dim3(256), 0, s, Bk, val);
     }
     V2D_LAUNCH();
-    return gj_invert_b(B, W.probe, s);
+    return gj_invert_b(B, W, W.probe, s);
+}
+
+template <typename T>
+static int direct_factor_levels(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
+
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
+{
+    if (W.update_kind == kUpdateMfma64) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, W.err, 0);
+        V2D_LAUNCH();
+    }
+    if (direct_factor_levels(P, W, val, s)) return -1;
+    if (W.update_kind == kUpdateMfma64) {
+        int e = 0;
+        V2D_HIP(hipMemcpyAsync(&e, W.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        V2D_HIP(hipStreamSynchronize(s));
+        if (e) {
+            set_error("block LU: a Gauss-Jordan step waited past its spin limit for the pivot "
+                      "inverse (results invalid)");
+            return -1;
+        }
+    }
+    return 0;
+}
+
+template <typename T>
+static int direct_factor_levels(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
 {
     const int kind = W.update_kind;
     const int nlev = P.L.nlev;

@@ -156,6 +156,10 @@ struct DirectWork {
     T *rowk = nullptr;   // Gauss-Jordan pivot-row panels, transposed, 2 x (ldmax x 64)
     T *dinv = nullptr;   // pivot-block inverses, 2 x (64 x 64)
     T *z = nullptr;      // level-ordered 2N work vector
+    T *colk2 = nullptr, *rowk2 = nullptr;   // the other buffer of each panel (dataflow steps)
+    int *flags = nullptr;   // 2 per-job "pivot inverse ready" words (device)
+    int *err = nullptr;     // device error word of the dataflow steps
+    int epoch = 0;          // value the current step's pivot block publishes
     int update_kind = kUpdateMfma64;
     LaunchProbe *probe = nullptr;   // set to time the update launches
 };
