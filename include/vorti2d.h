@@ -174,6 +174,38 @@ int v2d_ctx_time_assembly(v2d_ctx *ctx, double re, double invdtau, double inv2dt
 /* Same for the stencil SpMV y = A x on the last assembly (x = ones). */
 int v2d_ctx_time_spmv(v2d_ctx *ctx, int launches, double *ms_per_launch);
 
+/* ---------------------------------------------------------------------
+ * Discrete adjoint of the steady force coefficients.  None of these has a
+ * counterpart in the reference, which computes no derivatives; they serve the
+ * gradient of J = wd * Cd + wl * Cl (Cd, Cl as in vorti2d/forces.py:82-194,
+ * body axes, given reference length) with respect to the angle of attack,
+ * the Reynolds number and every mesh coordinate:
+ *     dJ/dtheta = dJ/dtheta|explicit + lambda^T db/dtheta,  A^T lambda = dJ/dU,
+ * with b the residual-form RHS of the assembly and A = -db/dU its Newton
+ * matrix.
+ * --------------------------------------------------------------------- */
+/* x = A^-T b with the current factorisation (host vectors of 2N doubles). */
+int v2d_ctx_solve_transpose(v2d_ctx *ctx, const double *b, double *x);
+/* g_m (10 planes of N, overwritten) = lambda^T db/d(metric planes) at the
+ * current state; gscal[0] = lambda . db/dRe, gscal[1] = lambda . db/dalpha
+ * (radians).  lam: host vector of 2N (psi rows then omega rows). */
+int v2d_ctx_residual_vjp(v2d_ctx *ctx, double re, double inv2dt, double urot, const double *lam,
+                         double *g_m, double *gscal);
+/* Metric backward: g_x, g_y (N each, i fastest) = g_m . d(compute_metrics)/d(xg, yg)
+ * for the 10 output planes of v2d_compute_metrics (host arrays). */
+int v2d_metrics_vjp(int imax, int jmax, double dksi, double deta, const double *xg,
+                    const double *yg, const double *g_m, double *g_x, double *g_y);
+/* Force backward at the current state: g_ome (N), g_m (10 planes) =
+ * dJ/d(omega, metric planes); out3 = {Cd, Cl, explicit dJ/dRe}. */
+int v2d_ctx_forces_vjp(v2d_ctx *ctx, double re, double ref_length, double wd, double wl,
+                       double *g_ome, double *g_m, double *out3);
+/* The whole adjoint at the stored (converged, steady) state of an FP64
+ * context: refactorises the Newton matrix, solves A^T lambda = dJ/dU and
+ * returns g_x, g_y = dJ/d(xg, yg) (N each), gscal = {dJ/dalpha per degree,
+ * dJ/dRe}, coef = {Cd, Cl}. */
+int v2d_ctx_adjoint_forces(v2d_ctx *ctx, double re, double urot, double ref_length, double wd,
+                           double wl, double *g_x, double *g_y, double *gscal, double *coef);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,8 @@ from .device import DeviceContext  # noqa: E402
 from .forces import ForceCoeffs, compute_force_coeffs  # noqa: E402
 from .mesh import generate_annulus, generate_cylinder, load_cgns_ogrid, load_mesh, save_mesh  # noqa: E402
 from .solver import Solver, run  # noqa: E402
+from .airfoil import AirfoilGrid  # noqa: E402
+from .adjoint import SteadyForces, force_gradients, metrics_vjp, steady_solve  # noqa: E402
 
 __all__ = [
     "Config", "Solver", "run", "generate_cylinder", "generate_annulus", "load_mesh",

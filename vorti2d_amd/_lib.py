@@ -61,6 +61,11 @@ _SIGS = {
     "v2d_ctx_timers": ([vp, vp, vp, i32], i32),
     "v2d_ctx_time_assembly": ([vp, f64, f64, f64, f64, i32, P(f64)], i32),
     "v2d_ctx_time_spmv": ([vp, i32, P(f64)], i32),
+    "v2d_ctx_solve_transpose": ([vp, vp, vp], i32),
+    "v2d_ctx_residual_vjp": ([vp, f64, f64, f64, vp, vp, vp], i32),
+    "v2d_metrics_vjp": ([i32, i32, f64, f64, vp, vp, vp, vp, vp], i32),
+    "v2d_ctx_forces_vjp": ([vp, f64, f64, f64, f64, vp, vp, vp], i32),
+    "v2d_ctx_adjoint_forces": ([vp, f64, f64, f64, f64, f64, vp, vp, vp, vp], i32),
 }
 
 for _name, (_args, _res) in _SIGS.items():

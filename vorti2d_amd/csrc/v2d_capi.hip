@@ -1100,4 +1100,189 @@ int v2d_ctx_time_spmv(v2d_ctx *ctx, int launches, double *ms_per_launch)
     return 0;
 }
 
+// ------------------------------------------------------------ adjoint ----
+int v2d_ctx_solve_transpose(v2d_ctx *ctx, const double *b, double *x)
+{
+    Ctx *c = ctx->c.get();
+    if (!c->factored) {
+        set_error("solve_transpose: no factorisation (call v2d_ctx_factor first)");
+        return -1;
+    }
+    const int64_t n = 2 * c->N;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    if (c->prec == V2D_FP64) {
+        DBuf db, dx;
+        V2D_ALLOC(db, sizeof(double) * n);
+        V2D_ALLOC(dx, sizeof(double) * n);
+        V2D_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (direct_solve_t<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), db.as<double>(),
+                                   dx.as<double>(), c->s))
+            return -1;
+        V2D_HIP(hipStreamSynchronize(c->s));
+        V2D_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<float> bf(n), xf(n);
+    for (int64_t q = 0; q < n; ++q) bf[q] = (float)b[q];
+    DBuf db, dx;
+    V2D_ALLOC(db, sizeof(float) * n);
+    V2D_ALLOC(dx, sizeof(float) * n);
+    V2D_HIP(hipMemcpy(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    if (direct_solve_t<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), db.as<float>(),
+                              dx.as<float>(), c->s))
+        return -1;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    V2D_HIP(hipMemcpy(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < n; ++q) x[q] = xf[q];
+    return 0;
+}
+
+// lambda^T db/d(metrics, Re, alpha) on device buffers: g_m (10N) is added to.
+static int residual_vjp_dev(Ctx *c, double re, double inv2dt, double urot, const double *d_lam,
+                            double *d_gm, double *gscal)
+{
+    const AsmArgs a = c->args(re, 0.0, inv2dt, urot);
+    const unsigned nb = grid_for(c->N);
+    DBuf part, red;
+    V2D_ALLOC(part, sizeof(double) * 2 * nb);
+    V2D_ALLOC(red, sizeof(double) * 2);
+    hipLaunchKernelGGL(k_residual_vjp, dim3(nb), dim3(256), 0, c->s, a, c->flags, d_lam, d_gm, part.as<double>());
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, 2, part.as<double>(),
+                       red.as<double>());
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpyAsync(gscal, red.p, sizeof(double) * 2, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int v2d_ctx_residual_vjp(v2d_ctx *ctx, double re, double inv2dt, double urot, const double *lam,
+                         double *g_m, double *gscal)
+{
+    Ctx *c = ctx->c.get();
+    const int64_t N = c->N;
+    DBuf dl, dg;
+    V2D_ALLOC(dl, sizeof(double) * 2 * N);
+    V2D_ALLOC(dg, sizeof(double) * 10 * N);
+    V2D_HIP(hipMemcpy(dl.p, lam, sizeof(double) * 2 * N, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(10 * N)), dim3(256), 0, c->s, 10 * N, dg.as<double>());
+    V2D_LAUNCH();
+    if (residual_vjp_dev(c, re, inv2dt, urot, dl.as<double>(), dg.as<double>(), gscal)) return -1;
+    V2D_HIP(hipMemcpy(g_m, dg.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int metrics_vjp_dev(int imax, int jmax, double dksi, double deta, const double *dx, const double *dy,
+                           const double *d_gm, double *d_gx, double *d_gy, hipStream_t s)
+{
+    const int64_t N = (int64_t)imax * jmax;
+    DBuf geta;
+    V2D_ALLOC(geta, sizeof(double) * 2 * N);
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(2 * N)), dim3(256), 0, s, 2 * N, geta.as<double>());
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(N)), dim3(256), 0, s, N, d_gx);
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(N)), dim3(256), 0, s, N, d_gy);
+    hipLaunchKernelGGL(k_metrics_vjp_1, dim3(grid_for(N)), dim3(256), 0, s, imax, jmax, dksi, deta, dx, dy,
+                       d_gm, d_gx, d_gy, geta.as<double>());
+    hipLaunchKernelGGL(k_metrics_vjp_2, dim3(grid_for(N)), dim3(256), 0, s, imax, jmax, deta,
+                       geta.as<double>(), d_gx, d_gy);
+    V2D_LAUNCH();
+    V2D_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int v2d_metrics_vjp(int imax, int jmax, double dksi, double deta, const double *xg, const double *yg,
+                    const double *g_m, double *g_x, double *g_y)
+{
+    if (check_grid(imax, jmax)) return -1;
+    const int64_t N = (int64_t)imax * jmax;
+    DBuf dx, dy, dg, gx, gy;
+    V2D_ALLOC(dx, sizeof(double) * N);
+    V2D_ALLOC(dy, sizeof(double) * N);
+    V2D_ALLOC(dg, sizeof(double) * 10 * N);
+    V2D_ALLOC(gx, sizeof(double) * N);
+    V2D_ALLOC(gy, sizeof(double) * N);
+    V2D_HIP(hipMemcpy(dx.p, xg, sizeof(double) * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(dy.p, yg, sizeof(double) * N, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpy(dg.p, g_m, sizeof(double) * 10 * N, hipMemcpyHostToDevice));
+    if (metrics_vjp_dev(imax, jmax, dksi, deta, dx.as<double>(), dy.as<double>(), dg.as<double>(),
+                        gx.as<double>(), gy.as<double>(), nullptr))
+        return -1;
+    V2D_HIP(hipMemcpy(g_x, gx.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(g_y, gy.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int forces_vjp_dev(Ctx *c, double re, double d, double wd, double wl, double *d_gome, double *d_gm,
+                          double *out3)
+{
+    const int64_t N = c->N;
+    if (!(d > 0.0)) {
+        set_error("force adjoint: ref_length must be given (> 0)");
+        return -1;
+    }
+    hipLaunchKernelGGL(k_forces_vjp, dim3(1), dim3(1024), 0, c->s, c->imax, re, c->dksi, c->deta, c->m, N,
+                       c->ome, d, wd, wl, d_gome, d_gm, c->hbuf);
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpyAsync(out3, c->hbuf, sizeof(double) * 3, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int v2d_ctx_forces_vjp(v2d_ctx *ctx, double re, double ref_length, double wd, double wl, double *g_ome,
+                       double *g_m, double *out3)
+{
+    Ctx *c = ctx->c.get();
+    const int64_t N = c->N;
+    DBuf go, gm;
+    V2D_ALLOC(go, sizeof(double) * N);
+    V2D_ALLOC(gm, sizeof(double) * 10 * N);
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(N)), dim3(256), 0, c->s, N, go.as<double>());
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(10 * N)), dim3(256), 0, c->s, 10 * N, gm.as<double>());
+    V2D_LAUNCH();
+    if (forces_vjp_dev(c, re, ref_length, wd, wl, go.as<double>(), gm.as<double>(), out3)) return -1;
+    V2D_HIP(hipMemcpy(g_ome, go.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(g_m, gm.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int v2d_ctx_adjoint_forces(v2d_ctx *ctx, double re, double urot, double ref_length, double wd, double wl,
+                           double *g_x, double *g_y, double *gscal, double *coef)
+{
+    Ctx *c = ctx->c.get();
+    if (c->prec != V2D_FP64) {
+        set_error("force adjoint: needs an FP64 context (exact transposed solve)");
+        return -1;
+    }
+    const int64_t N = c->N;
+    // Jacobian of the steady residual at the stored state (no pseudo-time shift)
+    if (ctx_assemble(c, re, 0.0, 0.0, urot)) return -1;
+    if (ctx_factor(c)) return -1;
+    DBuf gu, lam, gm, gx, gy;
+    V2D_ALLOC(gu, sizeof(double) * 2 * N);
+    V2D_ALLOC(lam, sizeof(double) * 2 * N);
+    V2D_ALLOC(gm, sizeof(double) * 10 * N);
+    V2D_ALLOC(gx, sizeof(double) * N);
+    V2D_ALLOC(gy, sizeof(double) * N);
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(2 * N)), dim3(256), 0, c->s, 2 * N, gu.as<double>());
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(10 * N)), dim3(256), 0, c->s, 10 * N, gm.as<double>());
+    V2D_LAUNCH();
+    double f3[3];
+    // dJ/dU lives in the omega half of the field-blocked vector
+    if (forces_vjp_dev(c, re, ref_length, wd, wl, gu.as<double>() + N, gm.as<double>(), f3)) return -1;
+    if (direct_solve_t<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), gu.as<double>(),
+                               lam.as<double>(), c->s))
+        return -1;
+    double rs[2];
+    if (residual_vjp_dev(c, re, 0.0, urot, lam.as<double>(), gm.as<double>(), rs)) return -1;
+    if (metrics_vjp_dev(c->imax, c->jmax, c->dksi, c->deta, c->x, c->y, gm.as<double>(), gx.as<double>(),
+                        gy.as<double>(), c->s))
+        return -1;
+    V2D_HIP(hipMemcpy(g_x, gx.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpy(g_y, gy.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    gscal[0] = rs[1] * (M_PI / 180.0);   // per degree of alpha
+    gscal[1] = f3[2] + rs[0];            // per unit Re
+    coef[0] = f3[0];
+    coef[1] = f3[1];
+    return 0;
+}
+
 }  // extern "C"

@@ -623,6 +623,9 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
     const bool is_piv = k1 < n && i0 == k1 && c0 == k1;
     const bool is_prow = k1 < n && i0 == k1 && c0 != k1;
     const bool is_pcol = k1 < n && c0 == k1 && i0 != k1;
+    // the pivot chain (next pivot inverse, then the next pivot rows) bounds the
+    // step: its waves take issue priority over the bulk update sharing the SIMDs
+    if (is_piv || is_prow) __builtin_amdgcn_s_setprio(3);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
     const int rw = i0 + (w >> 1) * 32, cw = c0 + (w & 1) * 32;
     const bool colK = c0 == k0;            // the pivot columns: old values are zero
@@ -1288,6 +1291,130 @@ int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, c
     }
     return 0;
 }
+// Transposed GEMV of one solve step: out[c] = [base[c]] + sgn_mv * sum_r S[r][c] y[r].
+// One workgroup per 64-column strip: lane = column, the 16 wavefronts split
+// the rows (each reads 64 contiguous entries of a row), LDS reduction.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_level_gemv_t_b(SweepBatch<T> B, Levels L, int64_t N,
+                                                         double sgn_mv)
+{
+    __shared__ double part[16][64];
+    const SweepJob<T> &J = B.j[blockIdx.y];
+    const int n = J.rhs.n, ld = J.ld;
+    const int c0 = blockIdx.x * 64;
+    if (c0 >= n) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = c0 + lane;
+    double acc = 0.0;
+    if (c < n)
+        for (int r = wave; r < n; r += 16) acc += (double)J.S[(int64_t)r * ld + c] * (double)J.y[r];
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && c < n) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t += part[q][lane];
+        const double b = J.base ? (double)J.base[c] : 0.0;
+        const T o = (T)(b + sgn_mv * t);
+        if (J.glob) J.out[L.global(c, N, J.rhs.l)] = o;
+        else J.out[c] = o;
+    }
+}
+
+template <typename T>
+static int launch_sweep_t(const SweepBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
+                          double sgn_rhs, double sgn_mv, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_level_rhs_b<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N, val,
+                       sgn_rhs);
+    hipLaunchKernelGGL(k_level_gemv_t_b<T>, dim3((nmax + 63) / 64, nj, 1), dim3(1024), 0, s, B, P.L, P.N,
+                       sgn_mv);
+    V2D_LAUNCH();
+    return 0;
+}
+
+// Solve A^T x = b with the stored twisted factorisation.  The Schur complements
+// of A^T in the same elimination order are the transposes of A's, so the
+// sweeps reuse the stored inverses (applied transposed) and the couplings by
+// columns: forward bottom  z_l = S_l^-T (b_l - U_{l-1}^T z_{l-1}),
+// forward top  z_l = S_l^-T (b_l - L_{l+1}^T z_{l+1}), the middle level both,
+// backward bottom  x_l = z_l - S_l^-T L_{l+1}^T x_{l+1},
+// backward top  x_l = z_l - S_l^-T U_{l-1}^T x_{l-1}.
+template <typename T>
+int direct_solve_t(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
+                   hipStream_t s)
+{
+    const int nlev = P.L.nlev;
+    const int M = (nlev - 1) / 2;
+    const int nbot = M, ntop = nlev - 1 - M;
+    const int steps = std::max(nbot, ntop);
+    auto zl = [&](int l) { return W.z + P.voff[l]; };
+    auto S = [&](int l) { return (const T *)(W.sinv + P.soff[l]); };
+    T *ybuf[2] = {W.colk, W.colk + (int64_t)P.nmax * NB};
+    auto UC = [&](int l, const T *vec, int vlev, int glob) {
+        return RhsTerm<T>{P.d_UCptr + P.voff[l], P.d_Uc, vec, vlev, glob};
+    };
+    auto LC = [&](int l, const T *vec, int vlev, int glob) {
+        return RhsTerm<T>{P.d_LCptr + P.voff[l], P.d_Lc, vec, vlev, glob};
+    };
+    for (int st = 0; st < steps; ++st) {
+        SweepBatch<T> G{};
+        int nj = 0, nmax = 0;
+        for (int side = 0; side < 2; ++side) {
+            if ((side == 0 && st >= nbot) || (side == 1 && st >= ntop)) continue;
+            const int l = side == 0 ? st : nlev - 1 - st;
+            RhsJob<T> &r = G.j[nj].rhs;
+            r.l = l; r.n = P.n[l]; r.b = b; r.nt = 0;
+            if (st > 0) {
+                const int p = side == 0 ? l - 1 : l + 1;
+                r.t[0] = side == 0 ? UC(l, zl(p), p, 0) : LC(l, zl(p), p, 0);
+                r.nt = 1;
+            }
+            G.j[nj].y = ybuf[nj];
+            G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 0;
+            G.j[nj].base = nullptr; G.j[nj].out = zl(l);
+            nmax = std::max(nmax, P.n[l]);
+            ++nj;
+        }
+        if (launch_sweep_t(G, nj, nmax, P, val, -1.0, 1.0, s)) return -1;
+    }
+    {
+        SweepBatch<T> G{};
+        RhsJob<T> &r = G.j[0].rhs;
+        r.l = M; r.n = P.n[M]; r.b = b; r.nt = 0;
+        if (M > 0) r.t[r.nt++] = UC(M, zl(M - 1), M - 1, 0);
+        if (M < nlev - 1) r.t[r.nt++] = LC(M, zl(M + 1), M + 1, 0);
+        G.j[0].y = ybuf[0];
+        G.j[0].S = S(M); G.j[0].ld = P.ld[M]; G.j[0].glob = 1;
+        G.j[0].base = nullptr; G.j[0].out = x;
+        if (launch_sweep_t(G, 1, P.n[M], P, val, -1.0, 1.0, s)) return -1;
+    }
+    for (int st = 0; st < steps; ++st) {
+        SweepBatch<T> G{};
+        int nj = 0, nmax = 0;
+        for (int side = 0; side < 2; ++side) {
+            const int l = side == 0 ? M - 1 - st : M + 1 + st;
+            if (l < 0 || l >= nlev) continue;
+            const int p = side == 0 ? l + 1 : l - 1;
+            RhsJob<T> &r = G.j[nj].rhs;
+            r.l = l; r.n = P.n[l]; r.b = nullptr; r.nt = 1;
+            r.t[0] = side == 0 ? LC(l, x, p, 1) : UC(l, x, p, 1);
+            G.j[nj].y = ybuf[nj];
+            G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 1;
+            G.j[nj].base = zl(l); G.j[nj].out = x;
+            nmax = std::max(nmax, P.n[l]);
+            ++nj;
+        }
+        if (!nj) break;
+        if (launch_sweep_t(G, nj, nmax, P, val, 1.0, -1.0, s)) return -1;
+    }
+    return 0;
+}
+template int direct_solve_t<double>(const DirectPattern &, const DirectWork<double> &, const double *,
+                                    const double *, double *, hipStream_t);
+template int direct_solve_t<float>(const DirectPattern &, const DirectWork<float> &, const float *,
+                                   const float *, float *, hipStream_t);
+
 template int direct_solve<double>(const DirectPattern &, const DirectWork<double> &, const double *,
                                   const double *, double *, hipStream_t);
 template int direct_solve<float>(const DirectPattern &, const DirectWork<float> &, const float *,

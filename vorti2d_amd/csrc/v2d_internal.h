@@ -109,6 +109,34 @@ void set_error(const std::string &msg);
 
 inline unsigned grid_for(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// ------------------------------------------------- device reductions ----
+// Wave64 sum with cross-lane shuffles, then one LDS slot per wavefront.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide sum returned to every thread (sh: one slot per wavefront + 1).
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        double t = (threadIdx.x < (blockDim.x >> 6)) ? sh[threadIdx.x] : 0.0;
+        t = wave_sum(t);
+        if (lane == 0) sh[16] = t;
+    }
+    __syncthreads();
+    const double total = sh[16];
+    __syncthreads();
+    return total;
+}
+
+
 // ------------------------------------------- block-tridiagonal direct solver
 // Exact LU of the Newton matrix in radial-level blocks: level l holds all
 // unknowns of its j rows (psi then omega, i fastest).  The factorisation keeps
@@ -169,5 +197,21 @@ int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStr
 template <typename T>
 int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
                  hipStream_t s);
+// Solve A^T x = b with the same stored factorisation.
+template <typename T>
+int direct_solve_t(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
+                   hipStream_t s);
+
+// adjoint kernels (v2d_adjoint.hip)
+__global__ void k_forces_vjp(int imax, double re, double dksi, double deta, const double *m, int64_t N,
+                             const double *ome, double d, double wd, double wl, double *g_ome,
+                             double *g_m, double *out);
+__global__ void k_residual_vjp(AsmArgs a, const uint8_t *flags, const double *lam, double *g_m,
+                               double *partial);
+__global__ void k_metrics_vjp_1(int imax, int jmax, double dksi, double deta, const double *x,
+                                const double *y, const double *g_m, double *g_x, double *g_y,
+                                double *g_eta);
+__global__ void k_metrics_vjp_2(int imax, int jmax, double deta, const double *g_eta, double *g_x,
+                                double *g_y);
 
 }  // namespace v2d

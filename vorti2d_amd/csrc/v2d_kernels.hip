@@ -594,32 +594,6 @@ template __global__ void k_spmv<double, double>(int, int, const uint8_t *, const
 template __global__ void k_spmv<float, float>(int, int, const uint8_t *, const float *, const float *, float *);
 
 // ======================================================= vector reductions ==
-// Wave64 sum with cross-lane shuffles, then one LDS slot per wavefront.
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Block-wide sum returned to every thread (sh: one slot per wavefront + 1).
-__device__ __forceinline__ double block_sum(double v, double *sh)
-{
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    if (wid == 0) {
-        double t = (threadIdx.x < (blockDim.x >> 6)) ? sh[threadIdx.x] : 0.0;
-        t = wave_sum(t);
-        if (lane == 0) sh[16] = t;
-    }
-    __syncthreads();
-    const double total = sh[16];
-    __syncthreads();
-    return total;
-}
-
 // partial[b*m + c] = sum over the block's share of  V_c . w   (c < m), FP64
 // accumulation.  V is column-major with leading dimension ld.  Fused
 // multi-dot for classical Gram-Schmidt: w is read once per block for all c.

@@ -168,6 +168,49 @@ class DeviceContext:
                                       float(urot), C.byref(res), C.byref(rn), C.byref(its)))
         return float(res.value), float(rn.value), int(its.value)
 
+    def solve_transpose(self, b):
+        """x = A^-T b with the stored factorisation."""
+        b = f64c(b)
+        if b.size != 2 * self.N:
+            raise ValueError("b must have length 2*imax*jmax")
+        x = np.empty(2 * self.N)
+        check(lib.v2d_ctx_solve_transpose(self._h, ptr(b), ptr(x)))
+        return x
+
+    def residual_vjp(self, lam, re, inv2dt=0.0, urot=0.0):
+        """lambda^T db/d(metric planes) (10 x N), lambda . db/dRe, lambda . db/dalpha [rad]."""
+        lam = f64c(lam)
+        if lam.size != 2 * self.N:
+            raise ValueError("lam must have length 2*imax*jmax")
+        gm = np.empty((10, self.N))
+        gs = np.zeros(2)
+        check(lib.v2d_ctx_residual_vjp(self._h, float(re), float(inv2dt), float(urot), ptr(lam),
+                                       ptr(gm), ptr(gs)))
+        return gm, float(gs[0]), float(gs[1])
+
+    def forces_vjp(self, re, ref_length, wd=1.0, wl=0.0):
+        """dJ/d(omega) (N), dJ/d(metric planes) (10 x N) and (Cd, Cl, explicit dJ/dRe)
+        for J = wd Cd + wl Cl at the current state."""
+        go = np.empty(self.N)
+        gm = np.empty((10, self.N))
+        out = np.zeros(3)
+        check(lib.v2d_ctx_forces_vjp(self._h, float(re), float(ref_length), float(wd), float(wl),
+                                     ptr(go), ptr(gm), ptr(out)))
+        return go, gm, out
+
+    def adjoint_forces(self, re, ref_length, wd=1.0, wl=0.0, urot=0.0):
+        """Adjoint gradient of J = wd Cd + wl Cl at the stored steady state:
+        returns dict(cd, cl, dalpha (per degree), dre, gx, gy (imax x jmax))."""
+        gx = np.empty(self.N)
+        gy = np.empty(self.N)
+        gs = np.zeros(2)
+        cf = np.zeros(2)
+        check(lib.v2d_ctx_adjoint_forces(self._h, float(re), float(urot), float(ref_length), float(wd),
+                                         float(wl), ptr(gx), ptr(gy), ptr(gs), ptr(cf)))
+        return dict(cd=float(cf[0]), cl=float(cf[1]), dalpha=float(gs[0]), dre=float(gs[1]),
+                    gx=gx.reshape(self.jmax, self.imax).T.copy(),
+                    gy=gy.reshape(self.jmax, self.imax).T.copy())
+
     def bdf2_shift(self):
         check(lib.v2d_ctx_bdf2_shift(self._h))
 
