@@ -9,10 +9,10 @@ to the node coordinates) and maps the mesh gradient onto the amplitudes.
 
 The reference computes no derivatives; its force coefficients
 (vorti2d/forces.py:82-194, body axes) and its steady inner loop
-(vorti2d/solver.py:136-145) are what is differentiated.  The solve uses
-switched-evolution pseudo-time steps while the residual is large
-(vorti2d/config.py:139-166 semantics: invdtau enters the Newton matrix only)
-and pure Newton once it has dropped.
+(vorti2d/solver.py:136-145) are what is differentiated.  The solve is pure
+Newton by default; optional switched-evolution pseudo-time steps use the
+reference's dtau semantics (vorti2d/config.py:139-166: invdtau enters the
+Newton matrix only).
 """
 from __future__ import annotations
 
@@ -35,11 +35,13 @@ class SteadyResult:
 
 
 def steady_solve(xg, yg, re, alpha_deg=0.0, ff_bc=0, ref_length=1.0, tol=1e-13, max_iter=200,
-                 dtau0=0.5):
-    """Steady FP64 solve on the device: Newton with switched-evolution
-    relaxation (dtau_k = dtau0 ||R_0|| / ||R_k||, dropped once ||R_k|| < 1e-6
-    ||R_0||) until res = ||delta||_2 / (2N) <= tol.  Returns the context
-    holding the converged state and its force coefficients."""
+                 dtau0=0.0):
+    """Steady FP64 solve on the device from the initial state of
+    vorti2d/solver.py:103-114: pure Newton (dtau0 = 0, the reference's steady
+    dtau = inf), or with switched-evolution relaxation (dtau_k = dtau0 ||R_0|| /
+    ||R_k||, dropped once ||R_k|| < 1e-6 ||R_0||), until res = ||delta||_2 / (2N)
+    <= tol.  Returns the context holding the converged state and its force
+    coefficients."""
     a = math.radians(alpha_deg)
     ctx = DeviceContext(xg, yg, ff_bc, math.cos(a), math.sin(a), "fp64")
     ctx.init_state()

@@ -8,11 +8,13 @@ arrays of shape (imax, jmax), i around the body with the seam stored twice
 (i = 0 and i = imax-1 at the trailing-edge base midpoint), j = 0 on the wall,
 j = jmax-1 on a far-field circle.  Nodes run clockwise (trailing edge ->
 lower surface -> leading edge -> upper surface -> trailing edge), which gives
-a positive metric Jacobian with j outward.
+a positive metric Jacobian with j outward.  The chord runs from x = -0.5 to
+0.5: the wall-vorticity row normalises the wall node position by its radius
+(vorti2d_core.f90:343-349), so no wall node may sit at the origin.
 
-Each grid line j is the straight segment from the surface point S_i to the
-far-field point F_i = c + R (cos phi_i, sin phi_i), phi_i = -2 pi i / (imax-1)
-about the mid-chord c, with a geometric radial distribution of first spacing
+Each grid line i is the straight segment from the surface point S_i to the
+far-field point F_i = R (cos phi_i, sin phi_i), phi_i = -2 pi i / (imax-1),
+about the mid-chord, with a geometric radial distribution of first spacing
 `first_spacing` (in units of |F - S|).  Hicks-Henne bumps
     b_k(x) = sin(pi x^(ln 0.5 / ln h_k))^width,   h_k in (0, 1)
 displace the upper (k < K) or lower (k >= K) surface in y by amplitude a_k; the
@@ -109,8 +111,9 @@ class AirfoilGrid:
             raise ValueError("n_bumps must be even (half upper, half lower)")
         self.imax, self.jmax = imax, jmax
         xs, ys, xcs, side = surface(imax, m, p, t)
+        xs = xs - 0.5                      # mid-chord at the origin
         phi = -2.0 * np.pi * np.arange(imax) / (imax - 1)
-        xf = 0.5 + radius * np.cos(phi)
+        xf = radius * np.cos(phi)
         yf = radius * np.sin(phi)
         r = radial_distribution(jmax, first_spacing)
         self.r = r
