@@ -149,6 +149,35 @@ def cpu_baseline(imax, jmax, n_newton, re):
                         f"x {n_newton} Newton iterations; 1 thread"))
 
 
+def adjoint_timing(v, reps=3):
+    """The adjoint workload: NACA 4-digit O-grid 257x129, Re 500, alpha 5 deg,
+    32 Hicks-Henne amplitudes ~ N(0, 1e-3); wall time of the FP64 steady solve
+    and of one adjoint gradient (dJ/d(alpha, Re, 32 amplitudes) for one
+    objective), median of `reps`."""
+    from vorti2d_amd.adjoint import steady_solve
+    grid = v.AirfoilGrid(257, 129, n_bumps=32)
+    amps = np.random.default_rng(0).normal(0.0, 1e-3, 32)
+    xg, yg = grid.grid(amps)
+    fw, ad = [], []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        r = steady_solve(xg, yg, 500.0, 5.0)
+        t1 = time.perf_counter()
+        g = r.ctx.adjoint_forces(500.0, 1.0, 1.0, 0.0)
+        grid.bump_gradient(g["gx"], g["gy"])
+        t2 = time.perf_counter()
+        fw.append(t1 - t0)
+        ad.append(t2 - t1)
+        r.ctx.close()
+    fw, ad = statistics.median(fw[1:]), statistics.median(ad[1:])
+    log(f"[adjoint] 257x129 Re 500: forward {fw * 1e3:.1f} ms ({r.iters} Newton iterations), "
+        f"adjoint gradient {ad * 1e3:.1f} ms, Cd {r.cd:.6f} Cl {r.cl:.6f}")
+    return {"workload": "NACA 4-digit (t 0.12, m 0.02, p 0.4) O-grid 257x129 to r 30, Re 500, "
+                        "alpha 5 deg, 32 Hicks-Henne amplitudes",
+            "forward_s": fw, "adjoint_gradient_s": ad, "params": 34, "newton_iters": r.iters,
+            "central_fd_equivalent_s": 2 * 34 * fw, "cd": r.cd, "cl": r.cl}
+
+
 def run_precision(v, xg, yg, prec, a):
     t0 = time.perf_counter()
     ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, prec)
@@ -200,6 +229,7 @@ def main():
     ap.add_argument("--re", type=float, default=40.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--refactor-every", type=int, default=0,
                     help="mixed: refactor the FP32 block LU every N Newton steps "
                          "(0 = only when GMRES on the stored one misses its tolerance)")
@@ -301,6 +331,8 @@ def main():
                        / 1e12}
         res["mixed_speedup_vs_fp64"] = f["seconds"] / mx["seconds"]
         res["cd_mixed_minus_fp64"] = mx["cd"] - f["cd"]
+    if not a.no_adjoint:
+        res["adjoint_c4"] = adjoint_timing(v)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(imax, jmax, last["iters"], a.re)
     print(json.dumps(res), flush=True)
