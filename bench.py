@@ -260,7 +260,7 @@ def main():
     upd_avg_ms = pr["update_ms"] / max(pr["launches"], 1)
     upd_tf = pr["flops"] / (pr["update_ms"] * 1e-3) / 1e12
     traffic = traffic_table()
-    kname = {"mfma64": "k_gj_update_mfma_b<float>", "valu64": "k_gj_update_b<float, 64>",
+    kname = {"mfma64": "k_gj_step_mfma_b<float>", "valu64": "k_gj_update_b<float, 64>",
              "valu128": "k_gj_update_b<float, 128>"}[min(mx["tune"], key=mx["tune"].get)]
     alg_upd = pr["bytes"] / max(pr["launches"], 1)
     ent = traffic.get(kname) or {}

@@ -153,3 +153,37 @@ def test_fields_io_format():
         rows = open(os.path.join(tmp, "forces.csv")).read().strip().splitlines()
         assert rows[0] == "t,cd,cl,cm,cdp,cdf,clp,clf,cmp,cmf" and len(rows) == 2
         assert rows[1].split(",")[0] == "1.0000000000e+00"
+
+
+def test_sweep_cases_and_elliptic_grids(C):
+    """Seeded sweep ranges; elliptic O-grids have the reference layout (seam
+    copies equal, wall semi-axes, far-field circle) and positive metric
+    Jacobians."""
+    from vorti2d_amd.batch import elliptic_grid, sweep_cases
+    cs = sweep_cases(64, seed=0)
+    assert len(cs) == 64 and cs == sweep_cases(64, seed=0)
+    assert all(129 <= c.imax <= 257 and 129 <= c.jmax <= 385 for c in cs)
+    assert all(0.5 <= c.aspect <= 1.0 and 20 <= c.re <= 45 and -10 <= c.alpha_deg <= 10 for c in cs)
+    xg, yg = elliptic_grid(33, 25, 0.6)
+    assert np.array_equal(xg[0], xg[-1]) and np.abs(yg[0] - yg[-1]).max() < 1e-12
+    assert abs(xg[:, 0].max() - 0.5) < 1e-12 and abs(yg[:, 0].max() - 0.3) < 1e-3
+    assert np.allclose(np.hypot(xg[:, -1], yg[:, -1]), 50.0)
+    m = C.compute_metrics(1 / 32, 1 / 24, xg, yg)
+    assert np.all(m[0] > 0) and np.all(np.isfinite(m[0]))
+
+
+def test_airfoil_grid_layout(C):
+    """NACA O-grid: seam copies equal, chord from -0.5 to 0.5, far field at r = 30,
+    positive metric Jacobians, bumps vanish at the leading / trailing edges."""
+    from vorti2d_amd.airfoil import AirfoilGrid
+    g = AirfoilGrid(65, 33, n_bumps=8)
+    xg, yg = g.grid(np.full(8, 2e-3))
+    assert np.array_equal(xg[0], xg[-1]) and np.abs(yg[0] - yg[-1]).max() < 1e-14
+    assert abs(xg[:, 0].min() + 0.5) < 1e-12 and abs(xg[:, 0].max() - 0.5) < 1e-12
+    assert np.allclose(np.hypot(xg[:, -1], yg[:, -1]), 30.0)
+    m = C.compute_metrics(1 / 64, 1 / 32, xg, yg)
+    assert np.all(m[0] > 0)
+    assert np.all(np.hypot(xg[:, 0], yg[:, 0]) > 1e-3)      # no wall node at the origin
+    nh = (65 - 1) // 2
+    assert np.all(g.basis[:, [0, nh, 64], 0] == 0.0)          # TE base, nose, seam
+    assert np.all(g.basis[:, :, -1] == 0.0)                    # far field fixed

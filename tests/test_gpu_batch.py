@@ -1,0 +1,52 @@
+"""GPU tests of the batched sweep driver: concurrent contexts reproduce single
+solves exactly, and the batch adjoint dCd/dalpha matches central differences."""
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_batch_equals_single_solves():
+    from vorti2d_amd.batch import Case, solve_batch, solve_case
+    cases = [Case(65, 49, 0.6, 25.0, 4.0), Case(97, 57, 1.0, 40.0, -8.0), Case(81, 65, 0.75, 33.0, 0.0),
+             Case(73, 41, 0.5, 45.0, 9.5), Case(65, 61, 0.9, 20.0, -3.0), Case(89, 53, 0.55, 30.0, 6.0)]
+    res, stats = solve_batch(cases, workers=4)
+    say(f"  batch of {len(cases)}: {stats}")
+    for c, r in zip(cases, res):
+        s = solve_case(c)
+        assert r.newton_iters == s.newton_iters
+        assert r.cd == s.cd and r.cl == s.cl
+        assert r.res <= 1e-10
+        if c.alpha_deg == 0.0:
+            assert abs(r.cl) < 1e-8
+
+
+def test_batch_fp64_matches_mixed():
+    """The mixed-precision sweep converges to the FP64 solution (Cd within 1e-6)."""
+    from vorti2d_amd.batch import Case, solve_case
+    for c in (Case(65, 49, 0.6, 25.0, 4.0), Case(97, 57, 1.0, 40.0, -8.0)):
+        m = solve_case(c, precision="mixed")
+        f = solve_case(c, precision="fp64")
+        assert abs(m.cd - f.cd) < 1e-6 and abs(m.cl - f.cl) < 1e-6
+
+
+def test_batch_adjoint_dcd_dalpha_vs_fd():
+    from vorti2d_amd.batch import Case, solve_case
+    c = Case(81, 57, 0.7, 35.0, 5.0)
+    r = solve_case(c, precision="mixed", tol=1e-13, adjoint=True)
+    h = 1e-6
+    cp = solve_case(Case(c.imax, c.jmax, c.aspect, c.re, c.alpha_deg + h), precision="fp64", tol=1e-13)
+    cm = solve_case(Case(c.imax, c.jmax, c.aspect, c.re, c.alpha_deg - h), precision="fp64", tol=1e-13)
+    fd = (cp.cd - cm.cd) / (2 * h)
+    say(f"  dCd/dalpha fd {fd:.8e} adjoint {r.dcd_dalpha:.8e}")
+    assert abs(fd - r.dcd_dalpha) <= 1e-5 * abs(fd)
