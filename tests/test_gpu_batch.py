@@ -1,5 +1,6 @@
 """GPU tests of the batched sweep driver: concurrent contexts reproduce single
-solves exactly, and the batch adjoint dCd/dalpha matches central differences."""
+solves (bit for bit in FP64, to the converged tolerance in mixed precision),
+and the batch adjoint dCd/dalpha matches central differences."""
 from __future__ import annotations
 
 import math
@@ -16,17 +17,40 @@ def say(msg):
     sys.stdout.flush()
 
 
-def test_batch_equals_single_solves():
-    from vorti2d_amd.batch import Case, solve_batch, solve_case
-    cases = [Case(65, 49, 0.6, 25.0, 4.0), Case(97, 57, 1.0, 40.0, -8.0), Case(81, 65, 0.75, 33.0, 0.0),
-             Case(73, 41, 0.5, 45.0, 9.5), Case(65, 61, 0.9, 20.0, -3.0), Case(89, 53, 0.55, 30.0, 6.0)]
-    res, stats = solve_batch(cases, workers=4)
-    say(f"  batch of {len(cases)}: {stats}")
+def _cases():
+    from vorti2d_amd.batch import Case
+    return [Case(65, 49, 0.6, 25.0, 4.0), Case(97, 57, 1.0, 40.0, -8.0), Case(81, 65, 0.75, 33.0, 0.0),
+            Case(73, 41, 0.5, 45.0, 9.5), Case(65, 61, 0.9, 20.0, -3.0), Case(89, 53, 0.55, 30.0, 6.0)]
+
+
+def test_batch_fp64_equals_single_solves():
+    """The FP64 path has no timing-dependent decision: concurrent contexts
+    reproduce the single solves bit for bit."""
+    from vorti2d_amd.batch import solve_batch, solve_case
+    cases = _cases()
+    res, stats = solve_batch(cases, workers=4, precision="fp64")
+    say(f"  fp64 batch of {len(cases)}: {stats}")
     for c, r in zip(cases, res):
-        s = solve_case(c)
+        s = solve_case(c, precision="fp64")
         assert r.newton_iters == s.newton_iters
         assert r.cd == s.cd and r.cl == s.cl
         assert r.res <= 1e-10
+        if c.alpha_deg == 0.0:
+            assert abs(r.cl) < 1e-8
+
+
+def test_batch_mixed_matches_single_solves():
+    """The mixed path refactors on measured timings, which concurrency changes:
+    batch and single solves agree to the converged tolerance."""
+    from vorti2d_amd.batch import solve_batch, solve_case
+    cases = _cases()
+    res, stats = solve_batch(cases, workers=4)
+    say(f"  mixed batch of {len(cases)}: {stats}")
+    for c, r in zip(cases, res):
+        s = solve_case(c)
+        say(f"    {c}: cd {r.cd:.17g} vs {s.cd:.17g}, iters {r.newton_iters} vs {s.newton_iters}")
+        assert r.res <= 1e-10 and s.res <= 1e-10
+        assert abs(r.cd - s.cd) <= 1e-9 * abs(s.cd) and abs(r.cl - s.cl) <= 1e-9 * max(1.0, abs(s.cl))
         if c.alpha_deg == 0.0:
             assert abs(r.cl) < 1e-8
 

@@ -3,8 +3,12 @@
 Every case owns a device context and therefore its own HIP stream; a host
 thread pool drives up to `workers` contexts at once (the ctypes calls release
 the GIL), so the small solves of a sweep overlap on the GPU instead of each
-leaving most of it idle.  A case runs exactly the kernels of a single solve,
-so its results equal the single-solve results bit for bit.
+leaving most of it idle.  A case runs exactly the kernels of a single solve.
+In FP64 its results equal the single-solve results bit for bit.  The mixed
+solver decides when to refactor from timings it measures on its own stream
+(refactorisation cost against iteration cost), and concurrency changes those
+timings, so a mixed case can take a different refactorisation path and agree
+with its single solve to the converged tolerance only.
 
 The reference has no batched driver; a case is the steady loop of
 vorti2d/solver.py:136-145 (stopping rule res = ||delta||_2 / (2N) <= tol) on an

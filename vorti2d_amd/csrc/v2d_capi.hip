@@ -17,13 +17,26 @@ namespace v2d {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
-// RAII device buffer for the host-pointer entry points
+// RAII device buffer: plain for the stateless host entry points (null
+// stream), stream-ordered (hipMallocAsync / hipFreeAsync) on a context's
+// stream, so that a context never synchronises the whole device.
 struct DBuf {
     void *p = nullptr;
-    ~DBuf() { if (p) (void)hipFree(p); }
+    hipStream_t s = nullptr;
+    ~DBuf()
+    {
+        if (!p) return;
+        if (s) (void)hipFreeAsync(p, s);
+        else (void)hipFree(p);
+    }
     template <typename T> T *as() { return static_cast<T *>(p); }
 };
 #define V2D_ALLOC(buf, bytes) V2D_HIP(hipMalloc(&(buf).p, std::max<size_t>((bytes), 8)))
+#define V2D_ALLOC_S(buf, bytes, st)                                                          \
+    do {                                                                                     \
+        (buf).s = (st);                                                                      \
+        V2D_HIP(hipMallocAsync(&(buf).p, std::max<size_t>((bytes), 8), (st)));               \
+    } while (0)
 
 static int check_grid(int imax, int jmax)
 {
@@ -123,8 +136,8 @@ struct Ctx {
     template <typename T> T *alloc(int64_t count, int &rc)
     {
         void *p = nullptr;
-        if (hipMalloc(&p, sizeof(T) * std::max<int64_t>(count, 1)) != hipSuccess) {
-            set_error("hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed");
+        if (hipMallocAsync(&p, sizeof(T) * std::max<int64_t>(count, 1), s) != hipSuccess) {
+            set_error("hipMallocAsync of " + std::to_string(sizeof(T) * count) + " bytes failed");
             rc = -1;
             return nullptr;
         }
@@ -134,9 +147,12 @@ struct Ctx {
     }
     ~Ctx()
     {
-        for (void *p : allocs) (void)hipFree(p);
-        direct_free_pattern(pat);
-        if (s) (void)hipStreamDestroy(s);
+        for (void *p : allocs) (void)hipFreeAsync(p, s);
+        direct_free_pattern(pat, s);
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
     }
     AsmArgs args(double re, double invdtau, double inv2dt, double urot) const
@@ -156,16 +172,18 @@ static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
     c->ff_bc = ff_bc; c->ca = ca; c->sa = sa;
     const int64_t N = c->N;
     std::vector<double> ex(c->imax), ey(c->imax);
-    V2D_HIP(hipMemcpy(ex.data(), c->m + 6 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(ey.data(), c->m + 7 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(ex.data(), c->m + 6 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(ey.data(), c->m + 7 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     std::vector<int> pp, po;
     outflow_flags(c->imax, ex.data(), ey.data(), ff_bc, ca, sa, c->hflags, pp, po, c->npsi_out);
-    V2D_HIP(hipMemcpy(c->flags, c->hflags.data(), c->imax, hipMemcpyHostToDevice));
-    V2D_HIP(hipMemcpy(c->pre_psi, pp.data(), sizeof(int) * c->imax, hipMemcpyHostToDevice));
-    V2D_HIP(hipMemcpy(c->pre_ome, po.data(), sizeof(int) * c->imax, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpyAsync(c->flags, c->hflags.data(), c->imax, hipMemcpyHostToDevice, c->s));
+    V2D_HIP(hipMemcpyAsync(c->pre_psi, pp.data(), sizeof(int) * c->imax, hipMemcpyHostToDevice, c->s));
+    V2D_HIP(hipMemcpyAsync(c->pre_ome, po.data(), sizeof(int) * c->imax, hipMemcpyHostToDevice, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     c->nnz = count_rows(c->imax, c->jmax, 0, 2 * N, c->hflags);
-    direct_free_pattern(c->pat);
-    if (direct_build_pattern(c->pat, c->imax, c->jmax, c->hflags.data())) return -1;
+    direct_free_pattern(c->pat, c->s);
+    if (direct_build_pattern(c->pat, c->imax, c->jmax, c->hflags.data(), c->s)) return -1;
     c->factored = false;
     return 0;
 }
@@ -190,6 +208,7 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
     g[0] = 1.0;
     int j = 0;
     double est = 1.0;
+    double hv = 1.0;   // host source of the async normalisation-scalar copies
     for (; j < m; ++j) {
         float *vj = c->gV + j * n, *zj = c->gZ + j * n, *w = c->gV + (j + 1) * n;
         if (direct_solve<float>(c->pat, c->wf, vpc, vj, zj, c->s)) return -1;
@@ -222,8 +241,8 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
         }
         for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
         H[(j + 1) * m + j] = hn;
-        {   // v_{j+1} = w / hn
-            const double hv = hn > 0.0 ? hn : 1.0;
+        {   // v_{j+1} = w / hn  (hv outlives the copy: the next iteration syncs before reuse)
+            hv = hn > 0.0 ? hn : 1.0;
             V2D_HIP(hipMemcpyAsync(c->red, &hv, sizeof(double), hipMemcpyHostToDevice, c->s));
             hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, w,
                                c->red, 1, w);
@@ -524,7 +543,9 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     c->dksi = 1.0 / (imax - 1);
     c->deta = 1.0 / (jmax - 1);
     const int64_t N = c->N;
-    V2D_HIP(hipStreamCreate(&c->s));
+    // non-blocking: independent contexts (a batch of solves) never wait on each
+    // other through the null stream
+    V2D_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto &e : c->ev) V2D_HIP(hipEventCreate(&e));
     int rc = 0;
     c->x = c->alloc<double>(N, rc); c->y = c->alloc<double>(N, rc);
@@ -538,12 +559,12 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     c->flags = c->alloc<uint8_t>(imax, rc);
     c->pre_psi = c->alloc<int>(imax, rc); c->pre_ome = c->alloc<int>(imax, rc);
     if (rc) return -1;
-    V2D_HIP(hipMemcpy(c->x, xg, sizeof(double) * N, hipMemcpyHostToDevice));
-    V2D_HIP(hipMemcpy(c->y, yg, sizeof(double) * N, hipMemcpyHostToDevice));
-    V2D_HIP(hipMemset(c->psi, 0, sizeof(double) * N));
-    V2D_HIP(hipMemset(c->ome, 0, sizeof(double) * N));
-    V2D_HIP(hipMemset(c->omeold, 0, sizeof(double) * N));
-    V2D_HIP(hipMemset(c->omeoldold, 0, sizeof(double) * N));
+    V2D_HIP(hipMemcpyAsync(c->x, xg, sizeof(double) * N, hipMemcpyHostToDevice, c->s));
+    V2D_HIP(hipMemcpyAsync(c->y, yg, sizeof(double) * N, hipMemcpyHostToDevice, c->s));
+    V2D_HIP(hipMemsetAsync(c->psi, 0, sizeof(double) * N, c->s));
+    V2D_HIP(hipMemsetAsync(c->ome, 0, sizeof(double) * N, c->s));
+    V2D_HIP(hipMemsetAsync(c->omeold, 0, sizeof(double) * N, c->s));
+    V2D_HIP(hipMemsetAsync(c->omeoldold, 0, sizeof(double) * N, c->s));
     if (v2d_compute_metrics_dev(imax, jmax, c->dksi, c->deta, c->x, c->y, c->m, c->s)) return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
     if (ctx_bc(c, ff_bc, ca, sa)) return -1;
@@ -578,7 +599,7 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     // pivot-ready flags (2 per precision) and the error word of the dataflow steps
     int *sync_words = c->alloc<int>(8, rc);
     if (rc) return -1;
-    V2D_HIP(hipMemset(sync_words, 0, sizeof(int) * 8));
+    V2D_HIP(hipMemsetAsync(sync_words, 0, sizeof(int) * 8, c->s));
     c->wd.flags = sync_words;
     c->wd.err = sync_words + 2;
     c->wf.flags = sync_words + 4;
@@ -603,7 +624,8 @@ int v2d_ctx_metrics(v2d_ctx *ctx, double *out)
 {
     Ctx *c = ctx->c.get();
     V2D_HIP(hipStreamSynchronize(c->s));
-    V2D_HIP(hipMemcpy(out, c->m, sizeof(double) * 10 * c->N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(out, c->m, sizeof(double) * 10 * c->N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -615,7 +637,8 @@ int v2d_ctx_set_state(v2d_ctx *ctx, const double *psi, const double *ome, const 
     double *dst[4] = {c->psi, c->ome, c->omeold, c->omeoldold};
     V2D_HIP(hipStreamSynchronize(c->s));
     for (int q = 0; q < 4; ++q)
-        if (src[q]) V2D_HIP(hipMemcpy(dst[q], src[q], sizeof(double) * c->N, hipMemcpyHostToDevice));
+        if (src[q]) V2D_HIP(hipMemcpyAsync(dst[q], src[q], sizeof(double) * c->N, hipMemcpyHostToDevice, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     // a new state makes the stored block LU (of an earlier Jacobian) stale
     if (psi || ome) {
         c->factored = false;
@@ -632,7 +655,8 @@ int v2d_ctx_get_state(v2d_ctx *ctx, double *psi, double *ome, double *omeold, do
     const double *src[4] = {c->psi, c->ome, c->omeold, c->omeoldold};
     V2D_HIP(hipStreamSynchronize(c->s));
     for (int q = 0; q < 4; ++q)
-        if (dst[q]) V2D_HIP(hipMemcpy(dst[q], src[q], sizeof(double) * c->N, hipMemcpyDeviceToHost));
+        if (dst[q]) V2D_HIP(hipMemcpyAsync(dst[q], src[q], sizeof(double) * c->N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -643,8 +667,9 @@ int v2d_ctx_init_state(v2d_ctx *ctx)
     V2D_HIP(hipStreamSynchronize(c->s));
     std::vector<double> psi(N, 0.0), z(N, 0.0);
     std::vector<double> xl(c->imax), yl(c->imax);
-    V2D_HIP(hipMemcpy(xl.data(), c->m + 8 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(yl.data(), c->m + 9 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(xl.data(), c->m + 8 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(yl.data(), c->m + 9 * N + (N - c->imax), sizeof(double) * c->imax, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     for (int i = 0; i < c->imax; ++i) psi[N - c->imax + i] = c->ca * yl[i] - c->sa * xl[i];
     return v2d_ctx_set_state(ctx, psi.data(), z.data(), z.data(), z.data());
 }
@@ -762,9 +787,9 @@ int v2d_ctx_export_coo(v2d_ctx *ctx, int *coo_i, int *coo_j, double *coo_v, doub
     Ctx *c = ctx->c.get();
     const int64_t N = c->N, nnz = c->nnz;
     DBuf dci, dcj, dcv;
-    V2D_ALLOC(dci, sizeof(int) * nnz);
-    V2D_ALLOC(dcj, sizeof(int) * nnz);
-    V2D_ALLOC(dcv, sizeof(double) * nnz);
+    V2D_ALLOC_S(dci, sizeof(int) * nnz, c->s);
+    V2D_ALLOC_S(dcj, sizeof(int) * nnz, c->s);
+    V2D_ALLOC_S(dcv, sizeof(double) * nnz, c->s);
     if (c->prec == V2D_FP64)
         hipLaunchKernelGGL(k_slots_to_coo<double>, dim3(grid_for(2 * N)), dim3(256), 0, c->s, c->imax,
                            c->jmax, c->flags, c->pre_psi, c->pre_ome, c->npsi_out,
@@ -775,10 +800,11 @@ int v2d_ctx_export_coo(v2d_ctx *ctx, int *coo_i, int *coo_j, double *coo_v, doub
                            static_cast<const float *>(c->val), dci.as<int>(), dcj.as<int>(), dcv.as<double>());
     V2D_LAUNCH();
     V2D_HIP(hipStreamSynchronize(c->s));
-    V2D_HIP(hipMemcpy(coo_i, dci.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(coo_j, dcj.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(coo_v, dcv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
-    if (bvec) V2D_HIP(hipMemcpy(bvec, c->rhs, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(coo_i, dci.p, sizeof(int) * nnz, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(coo_j, dcj.p, sizeof(int) * nnz, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(coo_v, dcv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->s));
+    if (bvec) V2D_HIP(hipMemcpyAsync(bvec, c->rhs, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -787,9 +813,9 @@ int v2d_ctx_export_csr(v2d_ctx *ctx, int64_t *indptr, int *indices, double *data
     Ctx *c = ctx->c.get();
     const int64_t N = c->N, nnz = c->nnz;
     DBuf dp, di, dd;
-    V2D_ALLOC(dp, sizeof(int64_t) * (2 * N + 1));
-    V2D_ALLOC(di, sizeof(int) * nnz);
-    V2D_ALLOC(dd, sizeof(double) * nnz);
+    V2D_ALLOC_S(dp, sizeof(int64_t) * (2 * N + 1), c->s);
+    V2D_ALLOC_S(di, sizeof(int) * nnz, c->s);
+    V2D_ALLOC_S(dd, sizeof(double) * nnz, c->s);
     if (c->prec == V2D_FP64)
         hipLaunchKernelGGL(k_slots_to_csr<double>, dim3(grid_for(2 * N + 1)), dim3(256), 0, c->s, c->imax,
                            c->jmax, c->flags, c->pre_psi, c->pre_ome, c->npsi_out,
@@ -800,9 +826,10 @@ int v2d_ctx_export_csr(v2d_ctx *ctx, int64_t *indptr, int *indices, double *data
                            static_cast<const float *>(c->val), dp.as<int64_t>(), di.as<int>(), dd.as<double>());
     V2D_LAUNCH();
     V2D_HIP(hipStreamSynchronize(c->s));
-    V2D_HIP(hipMemcpy(indptr, dp.p, sizeof(int64_t) * (2 * N + 1), hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(indices, di.p, sizeof(int) * nnz, hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(data, dd.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(indptr, dp.p, sizeof(int64_t) * (2 * N + 1), hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(indices, di.p, sizeof(int) * nnz, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(data, dd.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -811,14 +838,14 @@ int v2d_ctx_spmv(v2d_ctx *ctx, const double *x, double *y)
     Ctx *c = ctx->c.get();
     const int64_t n = 2 * c->N;
     DBuf dx, dy;
-    V2D_ALLOC(dx, sizeof(double) * n);
-    V2D_ALLOC(dy, sizeof(double) * n);
+    V2D_ALLOC_S(dx, sizeof(double) * n, c->s);
+    V2D_ALLOC_S(dy, sizeof(double) * n, c->s);
     V2D_HIP(hipStreamSynchronize(c->s));
-    V2D_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    V2D_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
     // FP64 vectors through the FP64-widening residual kernel: y = 0 - A x
     DBuf dz;
-    V2D_ALLOC(dz, sizeof(double) * n);
-    V2D_HIP(hipMemset(dz.p, 0, sizeof(double) * n));
+    V2D_ALLOC_S(dz, sizeof(double) * n, c->s);
+    V2D_HIP(hipMemsetAsync(dz.p, 0, sizeof(double) * n, c->s));
     if (c->prec == V2D_FP64)
         hipLaunchKernelGGL(k_residual<double>, dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax, c->jmax,
                            c->flags, static_cast<const double *>(c->val), dx.as<double>(), dz.as<double>(),
@@ -830,7 +857,8 @@ int v2d_ctx_spmv(v2d_ctx *ctx, const double *x, double *y)
     V2D_LAUNCH();
     V2D_HIP(hipStreamSynchronize(c->s));
     std::vector<double> t(n);
-    V2D_HIP(hipMemcpy(t.data(), dy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(t.data(), dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     for (int64_t q = 0; q < n; ++q) y[q] = -t[q];
     return 0;
 }
@@ -876,27 +904,29 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
     V2D_HIP(hipStreamSynchronize(c->s));
     if (c->prec == V2D_FP64) {
         DBuf db, dx;
-        V2D_ALLOC(db, sizeof(double) * n);
-        V2D_ALLOC(dx, sizeof(double) * n);
-        V2D_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        V2D_ALLOC_S(db, sizeof(double) * n, c->s);
+        V2D_ALLOC_S(dx, sizeof(double) * n, c->s);
+        V2D_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
         if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), db.as<double>(),
                                  dx.as<double>(), c->s))
             return -1;
         V2D_HIP(hipStreamSynchronize(c->s));
-        V2D_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        V2D_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+        V2D_HIP(hipStreamSynchronize(c->s));
         return 0;
     }
     std::vector<float> bf(n), xf(n);
     for (int64_t q = 0; q < n; ++q) bf[q] = (float)b[q];
     DBuf db, dx;
-    V2D_ALLOC(db, sizeof(float) * n);
-    V2D_ALLOC(dx, sizeof(float) * n);
-    V2D_HIP(hipMemcpy(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    V2D_ALLOC_S(db, sizeof(float) * n, c->s);
+    V2D_ALLOC_S(dx, sizeof(float) * n, c->s);
+    V2D_HIP(hipMemcpyAsync(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->s));
     if (direct_solve<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), db.as<float>(),
                             dx.as<float>(), c->s))
         return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
-    V2D_HIP(hipMemcpy(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     for (int64_t q = 0; q < n; ++q) x[q] = xf[q];
     return 0;
 }
@@ -1071,8 +1101,8 @@ int v2d_ctx_time_spmv(v2d_ctx *ctx, int launches, double *ms_per_launch)
     const int64_t n = 2 * c->N;
     DBuf dx, dy;
     const size_t vs = c->prec == V2D_FP64 ? sizeof(double) : sizeof(float);
-    V2D_ALLOC(dx, vs * n);
-    V2D_ALLOC(dy, vs * n);
+    V2D_ALLOC_S(dx, vs * n, c->s);
+    V2D_ALLOC_S(dy, vs * n, c->s);
     auto launch = [&]() {
         if (c->prec == V2D_FP64)
             hipLaunchKernelGGL((k_spmv<double, double>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
@@ -1112,27 +1142,29 @@ int v2d_ctx_solve_transpose(v2d_ctx *ctx, const double *b, double *x)
     V2D_HIP(hipStreamSynchronize(c->s));
     if (c->prec == V2D_FP64) {
         DBuf db, dx;
-        V2D_ALLOC(db, sizeof(double) * n);
-        V2D_ALLOC(dx, sizeof(double) * n);
-        V2D_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        V2D_ALLOC_S(db, sizeof(double) * n, c->s);
+        V2D_ALLOC_S(dx, sizeof(double) * n, c->s);
+        V2D_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
         if (direct_solve_t<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), db.as<double>(),
                                    dx.as<double>(), c->s))
             return -1;
         V2D_HIP(hipStreamSynchronize(c->s));
-        V2D_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        V2D_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+        V2D_HIP(hipStreamSynchronize(c->s));
         return 0;
     }
     std::vector<float> bf(n), xf(n);
     for (int64_t q = 0; q < n; ++q) bf[q] = (float)b[q];
     DBuf db, dx;
-    V2D_ALLOC(db, sizeof(float) * n);
-    V2D_ALLOC(dx, sizeof(float) * n);
-    V2D_HIP(hipMemcpy(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    V2D_ALLOC_S(db, sizeof(float) * n, c->s);
+    V2D_ALLOC_S(dx, sizeof(float) * n, c->s);
+    V2D_HIP(hipMemcpyAsync(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->s));
     if (direct_solve_t<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), db.as<float>(),
                               dx.as<float>(), c->s))
         return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
-    V2D_HIP(hipMemcpy(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     for (int64_t q = 0; q < n; ++q) x[q] = xf[q];
     return 0;
 }
@@ -1144,8 +1176,8 @@ static int residual_vjp_dev(Ctx *c, double re, double inv2dt, double urot, const
     const AsmArgs a = c->args(re, 0.0, inv2dt, urot);
     const unsigned nb = grid_for(c->N);
     DBuf part, red;
-    V2D_ALLOC(part, sizeof(double) * 2 * nb);
-    V2D_ALLOC(red, sizeof(double) * 2);
+    V2D_ALLOC_S(part, sizeof(double) * 2 * nb, c->s);
+    V2D_ALLOC_S(red, sizeof(double) * 2, c->s);
     hipLaunchKernelGGL(k_residual_vjp, dim3(nb), dim3(256), 0, c->s, a, c->flags, d_lam, d_gm, part.as<double>());
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, 2, part.as<double>(),
                        red.as<double>());
@@ -1161,13 +1193,14 @@ int v2d_ctx_residual_vjp(v2d_ctx *ctx, double re, double inv2dt, double urot, co
     Ctx *c = ctx->c.get();
     const int64_t N = c->N;
     DBuf dl, dg;
-    V2D_ALLOC(dl, sizeof(double) * 2 * N);
-    V2D_ALLOC(dg, sizeof(double) * 10 * N);
-    V2D_HIP(hipMemcpy(dl.p, lam, sizeof(double) * 2 * N, hipMemcpyHostToDevice));
+    V2D_ALLOC_S(dl, sizeof(double) * 2 * N, c->s);
+    V2D_ALLOC_S(dg, sizeof(double) * 10 * N, c->s);
+    V2D_HIP(hipMemcpyAsync(dl.p, lam, sizeof(double) * 2 * N, hipMemcpyHostToDevice, c->s));
     hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(10 * N)), dim3(256), 0, c->s, 10 * N, dg.as<double>());
     V2D_LAUNCH();
     if (residual_vjp_dev(c, re, inv2dt, urot, dl.as<double>(), dg.as<double>(), gscal)) return -1;
-    V2D_HIP(hipMemcpy(g_m, dg.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(g_m, dg.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -1233,14 +1266,15 @@ int v2d_ctx_forces_vjp(v2d_ctx *ctx, double re, double ref_length, double wd, do
     Ctx *c = ctx->c.get();
     const int64_t N = c->N;
     DBuf go, gm;
-    V2D_ALLOC(go, sizeof(double) * N);
-    V2D_ALLOC(gm, sizeof(double) * 10 * N);
+    V2D_ALLOC_S(go, sizeof(double) * N, c->s);
+    V2D_ALLOC_S(gm, sizeof(double) * 10 * N, c->s);
     hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(N)), dim3(256), 0, c->s, N, go.as<double>());
     hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(10 * N)), dim3(256), 0, c->s, 10 * N, gm.as<double>());
     V2D_LAUNCH();
     if (forces_vjp_dev(c, re, ref_length, wd, wl, go.as<double>(), gm.as<double>(), out3)) return -1;
-    V2D_HIP(hipMemcpy(g_ome, go.p, sizeof(double) * N, hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(g_m, gm.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(g_ome, go.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(g_m, gm.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -1257,11 +1291,11 @@ int v2d_ctx_adjoint_forces(v2d_ctx *ctx, double re, double urot, double ref_leng
     if (ctx_assemble(c, re, 0.0, 0.0, urot)) return -1;
     if (ctx_factor(c)) return -1;
     DBuf gu, lam, gm, gx, gy;
-    V2D_ALLOC(gu, sizeof(double) * 2 * N);
-    V2D_ALLOC(lam, sizeof(double) * 2 * N);
-    V2D_ALLOC(gm, sizeof(double) * 10 * N);
-    V2D_ALLOC(gx, sizeof(double) * N);
-    V2D_ALLOC(gy, sizeof(double) * N);
+    V2D_ALLOC_S(gu, sizeof(double) * 2 * N, c->s);
+    V2D_ALLOC_S(lam, sizeof(double) * 2 * N, c->s);
+    V2D_ALLOC_S(gm, sizeof(double) * 10 * N, c->s);
+    V2D_ALLOC_S(gx, sizeof(double) * N, c->s);
+    V2D_ALLOC_S(gy, sizeof(double) * N, c->s);
     hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(2 * N)), dim3(256), 0, c->s, 2 * N, gu.as<double>());
     hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(10 * N)), dim3(256), 0, c->s, 10 * N, gm.as<double>());
     V2D_LAUNCH();
@@ -1276,8 +1310,9 @@ int v2d_ctx_adjoint_forces(v2d_ctx *ctx, double re, double urot, double ref_leng
     if (metrics_vjp_dev(c->imax, c->jmax, c->dksi, c->deta, c->x, c->y, gm.as<double>(), gx.as<double>(),
                         gy.as<double>(), c->s))
         return -1;
-    V2D_HIP(hipMemcpy(g_x, gx.p, sizeof(double) * N, hipMemcpyDeviceToHost));
-    V2D_HIP(hipMemcpy(g_y, gy.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    V2D_HIP(hipMemcpyAsync(g_x, gx.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(g_y, gy.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
     gscal[0] = rs[1] * (M_PI / 180.0);   // per degree of alpha
     gscal[1] = f3[2] + rs[0];            // per unit Re
     coef[0] = f3[0];

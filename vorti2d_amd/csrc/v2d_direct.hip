@@ -29,7 +29,7 @@ namespace v2d {
 constexpr int NB = 64;   // Gauss-Jordan block width == update tile edge
 
 // ------------------------------------------------------------ pattern -----
-int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags)
+int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags, hipStream_t s)
 {
     P.L = make_levels(imax, jmax);
     const Levels &L = P.L;
@@ -105,28 +105,29 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
             }
         }
     }
-    auto up = [](int **dptr, int2 **dent, const std::vector<int> &ptr,
+    auto up = [s](int **dptr, int2 **dent, const std::vector<int> &ptr,
                  const std::vector<int2> &ent) -> int {
-        V2D_HIP(hipMalloc(dptr, sizeof(int) * ptr.size()));
-        V2D_HIP(hipMemcpy(*dptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
-        V2D_HIP(hipMalloc(dent, sizeof(int2) * std::max<size_t>(ent.size(), 1)));
+        V2D_HIP(hipMallocAsync(dptr, sizeof(int) * ptr.size(), s));
+        V2D_HIP(hipMemcpyAsync(*dptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, s));
+        V2D_HIP(hipMallocAsync(dent, sizeof(int2) * std::max<size_t>(ent.size(), 1), s));
         if (!ent.empty())
-            V2D_HIP(hipMemcpy(*dent, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+            V2D_HIP(hipMemcpyAsync(*dent, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice, s));
         return 0;
     };
     if (up(&P.d_Dptr, &P.d_D, cD, eD) || up(&P.d_Lptr, &P.d_Lr, cL, eL) ||
         up(&P.d_Uptr, &P.d_Ur, cU, eU) || up(&P.d_UCptr, &P.d_Uc, cUC, eUC) ||
         up(&P.d_LCptr, &P.d_Lc, cLC, eLC))
         return -1;
+    V2D_HIP(hipStreamSynchronize(s));   // the host lists above are locals
     return 0;
 }
 
-void direct_free_pattern(DirectPattern &P)
+void direct_free_pattern(DirectPattern &P, hipStream_t s)
 {
     int *ptrs[] = {P.d_Dptr, P.d_Lptr, P.d_Uptr, P.d_UCptr, P.d_LCptr};
     int2 *ents[] = {P.d_D, P.d_Lr, P.d_Ur, P.d_Uc, P.d_Lc};
-    for (int *p : ptrs) if (p) (void)hipFree(p);
-    for (int2 *p : ents) if (p) (void)hipFree(p);
+    for (int *p : ptrs) if (p) (void)hipFreeAsync(p, s);
+    for (int2 *p : ents) if (p) (void)hipFreeAsync(p, s);
     P.d_Dptr = P.d_Lptr = P.d_Uptr = P.d_UCptr = P.d_LCptr = nullptr;
     P.d_D = P.d_Lr = P.d_Ur = P.d_Uc = P.d_Lc = nullptr;
 }

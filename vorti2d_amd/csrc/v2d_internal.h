@@ -155,8 +155,9 @@ struct DirectPattern {
     int2 *d_D = nullptr, *d_Lr = nullptr, *d_Ur = nullptr, *d_Uc = nullptr, *d_Lc = nullptr;
 };
 
-int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags_host);
-void direct_free_pattern(DirectPattern &P);
+// Device lists are allocated and freed in stream order on s.
+int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags_host, hipStream_t s);
+void direct_free_pattern(DirectPattern &P, hipStream_t s);
 
 // Gauss-Jordan update kernels: VALU with 64- or 128-wide output tiles, or
 // MFMA 16x16x4 (FP32 / FP64 matrix cores) on 64-wide tiles.
