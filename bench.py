@@ -178,6 +178,27 @@ def adjoint_timing(v, reps=3):
             "central_fd_equivalent_s": 2 * 34 * fw, "cd": r.cd, "cl": r.cl}
 
 
+def sweep_timing(workers=8, n=64):
+    """The sweep workload: 64 elliptic O-grids from numpy default_rng(0) (angular
+    129..257, radial 129..385 nodes, aspect 0.5..1, Re 20..45, alpha -10..10 deg),
+    each a mixed-precision steady solve to res <= 1e-10 plus the adjoint
+    dCd/dalpha, up to `workers` contexts (streams) in flight.  One untimed pass
+    of the first `workers` cases warms the allocator and the code objects."""
+    from vorti2d_amd.batch import solve_batch, sweep_cases
+    cases = sweep_cases(n, seed=0)
+    solve_batch(cases[:workers], workers=workers, adjoint=True)
+    res, st = solve_batch(cases, workers=workers, adjoint=True)
+    log(f"[sweep] {n} elliptic O-grids, {workers} streams: {st['solves_per_s']:.2f} solves/s, "
+        f"p50 {st['p50_s'] * 1e3:.0f} ms, p99 {st['p99_s'] * 1e3:.0f} ms")
+    return {"workload": f"{n} elliptic O-grids (default_rng(0): angular 129..257, radial 129..385, "
+                        "aspect 0.5..1, Re 20..45, alpha -10..10 deg), mixed steady solve to "
+                        "res <= 1e-10 + adjoint dCd/dalpha per case",
+            "solves_per_s": st["solves_per_s"], "wall_s": st["wall_s"], "p50_s": st["p50_s"],
+            "p99_s": st["p99_s"], "streams": workers,
+            "newton_iters_mean": sum(r.newton_iters for r in res) / len(res),
+            "max_res": max(r.res for r in res)}
+
+
 def run_precision(v, xg, yg, prec, a):
     t0 = time.perf_counter()
     ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, prec)
@@ -230,6 +251,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--refactor-every", type=int, default=0,
                     help="mixed: refactor the FP32 block LU every N Newton steps "
                          "(0 = only when GMRES on the stored one misses its tolerance)")
@@ -333,6 +355,8 @@ def main():
         res["cd_mixed_minus_fp64"] = mx["cd"] - f["cd"]
     if not a.no_adjoint:
         res["adjoint_c4"] = adjoint_timing(v)
+    if not a.no_sweep:
+        res["sweep_c5"] = sweep_timing()
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(imax, jmax, last["iters"], a.re)
     print(json.dumps(res), flush=True)
