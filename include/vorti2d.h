@@ -126,6 +126,16 @@ int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max);
  * factorisation. */
 enum { V2D_UPDATE_VALU64 = 0, V2D_UPDATE_VALU128 = 1, V2D_UPDATE_MFMA64 = 2 };
 int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind);
+/* Kernel of the block-LU solve sweeps: two launches per level step
+ * (V2D_SWEEP_LEVELS, the default), or one persistent launch for the whole
+ * solve (V2D_SWEEP_PERSISTENT: one workgroup per compute unit, each owning a
+ * slice of every level, synchronised through per-workgroup progress words).  The persistent launch needs all
+ * its workgroups resident at once; if another persistent launch holds the GPU
+ * its bounded waits expire and the solve is redone with per-level launches,
+ * which the context then keeps.  A tuning knob with no counterpart in the
+ * reference: both compute the same sums in the same order. */
+enum { V2D_SWEEP_LEVELS = 0, V2D_SWEEP_PERSISTENT = 1 };
+int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind);
 /* Factorise the last assembled Jacobian once with every Gauss-Jordan update
  * launch bracketed by HIP events on the context's stream (no counterpart in the
  * reference; measurement only).  out[5] = {update launches, summed update

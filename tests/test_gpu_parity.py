@@ -222,6 +222,35 @@ def test_direct_solve_fp32_update_kernels(v, C, oracle_mod, imax, jmax, kind, ff
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("imax,jmax,kind,ff", [(17, 4, "cyl", 0), (17, 5, "ann", 1), (41, 23, "cyl", 1),
+                                               (73, 40, "ann", 2), (1025, 6, "ann", 0)])
+def test_persistent_sweep_equals_level_sweeps(v, C, oracle_mod, prec, imax, jmax, kind, ff):
+    """The one-launch persistent solve (ticketed level steps, device counters)
+    forms the same sums in the same order as one launch pair per level step:
+    the solutions are bit-identical.  Shapes: 2 and 3 levels, odd and even
+    level counts, and 4100-unknown levels (rows past the register prefetch,
+    GEMV rows past the prefetched width)."""
+    say(f"persistent sweep {prec} {imax}x{jmax} {kind} ff={ff}")
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, prec, ff)
+    ctx.assemble(40.0, 0.0, 2.5, 0.1)
+    ci, cj, cv, b = ctx.export_coo()
+    ctx.factor()
+    ctx.set_sweep_kernel("levels")
+    x_lv = ctx.solve(b)
+    ctx.set_sweep_kernel("persistent")
+    x_ps = ctx.solve(b)
+    x_ps2 = ctx.solve(b)
+    assert np.all(np.isfinite(x_ps))
+    assert np.array_equal(x_ps, x_lv)
+    assert np.array_equal(x_ps, x_ps2)
+    N = imax * jmax
+    A = sp.csc_matrix((cv, (ci, cj)), shape=(2 * N, 2 * N))
+    rel = np.linalg.norm(b - A @ x_ps) / np.linalg.norm(b)
+    say(f"  relative residual {rel:.2e}")
+    assert rel <= (1e-10 if prec == "fp64" else 5e-2)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_stored_factorisation_is_self_contained(v, C, oracle_mod, prec):
     """A stored block LU keeps solving the system it was built from after later
     assemblies overwrite the Jacobian (its solve sweeps must not read the new

@@ -145,6 +145,7 @@ struct Ctx {
         bytes += sizeof(T) * std::max<int64_t>(count, 1);
         return static_cast<T *>(p);
     }
+    int *err_host = nullptr;   // pinned: the solve error words of wd, wf
     ~Ctx()
     {
         for (void *p : allocs) (void)hipFreeAsync(p, s);
@@ -153,6 +154,7 @@ struct Ctx {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
         }
+        if (err_host) (void)hipHostFree(err_host);
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
     }
     AsmArgs args(double re, double invdtau, double inv2dt, double urot) const
@@ -185,6 +187,18 @@ static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
     direct_free_pattern(c->pat, c->s);
     if (direct_build_pattern(c->pat, c->imax, c->jmax, c->hflags.data(), c->s)) return -1;
     c->factored = false;
+    return 0;
+}
+
+// A persistent solve sweep that waited past its spin limit leaves 2 in the
+// error word; direct_solve copies the word to pinned memory after every solve,
+// so it is current after the caller's next synchronisation.
+static int sweep_failed(const int *err_host)
+{
+    if (err_host && *err_host) {
+        set_error("block-LU solve: a sweep step waited past its spin limit (results invalid)");
+        return 1;
+    }
     return 0;
 }
 
@@ -233,6 +247,7 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
         double wn2 = 0.0;
         V2D_HIP(hipMemcpyAsync(&wn2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
         V2D_HIP(hipStreamSynchronize(c->s));
+        if (sweep_failed(c->wf.err_host)) return -1;
         const double hn = std::sqrt(std::max(wn2, 0.0));
         if (!std::isfinite(hn)) {   // overflow in the FP32 vectors: not converged
             *iters = j + 1;
@@ -581,6 +596,11 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->wd.rowk2 = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
         c->wd.dinv = c->alloc<double>(2 * 64 * 64, rc);
         c->wd.z = c->alloc<double>(2 * N, rc);
+        c->wd.wv = c->alloc<double>(2 * N, rc);
+        c->wd.cfL = c->alloc<double>(kCouple * 2 * N, rc);
+        c->wd.cfU = c->alloc<double>(kCouple * 2 * N, rc);
+        c->wd.yv = c->alloc<double>(2 * N, rc);
+        c->wd.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
     } else {
         c->wf.sinv = c->alloc<float>(c->pat.sinv_elems, rc);
         c->wf.tmat = c->alloc<float>(2 * nm * c->pat.ldmax, rc);
@@ -590,6 +610,11 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->wf.rowk2 = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
         c->wf.dinv = c->alloc<float>(2 * 64 * 64, rc);
         c->wf.z = c->alloc<float>(2 * N, rc);
+        c->wf.wv = c->alloc<float>(2 * N, rc);
+        c->wf.cfL = c->alloc<float>(kCouple * 2 * N, rc);
+        c->wf.cfU = c->alloc<float>(kCouple * 2 * N, rc);
+        c->wf.yv = c->alloc<float>(2 * N, rc);
+        c->wf.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
         c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);
         c->gZ = c->alloc<float>((int64_t)c->gm_cap * 2 * N, rc);
         c->gb = c->alloc<float>(2 * N, rc);
@@ -604,6 +629,15 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     c->wd.err = sync_words + 2;
     c->wf.flags = sync_words + 4;
     c->wf.err = sync_words + 6;
+    // Per-level sweep launches by default: on a 1025^2 mesh they take 17 us
+    // per level step against 25 us for the persistent launch, whose every step
+    // pays two cross-die release / acquire round trips.
+    c->wd.sweep_kind = c->wf.sweep_kind = 0;
+    V2D_HIP(hipHostMalloc((void **)&c->err_host, 2 * sizeof(int), hipHostMallocDefault));
+    c->err_host[0] = c->err_host[1] = 0;
+    c->wd.err_host = c->err_host;
+    c->wf.err_host = c->err_host + 1;
+    V2D_HIP(hipStreamSynchronize(c->s));
     *out = h.release();
     return 0;
 }
@@ -715,6 +749,17 @@ static int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double
         hipLaunchKernelGGL(k_assemble<float>, dim3(grid_for(c->N)), dim3(256), 0, c->s, a, k, c->flags,
                            static_cast<float *>(c->val), c->rhs);
     V2D_LAUNCH();
+    return 0;
+}
+
+int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind)
+{
+    if (kind != V2D_SWEEP_LEVELS && kind != V2D_SWEEP_PERSISTENT) {
+        set_error("sweep kernel must be V2D_SWEEP_LEVELS or V2D_SWEEP_PERSISTENT");
+        return -1;
+    }
+    ctx->c->wd.sweep_kind = kind;
+    ctx->c->wf.sweep_kind = kind;
     return 0;
 }
 
@@ -911,6 +956,7 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
                                  dx.as<double>(), c->s))
             return -1;
         V2D_HIP(hipStreamSynchronize(c->s));
+        if (sweep_failed(c->wd.err_host)) return -1;
         V2D_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
         V2D_HIP(hipStreamSynchronize(c->s));
         return 0;
@@ -925,6 +971,7 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
                             dx.as<float>(), c->s))
         return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
+    if (sweep_failed(c->wf.err_host)) return -1;
     V2D_HIP(hipMemcpyAsync(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->s));
     V2D_HIP(hipStreamSynchronize(c->s));
     for (int64_t q = 0; q < n; ++q) x[q] = xf[q];
@@ -1022,6 +1069,7 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
     V2D_LAUNCH();
     double dn = 0.0;
     if (dnorm2(c, c->delta, n, &dn)) return -1;
+    if (sweep_failed(c->prec == V2D_FP64 ? c->wd.err_host : c->wf.err_host)) return -1;
     float a_ms = 0.f, t_ms = 0.f;
     V2D_HIP(hipEventElapsedTime(&a_ms, c->ev[0], c->ev[1]));
     if (c->fac_in_step) V2D_HIP(hipEventElapsedTime(&fac_ms, c->ev[4], c->ev[5]));

@@ -28,6 +28,47 @@ namespace v2d {
 
 constexpr int NB = 64;   // Gauss-Jordan block width == update tile edge
 
+// ----------------------------------------------- persistent solve sweeps ---
+// Level shapes in closed form (levels 0 and nlev-1 hold two j rows, the others
+// one); the pattern build checks them against its own tables.
+V2D_HD inline int lvl_ld(int n) { return (n + 63) / 64 * 64; }
+V2D_HD inline int64_t lvl_voff(const Levels &L, int l) { return (int64_t)2 * L.imax * (l + (l > 0)); }
+V2D_HD inline int64_t lvl_soff(const Levels &L, int l)
+{
+    if (l == 0) return 0;
+    const int n0 = L.size(0), n1 = 2 * L.imax;
+    return (int64_t)n0 * lvl_ld(n0) + (int64_t)(l - 1) * n1 * lvl_ld(n1);
+}
+
+// The levels of sweep step s: forward steps s < steps (both chains toward the
+// meeting level M), the middle level at s == steps, then the backward steps
+// outward from M.  term: 1 = coupling to level l-1, 2 = to level l+1.
+struct SwJob {
+    int l, term;
+};
+V2D_HD inline int sweep_jobs(int s, int nlev, int M, int steps, SwJob (&jb)[2], int &phase)
+{
+    const int nbot = M, ntop = nlev - 1 - M;
+    int nj = 0;
+    if (s < steps) {
+        phase = 0;
+        if (s < nbot) jb[nj++] = SwJob{s, s > 0 ? 1 : 0};
+        if (s < ntop) jb[nj++] = SwJob{nlev - 1 - s, s > 0 ? 2 : 0};
+    } else if (s == steps) {
+        phase = 1;
+        jb[nj++] = SwJob{M, (M > 0 ? 1 : 0) | (M < nlev - 1 ? 2 : 0)};
+    } else {
+        phase = 2;
+        const int st = s - steps - 1;
+        if (M - 1 - st >= 0) jb[nj++] = SwJob{M - 1 - st, 2};
+        if (M + 1 + st < nlev) jb[nj++] = SwJob{M + 1 + st, 1};
+    }
+    return nj;
+}
+
+constexpr int kSwThreads = 1024;              // persistent sweep workgroup
+constexpr int kSwRows = kSwThreads / 64;      // rows per wavefront pass
+
 // ------------------------------------------------------------ pattern -----
 int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags, hipStream_t s)
 {
@@ -118,18 +159,79 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
         up(&P.d_Uptr, &P.d_Ur, cU, eU) || up(&P.d_UCptr, &P.d_Uc, cUC, eUC) ||
         up(&P.d_LCptr, &P.d_Lc, cLC, eLC))
         return -1;
+    // tables of the persistent solve sweeps
+    P.sweep_ok = false;
+    bool closed = nlev >= 2;
+    for (int l = 0; l < nlev && closed; ++l)
+        closed = P.ld[l] == lvl_ld(P.n[l]) && P.voff[l] == lvl_voff(L, l) && P.soff[l] == lvl_soff(L, l);
+    const int M = (nlev - 1) / 2;
+    const int steps = std::max(M, nlev - 1 - M);
+    P.nsteps = 2 * steps + 1;
+    std::vector<int> ixL, ixU, vxL, vxU;
+    if (closed) {
+        const int64_t tot = (int64_t)kCouple * n2;
+        ixL.assign(tot, 0); ixU.assign(tot, 0); vxL.assign(tot, -1); vxU.assign(tot, -1);
+        auto pad = [&](const std::vector<int> &ptr, const std::vector<int2> &ent, int l, int lc,
+                       std::vector<int> &ix, std::vector<int> &vx) -> bool {
+            const int nl = P.n[l];
+            const int64_t base = (int64_t)kCouple * P.voff[l];
+            for (int r = 0; r < nl; ++r) {
+                const int64_t R = P.voff[l] + r;
+                const int e0 = ptr[R], e1 = ptr[R + 1];
+                if (e1 - e0 > kCouple) return false;
+                if (e1 > e0 && (lc < 0 || lc >= nlev)) return false;
+                for (int t = 0; t < kCouple; ++t) {
+                    const int64_t q = base + (int64_t)t * nl + r;
+                    if (e0 + t < e1) {
+                        ix[q] = (int)(P.voff[lc] + ent[e0 + t].x);
+                        vx[q] = ent[e0 + t].y;
+                    } else {
+                        ix[q] = (lc >= 0 && lc < nlev) ? (int)P.voff[lc] : 0;   // any valid source
+                    }
+                }
+            }
+            return true;
+        };
+        for (int l = 0; l < nlev; ++l)
+            if (!pad(cL, eL, l, l - 1, ixL, vxL) || !pad(cU, eU, l, l + 1, ixU, vxU)) {
+                set_error("direct solver: a row couples to more than " + std::to_string(kCouple) +
+                          " unknowns of a neighbouring level");
+                return -1;
+            }
+        // C workgroups per chain, one per compute unit: at most 16 rows of an
+        // interior level each, at most 128 rows of any level (8 lanes per row
+        // build the right-hand side)
+        int dev = 0, ncu = 0;
+        V2D_HIP(hipGetDevice(&dev));
+        V2D_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        P.sweep_C = std::max(1, std::min(ncu / 2, (2 * imax + kSwRows - 1) / kSwRows));
+        P.sweep_rows = 1;
+        for (int l = 0; l < nlev; ++l) P.sweep_rows = std::max(P.sweep_rows, (P.n[l] + P.sweep_C - 1) / P.sweep_C);
+        P.sweep_grid = 2 * P.sweep_C;
+        auto upi = [s](int **d, const std::vector<int> &h) -> int {
+            V2D_HIP(hipMallocAsync(d, sizeof(int) * h.size(), s));
+            V2D_HIP(hipMemcpyAsync(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s));
+            return 0;
+        };
+        if (upi(&P.d_ixL, ixL) || upi(&P.d_ixU, ixU) || upi(&P.d_vxL, vxL) || upi(&P.d_vxU, vxU))
+            return -1;
+        P.sweep_ok = P.sweep_rows <= kSwThreads / 8 && ncu >= 2;
+    }
     V2D_HIP(hipStreamSynchronize(s));   // the host lists above are locals
     return 0;
 }
 
 void direct_free_pattern(DirectPattern &P, hipStream_t s)
 {
-    int *ptrs[] = {P.d_Dptr, P.d_Lptr, P.d_Uptr, P.d_UCptr, P.d_LCptr};
+    int *ptrs[] = {P.d_Dptr, P.d_Lptr, P.d_Uptr, P.d_UCptr, P.d_LCptr,
+                   P.d_ixL, P.d_ixU, P.d_vxL, P.d_vxU};
     int2 *ents[] = {P.d_D, P.d_Lr, P.d_Ur, P.d_Uc, P.d_Lc};
     for (int *p : ptrs) if (p) (void)hipFreeAsync(p, s);
     for (int2 *p : ents) if (p) (void)hipFreeAsync(p, s);
     P.d_Dptr = P.d_Lptr = P.d_Uptr = P.d_UCptr = P.d_LCptr = nullptr;
+    P.d_ixL = P.d_ixU = P.d_vxL = P.d_vxU = nullptr;
     P.d_D = P.d_Lr = P.d_Ur = P.d_Uc = P.d_Lc = nullptr;
+    P.sweep_ok = false;
 }
 
 
@@ -987,6 +1089,8 @@ static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, b
 
 template <typename T>
 static int direct_factor_levels(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
+template <typename T>
+int direct_couplings(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
 
 __global__ void k_set_int(int *p, int v)
 {
@@ -996,11 +1100,10 @@ __global__ void k_set_int(int *p, int v)
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
 {
-    if (W.update_kind == kUpdateMfma64) {
-        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, W.err, 0);
-        V2D_LAUNCH();
-    }
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, W.err, 0);
+    V2D_LAUNCH();
     if (direct_factor_levels(P, W, val, s)) return -1;
+    if (direct_couplings(P, W, val, s)) return -1;
     if (W.update_kind == kUpdateMfma64) {
         int e = 0;
         V2D_HIP(hipMemcpyAsync(&e, W.err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1213,13 +1316,349 @@ static int launch_sweep(const SweepBatch<T> &B, int nj, int nmax, const DirectPa
     return 0;
 }
 
+// ------------------------------------------ persistent sweep kernel -----
+// The whole solve (every forward step, the middle level, every backward step)
+// in one launch of 2 C workgroups, C = min(half the compute units, 2 imax /
+// 16): workgroups [0, C) run the wall chain (and the middle level), [C, 2C)
+// the far-field chain, and workgroup q of a chain owns rows [q n / C,
+// (q + 1) n / C) of its chain's level at every step.  Each workgroup publishes
+// its progress in its own word (2 s + 1: its part of the level right-hand side
+// of step s is stored; 2 s + 2: its rows of step s are stored; release), and
+// waiting workgroups poll the words of the chain in parallel, so no counter is
+// contended.  Per step:
+//   1. every wavefront issues the loads of its S_l^-1 row, the threads those
+//      of their coupling entries and right-hand side (registers);
+//   Y. wait until the producing chain has stored step s-1 (the middle level
+//      waits for both chains, the first backward step for the middle level);
+//      the workgroup's rows of the level right-hand side (forward: b_l -
+//      couplings . z; backward: couplings . w), eight lanes per row summing
+//      in list order, go to a level-ordered buffer;
+//   D. wait until the chain has stored all of y_l, load it into LDS, form the
+//      workgroup's rows of S_l^-1 y (FP64 accumulation) and store them
+//      (forward: z_l; middle and backward: w_l and the solution).
+// Every wait is bounded (error word 2 on expiry; the host then solves again
+// with per-level launches): the mapping needs all 2 C workgroups resident at
+// once, one per compute unit, which an idle GPU gives.
+template <typename T>
+struct SweepArgs {
+    const T *S;
+    const T *cfL, *cfU;
+    const int *ixL, *ixU;
+    const T *b;
+    T *z, *w, *x, *y;
+    int *flag, *err;
+    Levels L;
+    int64_t N;
+    int M, steps, nsteps, C, ys_len, rows_max;
+};
+
+// Wait until flag[first .. first + count) >= target: the first waves poll one
+// word per lane; bounded, error word 2 on expiry (every later wait then gives
+// up at once, so a failed launch drains quickly).
+__device__ __forceinline__ void sweep_wait(const int *flag, int first, int count, int target, int *err)
+{
+    const int tid = threadIdx.x, wave = tid >> 6;
+    if (wave * 64 < count) {
+        int spins = 0;
+        for (;;) {
+            const int i = tid;
+            const bool mine =
+                i >= count || __hip_atomic_load(flag + first + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+            if (__all(mine)) break;
+            if (++spins > kSpinLimit ||
+                ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// An empty asm that "changes" a register-held vector: the compiler cannot
+// move work on it (the FP64 conversion of a prefetched FP32 row) above here.
+__device__ __forceinline__ void pin(float4 &v)
+{
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+}
+__device__ __forceinline__ void pin(double2 &v)
+{
+    asm volatile("" : "+v"(v.x), "+v"(v.y));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
+{
+    using V = typename Vec16<T>::type;
+    constexpr int W = Vec16<T>::w;
+    extern __shared__ unsigned char smem[];
+    T *ys = reinterpret_cast<T *>(smem);
+    T *so = ys + A.ys_len;   // the workgroup's results of a step
+    const int C = A.C;
+    const int chain = blockIdx.x >= C, q = blockIdx.x - chain * C;
+    const int nlev = A.L.nlev, M = A.M;
+    for (int s = 0; s < A.nsteps; ++s) {
+        // lane-dependent offsets are cheap to form per step; an empty asm
+        // keeps the compiler from hoisting (and then spilling) them
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int wave = tid >> 6, lane = tid & 63;
+        // this chain's level at step s (-1: none), its coupling terms, phase
+        int l = -1, term = 0, phase = 0;
+        if (s < A.steps) {
+            if (chain == 0 && s < M) { l = s; term = s > 0 ? 1 : 0; }
+            if (chain == 1 && s < nlev - 1 - M) { l = nlev - 1 - s; term = s > 0 ? 2 : 0; }
+        } else if (s == A.steps) {
+            phase = 1;
+            if (chain == 0) { l = M; term = (M > 0 ? 1 : 0) | (M < nlev - 1 ? 2 : 0); }
+        } else {
+            phase = 2;
+            const int st = s - A.steps - 1;
+            if (chain == 0 && M - 1 - st >= 0) { l = M - 1 - st; term = 2; }
+            if (chain == 1 && M + 1 + st < nlev) { l = M + 1 + st; term = 1; }
+        }
+        if (l < 0) {   // nothing for this chain: the step is done at once
+            if (tid == 0) __hip_atomic_store(A.flag + blockIdx.x, 2 * s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        const int n = A.L.size(l), ld = lvl_ld(n);
+        const int r0 = (int)((int64_t)q * n / C), r1 = (int)((int64_t)(q + 1) * n / C);
+        const int64_t vo = lvl_voff(A.L, l);
+        const T *Sl = A.S + lvl_soff(A.L, l);
+        const int row = r0 + wave;
+        // 1. loads that do not depend on earlier steps (indices clamped into
+        //    the level: every load in bounds, none branched)
+        const V *Sr = reinterpret_cast<const V *>(Sl + (int64_t)min(row, n - 1) * ld);
+        const int nv = (n + W - 1) / W;
+        V pre[kSweepPre];
+#pragma unroll
+        for (int u = 0; u < kSweepPre; ++u) pre[u] = Sr[min(lane + 64 * u, nv - 1)];
+#pragma unroll
+        for (int u = 0; u < kSweepPre; ++u)
+            if (lane + 64 * u >= nv) pre[u] = V{};
+        // an item of 17 rows (n = 16 C + k): its 17th row is prefetched one
+        // 16-byte vector per lane by wavefronts 0..8 (u = wave)
+        const bool x17 = r1 - r0 == kSwRows + 1;
+        V pre17 = V{};
+        if (x17 && wave < kSweepPre) {
+            const V *R17 = reinterpret_cast<const V *>(Sl + (int64_t)(r0 + kSwRows) * ld);
+            pre17 = R17[min(lane + 64 * wave, nv - 1)];
+            if (lane + 64 * wave >= nv) pre17 = V{};
+        }
+        // y part: eight lanes per row, lane k < 4 entry k of the first term,
+        // k >= 4 entry k - 4 of the second (the middle level's U term)
+        const int yr = r0 + (tid >> 3), yk = tid & 7;
+        const bool yon = yr < r1;
+        const int yrc = min(yr, n - 1);
+        const bool first_term = yk < 4;
+        const bool use = term != 0 && (first_term || term == 3);
+        const T *cft = (first_term ? ((term & 1) ? A.cfL : A.cfU) : A.cfU) + (int64_t)kCouple * vo;
+        const int *ixt = (first_term ? ((term & 1) ? A.ixL : A.ixU) : A.ixU) + (int64_t)kCouple * vo;
+        const int e = yk & 3;
+        const T ycf = cft[e * n + yrc];
+        const int yix = ixt[e * n + yrc];
+        const T yb = A.b[A.L.global(yrc, A.N, l)];
+        // Y. the producing chain has stored step s - 1
+        if (s > 0) {
+            const int first = (s == A.steps || s == A.steps + 1) ? 0 : chain * C;
+            const int count = s == A.steps ? 2 * C : C;
+            sweep_wait(A.flag, first, count, 2 * s, A.err);
+        }
+        {
+            const T *src = phase == 2 ? A.w : A.z;
+            const double p = use ? (double)ycf * (double)src[yix] : 0.0;
+            // in list order: first term entries 0..3, then the second term's
+            double acc = p;
+#pragma unroll
+            for (int k = 1; k < 8; ++k) {
+                const double pk = __shfl(p, (lane & ~7) + k, 64);
+                if (k < 4 || term == 3) acc += pk;
+            }
+            if (yk == 0 && yon) so[yr - r0] = phase < 2 ? (T)((double)yb - acc) : (T)acc;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (int r = r0 + lane; r < r1; r += 64) A.y[vo + r] = so[r - r0];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) __hip_atomic_store(A.flag + blockIdx.x, 2 * s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // D. the chain has stored all of y_l
+        sweep_wait(A.flag, chain * C, C, 2 * s + 1, A.err);
+        for (int r = tid; r < n; r += kSwThreads) ys[r] = A.y[vo + r];
+        // zero the tail the dot products read (up to the prefetch width)
+        for (int r = n + tid; r < max(nv, 64 * kSweepPre) * W; r += kSwThreads) ys[r] = T(0);
+        __syncthreads();
+        // this wavefront's rows of S_l^-1 y: the prefetched one, then any
+        // further rows of a wide level streamed, with the same split into two
+        // partial sums as the per-level kernel (prefetched entries past the
+        // row are zero and meet zeros in ys)
+        // the 17th row: every wavefront u < 9 forms its product of that row
+        // (lane l: columns l + 64 u), wavefront 0 sums them below in the
+        // order of the per-level kernel
+        double *x17p = reinterpret_cast<double *>(so + A.rows_max);   // [9][64]
+        if (x17) {
+            if (wave < kSweepPre) {
+                pin(pre17);
+                x17p[wave * 64 + lane] = dot16<T>(pre17, ys + (lane + 64 * wave) * W);
+            }
+            __syncthreads();
+        }
+        for (int r = row; r < r1; r += kSwRows) {
+            const V *Rr = reinterpret_cast<const V *>(Sl + (int64_t)r * ld);
+            double acc0 = 0.0, acc1 = 0.0;
+            if (x17 && r == r0 + kSwRows) {
+#pragma unroll
+                for (int u = 0; u < kSweepPre; ++u) {
+                    if (u & 1) acc1 += x17p[u * 64 + lane];
+                    else acc0 += x17p[u * 64 + lane];
+                }
+            } else if (r == row) {
+                // three groups, so that the LDS operands of all nine are not
+                // held at once beside the prefetched row
+#pragma unroll
+                for (int u = 0; u < kSweepPre; ++u) {
+                    pin(pre[u]);
+                    if (u & 1) acc1 += dot16<T>(pre[u], ys + (lane + 64 * u) * W);
+                    else acc0 += dot16<T>(pre[u], ys + (lane + 64 * u) * W);
+                    if (u % 3 == 2) __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+                for (int u = 0; u < kSweepPre; ++u) {
+                    const int cu = lane + 64 * u;
+                    if (cu < nv) {
+                        if (u & 1) acc1 += dot16<T>(Rr[cu], ys + cu * W);
+                        else acc0 += dot16<T>(Rr[cu], ys + cu * W);
+                    }
+                }
+            }
+            for (int c = lane + 64 * kSweepPre; c < nv; c += 128) {
+                acc0 += dot16<T>(Rr[c], ys + c * W);
+                if (c + 64 < nv) acc1 += dot16<T>(Rr[c + 64], ys + (c + 64) * W);
+            }
+            const double acc = wsum(acc0 + acc1);
+            if (lane == 0) so[r - r0] = phase == 2 ? (T)((double)A.z[vo + r] - acc) : (T)acc;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (int r = r0 + lane; r < r1; r += 64) {
+                const T o = so[r - r0];
+                if (phase == 0) {
+                    A.z[vo + r] = o;
+                } else {
+                    A.w[vo + r] = o;
+                    A.x[A.L.global(r, A.N, l)] = o;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) __hip_atomic_store(A.flag + blockIdx.x, 2 * s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();   // so[] and ys[] are rewritten by the next step
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_couplings(int64_t n, const int *__restrict__ vxL,
+                                                        const int *__restrict__ vxU, const T *__restrict__ val,
+                                                        T *__restrict__ cfL, T *__restrict__ cfU)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int a = vxL[q], b = vxU[q];
+    cfL[q] = a >= 0 ? val[a] : T(0);
+    cfU[q] = b >= 0 ? val[b] : T(0);
+}
+
+__global__ void k_zero_ints(int *p, int n)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) p[q] = 0;
+}
+
+template <typename T>
+int direct_couplings(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
+{
+    if (!P.sweep_ok || !W.cfL) return 0;
+    const int64_t n = (int64_t)kCouple * 2 * P.N;
+    hipLaunchKernelGGL(k_fill_couplings<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, P.d_vxL,
+                       P.d_vxU, val, W.cfL, W.cfU);
+    V2D_LAUNCH();
+    return 0;
+}
+
+// dynamic LDS of k_sweep: the level right-hand side, at least as wide as the
+// prefetched part of a row
+template <typename T>
+static int sweep_ys(const DirectPattern &P)
+{
+    return std::max(P.ldmax, 64 * kSweepPre * Vec16<T>::w);
+}
+// ys, the results of the item (rows rounded up to an even count so that the
+// FP64 scratch of a 17th row after them is 8-byte aligned), that scratch
+template <typename T>
+static size_t sweep_lds(const DirectPattern &P)
+{
+    return sizeof(T) * ((size_t)sweep_ys<T>(P) + (P.sweep_rows + 1) / 2 * 2) + sizeof(double) * 64 * kSweepPre;
+}
+
+template <typename T>
+static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
+                               hipStream_t s);
+
+// One persistent launch; if one of its bounded waits expired (the workgroups
+// were not all resident at once) the solve is redone with per-level launches
+// and the work object keeps to them.
+template <typename T>
+static int direct_solve_persistent(const DirectPattern &P, DirectWork<T> &W, const T *val, const T *b, T *x,
+                                   hipStream_t s)
+{
+    const int nlev = P.L.nlev;
+    SweepArgs<T> A{};
+    A.S = W.sinv;
+    A.cfL = W.cfL; A.cfU = W.cfU;
+    A.ixL = P.d_ixL; A.ixU = P.d_ixU;
+    A.b = b; A.z = W.z; A.w = W.wv; A.x = x; A.y = W.yv;
+    A.flag = W.ctr; A.err = W.err;
+    A.L = P.L; A.N = P.N;
+    A.M = (nlev - 1) / 2;
+    A.steps = std::max(A.M, nlev - 1 - A.M);
+    A.nsteps = P.nsteps;
+    A.C = P.sweep_C;
+    A.ys_len = sweep_ys<T>(P);
+    A.rows_max = (P.sweep_rows + 1) / 2 * 2;
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(1024), 0, s, W.ctr, P.sweep_grid);
+    hipLaunchKernelGGL(k_sweep<T>, dim3(P.sweep_grid), dim3(kSwThreads), sweep_lds<T>(P), s, A);
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpyAsync(W.err_host, W.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    V2D_HIP(hipStreamSynchronize(s));
+    if (*W.err_host == 2) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, W.err, 0);
+        V2D_LAUNCH();
+        *W.err_host = 0;
+        W.sweep_kind = 0;
+        return direct_solve_levels(P, W, val, b, x, s);
+    }
+    return 0;
+}
+
 // Two sweeps over the twisted factorisation:
 //   forward  bottom: z_l = S_l^-1 (b_l - L_l z_{l-1}),  top: z_l = S_l^-1 (b_l - U_l z_{l+1})
 //   middle        : x_M = S_M^-1 (b_M - L_M z_{M-1} - U_M z_{M+1})
 //   backward bottom: x_l = z_l - S_l^-1 U_l x_{l+1},   top: x_l = z_l - S_l^-1 L_l x_{l-1}
+// sweep_kind 1 runs them as one persistent launch (k_sweep), 0 as one launch
+// pair per level step.
 template <typename T>
-int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
-                 hipStream_t s)
+int direct_solve(const DirectPattern &P, DirectWork<T> &W, const T *val, const T *b, T *x, hipStream_t s)
+{
+    const bool lds_fits = sweep_lds<T>(P) <= 65536;
+    if (W.sweep_kind == 1 && P.sweep_ok && lds_fits && W.cfL && W.wv && W.yv && W.ctr && W.err_host)
+        return direct_solve_persistent(P, W, val, b, x, s);
+    return direct_solve_levels(P, W, val, b, x, s);
+}
+
+template <typename T>
+static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
+                               hipStream_t s)
 {
     const int nlev = P.L.nlev;
     if (sizeof(T) * (size_t)P.ldmax > 65536) {
@@ -1416,9 +1855,9 @@ template int direct_solve_t<double>(const DirectPattern &, const DirectWork<doub
 template int direct_solve_t<float>(const DirectPattern &, const DirectWork<float> &, const float *,
                                    const float *, float *, hipStream_t);
 
-template int direct_solve<double>(const DirectPattern &, const DirectWork<double> &, const double *,
+template int direct_solve<double>(const DirectPattern &, DirectWork<double> &, const double *,
                                   const double *, double *, hipStream_t);
-template int direct_solve<float>(const DirectPattern &, const DirectWork<float> &, const float *,
+template int direct_solve<float>(const DirectPattern &, DirectWork<float> &, const float *,
                                  const float *, float *, hipStream_t);
 
 }  // namespace v2d

@@ -153,7 +153,19 @@ struct DirectPattern {
     // device CSR (rows / columns indexed in level order), entries (local, vidx)
     int *d_Dptr = nullptr, *d_Lptr = nullptr, *d_Uptr = nullptr, *d_UCptr = nullptr, *d_LCptr = nullptr;
     int2 *d_D = nullptr, *d_Lr = nullptr, *d_Ur = nullptr, *d_Uc = nullptr, *d_Lc = nullptr;
+    // Solve sweeps in one persistent launch: the couplings of every level row
+    // into level l-1 (L) and l+1 (U), padded to kCouple entries, stored
+    // [kCouple][n_l] at kCouple * voff[l]: the level-ordered source index and
+    // the value index (-1: padding).
+    int *d_ixL = nullptr, *d_ixU = nullptr, *d_vxL = nullptr, *d_vxU = nullptr;
+    int nsteps = 0;          // sweep steps: forward, middle level, backward
+    int sweep_C = 1;         // workgroups per elimination chain
+    int sweep_grid = 0;      // persistent workgroups: 2 sweep_C, one per compute unit
+    int sweep_rows = 1;      // most rows of one workgroup at one level
+    bool sweep_ok = false;   // closed-form level offsets verified, tables built
 };
+
+constexpr int kCouple = 4;   // widest inter-level coupling of a row (host-checked)
 
 // Device lists are allocated and freed in stream order on s.
 int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags_host, hipStream_t s);
@@ -191,13 +203,19 @@ struct DirectWork {
     int epoch = 0;          // value the current step's pivot block publishes
     int update_kind = kUpdateMfma64;
     LaunchProbe *probe = nullptr;   // set to time the update launches
+    // persistent solve sweeps (sweep_kind 1; 0: one launch pair per level step)
+    int sweep_kind = 1;
+    T *cfL = nullptr, *cfU = nullptr;   // coupling values of the factored Jacobian (padded lists)
+    T *wv = nullptr;                    // level-ordered backward-sweep results
+    T *yv = nullptr;                    // level-ordered right-hand sides of the sweep steps
+    int *ctr = nullptr;                 // progress word of every sweep workgroup
+    int *err_host = nullptr;            // pinned copy of err, refreshed after every solve
 };
 
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
 template <typename T>
-int direct_solve(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,
-                 hipStream_t s);
+int direct_solve(const DirectPattern &P, DirectWork<T> &W, const T *val, const T *b, T *x, hipStream_t s);
 // Solve A^T x = b with the same stored factorisation.
 template <typename T>
 int direct_solve_t(const DirectPattern &P, const DirectWork<T> &W, const T *val, const T *b, T *x,

@@ -14,6 +14,7 @@ from ._lib import check, f64c, lib, ptr
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
 UPDATE_KERNELS = {"valu64": 0, "valu128": 1, "mfma64": 2}
+SWEEP_KERNELS = {"levels": 0, "persistent": 1}
 
 
 class DeviceContext:
@@ -86,6 +87,11 @@ class DeviceContext:
         """Gauss-Jordan update kernel of the block LU: "valu64", "valu128" or
         "mfma64" (the default)."""
         check(lib.v2d_ctx_set_update_kernel(self._h, UPDATE_KERNELS.get(kind, kind)))
+
+    def set_sweep_kernel(self, kind: str | int):
+        """Block-LU solve sweeps: "levels" (two launches per level step, the
+        default) or "persistent" (one launch per solve)."""
+        check(lib.v2d_ctx_set_sweep_kernel(self._h, SWEEP_KERNELS.get(kind, kind)))
 
     def autotune_update_kernel(self, re: float, kinds=("mfma64", "valu64")) -> dict:
         """Time one block-LU factorisation of the current state's Jacobian with
