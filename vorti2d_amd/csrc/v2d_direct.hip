@@ -333,19 +333,6 @@ __global__ void __launch_bounds__(256) k_schur_S_b(Batch<T> B, const T *__restri
 }
 
 // --------------------------------------------------- Gauss-Jordan ---------
-// Inverse of a kb x kb (kb <= 64) block, unpivoted Gauss-Jordan, held in
-// registers: thread t owns column c = t % 64 of the 16 consecutive rows
-// r = 16*(t/64) + q.  Per pivot p the owners of row p and column p publish
-// them to LDS (double-buffered, so one barrier per pivot suffices), every
-// thread reads the pivot, its row entry and its 16 column entries in one
-// round trip, and spends one FMA per entry
-//     a[r][c] -= a[r][p] * (a[p][c] / a[p][p]).
-// The pivot column is left unscaled during the sweep (later pivots act on it
-// linearly) and multiplied by -1/a[p][p] once at the end (each thread keeps
-// the factor of its own column), with its diagonal entry set to -1 so that it
-// ends as 1/a[p][p]; only the pivot-row entry of a thread needs a select.
-// Entries outside kb x kb must be zero on entry; the result is left in a[] and
-// written to out (row stride NB).
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x)
 {
@@ -354,58 +341,137 @@ __device__ __forceinline__ double recip(double x)
     return fma(fma(-x, r, 1.0), r, r);
 }
 
+template <typename T> struct Mfma;
+
+// Inverse of the leading kb x kb block (kb <= 64) of a 64 x 64 tile held in
+// LDS (sa, row stride lda), in place, by blocked Gauss-Jordan without
+// pivoting over 16-wide steps K = [16 k, 16 k + 16):
+//   1. wavefront 0 inverts the diagonal block D = A[K, K] in registers: lane l
+//      holds column l % 16 of rows 4 (l / 16) + q, q < 4; per pivot p the
+//      pivot row, pivot value and the lane's pivot-column entries arrive by
+//      cross-lane shuffles (no barrier); the pivot column is scaled by
+//      -1 / a[p][p] once at the end, its diagonal entry set to -1 on the way
+//      so that it ends as 1 / a[p][p];
+//   2. on the matrix cores (16x16x4 blocks, operands straight from LDS),
+//      wavefronts 0..2 form the new pivot rows R = D^-1 A[K, J] and pivot
+//      columns C = -A[I, K] D^-1 of the three 16-blocks I, J outside K;
+//   3. C replaces A[I, K]; the four wavefronts form the nine trailing blocks
+//      A[I, J] + C_I A[K, J] with the pivot rows still old;
+//   4. the trailing blocks and R are stored.
+// Five barriers per step.  Padding rows and columns (kb .. 63) are zero on
+// entry; they are set to the identity for the sweep (so the tile is block
+// diagonal) and to zero again at the end.  256 threads.  On exit
+// a[q] = inverse[16 (t / 64) + q][t % 64] and out (row stride 64) holds the
+// inverse.
 template <typename T>
-__device__ __forceinline__ void gj_regs(T (&a)[16], int kb, T *prow, T *pcol, T *out)
+__device__ __forceinline__ typename Mfma<T>::C mm16(const T *sa, int lda, int xr, int xc, int yr, int yc,
+                                                    typename Mfma<T>::C acc)
 {
-    const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
-    T sc = T(0);
-    if (rg == 0) prow[c] = a[0];
-    if (c == 0)
+    // acc += X Y, X = sa[xr.., xc..], Y = sa[yr.., yc..] (16 x 16 each);
+    // element v of lane l: row drow(l / 16, v), column l % 16
+    const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) pcol[rg * 16 + q] = a[q];
-    for (int pw = 0; pw < 4; ++pw) {
+    for (int h = 0; h < 4; ++h)
+        acc = Mfma<T>::opd(sa[(xr + li) * lda + xc + 4 * h + lk], sa[(yr + 4 * h + lk) * lda + yc + li], acc);
+    return acc;
+}
+
+template <typename T>
+__device__ void gj_tile(T *sa, int lda, int kb, T (&a)[16], T *out)
+{
+    using M = Mfma<T>;
+    using C = typename M::C;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    __syncthreads();
+    for (int r = kb + t; r < NB; r += 256) sa[r * lda + r] = T(1);
+    __syncthreads();
+    const int nk = (kb + 15) / 16;
+    for (int k = 0; k < nk; ++k) {
+        const int K0 = 16 * k;
+        auto ob = [k](int o) { return 16 * (o < k ? o : o + 1); };   // the three blocks outside K
+        if (wave == 0) {
+            const int c = lane & 15, g = lane >> 4;
+            T d[4];
 #pragma unroll
-        for (int pq = 0; pq < 16; ++pq) {
-            const int p = pw * 16 + pq;
-            if (p < kb) {
-                const T *pr = prow + (p & 1) * NB, *pc = pcol + (p & 1) * NB;
-                __syncthreads();
-                const T piv = pr[p];
-                const T prc = pr[c];
-                T cr[16];
+            for (int q = 0; q < 4; ++q) d[q] = sa[(K0 + 4 * g + q) * lda + K0 + c];
+            T sc = T(0);
 #pragma unroll
-                for (int q = 0; q < 16; ++q) cr[q] = pc[rg * 16 + q];
+            for (int p = 0; p < 16; ++p) {
+                const int pg = p >> 2, pq = p & 3;
+                const T prow = __shfl(d[pq], pg * 16 + c, 64);
+                const T piv = __shfl(d[pq], pg * 16 + p, 64);
+                T pc[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) pc[q] = __shfl(d[q], g * 16 + p, 64);
                 const T ip = recip(piv);
                 const bool isp = c == p;
-                const T rp = isp ? T(0) : prc * ip;
+                const T rp = isp ? T(0) : prow * ip;
                 sc = isp ? -ip : sc;
 #pragma unroll
-                for (int q = 0; q < 16; ++q) a[q] = fma(-cr[q], rp, a[q]);
-                const T rowv = isp ? T(-1) : rp;
-                a[pq] = (rg == pw) ? rowv : a[pq];
-                // publish pivot p + 1 into the other buffer
-                const int p1 = p + 1;
-                if (p1 < kb) {
-                    T *pr1 = prow + (p1 & 1) * NB, *pc1 = pcol + (p1 & 1) * NB;
-                    if (rg == (pq == 15 ? pw + 1 : pw)) pr1[c] = a[(pq + 1) & 15];
-                    if (c == p1)
+                for (int q = 0; q < 4; ++q) d[q] = fma(-pc[q], rp, d[q]);
+                if (g == pg) d[pq] = isp ? T(-1) : rp;
+            }
 #pragma unroll
-                        for (int q = 0; q < 16; ++q) pc1[rg * 16 + q] = a[q];
-                }
+            for (int q = 0; q < 4; ++q) sa[(K0 + 4 * g + q) * lda + K0 + c] = d[q] * sc;
+        }
+        __syncthreads();
+        C rt = {}, ct = {};
+        if (wave < 3) {
+            const int B0 = ob(wave);
+            rt = mm16<T>(sa, lda, K0, K0, K0, B0, rt);   // D^-1 A[K, B]
+            ct = mm16<T>(sa, lda, B0, K0, K0, K0, ct);   // A[B, K] D^-1
+        }
+        __syncthreads();
+        if (wave < 3) {
+            const int B0 = ob(wave);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sa[(B0 + M::drow(lk, v)) * lda + K0 + li] = -ct[v];
+        }
+        __syncthreads();
+        C tt[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int ti = wave + 4 * u;
+            if (ti < 9) {
+                const int I0 = ob(ti / 3), J0 = ob(ti % 3);
+                C acc;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[v] = sa[(I0 + M::drow(lk, v)) * lda + J0 + li];
+                tt[u] = mm16<T>(sa, lda, I0, K0, K0, J0, acc);
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int ti = wave + 4 * u;
+            if (ti < 9) {
+                const int I0 = ob(ti / 3), J0 = ob(ti % 3);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sa[(I0 + M::drow(lk, v)) * lda + J0 + li] = tt[u][v];
+            }
+        }
+        if (wave < 3) {
+            const int B0 = ob(wave);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sa[(K0 + M::drow(lk, v)) * lda + B0 + li] = rt[v];
+        }
+        __syncthreads();
     }
+    const int c = lane, rg = wave;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        a[q] = (c < kb) ? a[q] * sc : T(0);
-        out[(rg * 16 + q) * NB + c] = a[q];
+        const int r = rg * 16 + q;
+        a[q] = (r < kb && c < kb) ? sa[r * lda + c] : T(0);
+        out[r * NB + c] = a[q];
     }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_gj_diag_b(Batch<T> B, int k0)
 {
-    __shared__ T prow[2 * NB], pcol[2 * NB];
+    constexpr int LT = NB + 4;   // padded LDS row stride
+    __shared__ T sa[NB * LT];
     const Job<T> &J = B.j[blockIdx.z];
     const int n = J.n, ld = J.ld;
     if (k0 >= n) return;
@@ -415,9 +481,9 @@ __global__ void __launch_bounds__(256) k_gj_diag_b(Batch<T> B, int k0)
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int r = rg * 16 + q;
-        a[q] = (r < kb && c < kb) ? J.S[(int64_t)(k0 + r) * ld + k0 + c] : T(0);
+        sa[r * LT + c] = (r < kb && c < kb) ? J.S[(int64_t)(k0 + r) * ld + k0 + c] : T(0);
     }
-    gj_regs(a, kb, prow, pcol, J.dinv);
+    gj_tile(sa, LT, kb, a, J.dinv);
 }
 
 // 16-byte vectors of T
@@ -608,12 +674,8 @@ __global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
                 if (r >= 0 && r < NB && c >= 0 && c < NB)
                     sc[r * NB + c] = (r < kb2 && c < kb2) ? acc[a][b] : T(0);
             }
-        __syncthreads();
-        const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
         T g[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) g[q] = sc[(rg * 16 + q) * NB + c];
-        gj_regs(g, kb2, sr, sr + 2 * NB, J.dinv);
+        gj_tile(sc, NB, kb2, g, J.dinv);
     }
 }
 
@@ -637,7 +699,6 @@ __global__ void __launch_bounds__(256) k_gj_update_b(Batch<T> B, int k0)
 // values are taken as zero; operand entries with t >= kb are zeroed in
 // registers; rows past n read a valid row and are not stored; columns past n
 // only feed unstored outputs.
-template <typename T> struct Mfma;
 template <> struct Mfma<float> {
     using C = float __attribute__((ext_vector_type(4)));
     static constexpr bool kSwap = true;
@@ -695,7 +756,6 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
     constexpr int VW = Vec8<T>::w;
     constexpr int LT = NB + 4;    // padded LDS row stride
     __shared__ T sT[NB * LT];     // next pivot block, or a tile stored transposed
-    __shared__ T sx[4 * NB];      // gj_regs scratch
     const int z = blockIdx.x % B.nj;
     const Job<T> &J = B.j[z];
     T *__restrict__ A = J.S;
@@ -846,12 +906,9 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
                     const int c = (w & 1) * 32 + bj * 16 + M::col(li, lk, v);
                     sT[r * LT + c] = (r < kb2 && c < kb2) ? acc[bi][bj][v] : T(0);
                 }
-        __syncthreads();
         const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
         T g[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) g[q] = sT[(rg * 16 + q) * LT + c];
-        gj_regs(g, kb2, sx, sx + 2 * NB, J.dinv);
+        gj_tile(sT, LT, kb2, g, J.dinv);
         T *rk = J.rowk_w + (int64_t)(k1 + c) * NB + rg * 16;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
