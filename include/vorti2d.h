@@ -136,6 +136,13 @@ int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind);
  * reference: both compute the same sums in the same order. */
 enum { V2D_SWEEP_LEVELS = 0, V2D_SWEEP_PERSISTENT = 1 };
 int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind);
+/* Mixed precision: run the FP32 block-LU apply of every GMRES iteration as one
+ * replay of a HIP graph captured over fixed device buffers (enable != 0, the
+ * default; per-level sweeps only), instead of launching its 2 (levels - 1)
+ * kernels one by one.  Same kernels, same results; it takes the launches off
+ * the host thread, which matters when many contexts run at once.  No
+ * counterpart in the reference. */
+int v2d_ctx_set_graph(v2d_ctx *ctx, int enable);
 /* Factorise the last assembled Jacobian once with every Gauss-Jordan update
  * launch bracketed by HIP events on the context's stream (no counterpart in the
  * reference; measurement only).  out[5] = {update launches, summed update

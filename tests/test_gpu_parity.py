@@ -250,6 +250,40 @@ def test_persistent_sweep_equals_level_sweeps(v, C, oracle_mod, prec, imax, jmax
     assert rel <= (1e-10 if prec == "fp64" else 5e-2)
 
 
+def test_graph_replayed_gmres_equals_eager(v, C, oracle_mod):
+    """The mixed Newton step with the block-LU apply replayed from a captured
+    HIP graph equals the step with the apply's kernels launched one by one, bit
+    for bit, over several steps (each replay followed by eager work of the
+    step: SpMV, orthogonalisation, host synchronisation), across
+    refactorisations, and after switching graphs off and on again."""
+    imax, jmax = 97, 49
+    xg, yg = oracle_mod.generate_annulus(imax, jmax)
+    hist = {}
+    for mode in ("eager", "graph", "toggled"):
+        ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, "mixed")
+        # refactor every step: the iteration counts then do not depend on
+        # measured timings, and the graph is replayed across refactorisations
+        ctx.set_solver(refactor_every=1, gmres_rtol=1e-8, gmres_max_iter=30)
+        ctx.set_graph(mode != "eager")
+        ctx.init_state()
+        out = []
+        for k in range(5):
+            if mode == "toggled" and k == 2:
+                ctx.set_graph(False)
+                ctx.set_graph(True)
+            res, rn, its = ctx.newton_step(40.0)
+            out.append((res, rn, its))
+        out.append(tuple(ctx.get_state(("psi", "ome"))))
+        hist[mode] = out
+        ctx.close()
+    say(f"  eager {[h[:3] for h in hist['eager'][:5]]}")
+    for mode in ("graph", "toggled"):
+        for a, b in zip(hist["eager"][:5], hist[mode][:5]):
+            assert a == b, (mode, a, b)
+        assert np.array_equal(hist["eager"][5][0], hist[mode][5][0])
+        assert np.array_equal(hist["eager"][5][1], hist[mode][5][1])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_stored_factorisation_is_self_contained(v, C, oracle_mod, prec):
     """A stored block LU keeps solving the system it was built from after later

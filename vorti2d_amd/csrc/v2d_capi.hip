@@ -114,7 +114,21 @@ struct Ctx {
     int gm_max = 30;
     double gm_rtol = 1e-6;
     int refactor_every = 1;
-    float *gV = nullptr, *gZ = nullptr, *gb = nullptr, *gx = nullptr;
+    float *gV = nullptr, *gZ = nullptr, *gb = nullptr;
+    // the FP32 block-LU apply as a captured graph over fixed buffers
+    // (ga_in -> ga_out), replayed by GMRES; built on first use, dropped when
+    // the pattern or the sweep kernel changes
+    bool use_graph = true;
+    float *ga_in = nullptr, *ga_out = nullptr;
+    hipGraph_t ag = nullptr;
+    hipGraphExec_t age = nullptr;
+    void drop_graph()
+    {
+        if (age) (void)hipGraphExecDestroy(age);
+        if (ag) (void)hipGraphDestroy(ag);
+        age = nullptr;
+        ag = nullptr;
+    }
     double *gxd = nullptr;
     bool factored = false;
     int since_factor = 0;
@@ -148,6 +162,7 @@ struct Ctx {
     int *err_host = nullptr;   // pinned: the solve error words of wd, wf
     ~Ctx()
     {
+        drop_graph();
         for (void *p : allocs) (void)hipFreeAsync(p, s);
         direct_free_pattern(pat, s);
         if (s) {
@@ -184,6 +199,7 @@ static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
     V2D_HIP(hipMemcpyAsync(c->pre_ome, po.data(), sizeof(int) * c->imax, hipMemcpyHostToDevice, c->s));
     V2D_HIP(hipStreamSynchronize(c->s));
     c->nnz = count_rows(c->imax, c->jmax, 0, 2 * N, c->hflags);
+    c->drop_graph();
     direct_free_pattern(c->pat, c->s);
     if (direct_build_pattern(c->pat, c->imax, c->jmax, c->hflags.data(), c->s)) return -1;
     c->factored = false;
@@ -202,6 +218,36 @@ static int sweep_failed(const int *err_host)
     return 0;
 }
 
+// z = M^-1 v with the stored FP32 block LU.  With graphs on and the per-level
+// sweeps, the apply is one replay of a graph captured over ga_in -> ga_out
+// (captured on first use; its kernels read the factorisation through fixed
+// pointers, so it stays valid across refactorisations) between two
+// device-to-device copies; otherwise the kernels are launched directly.
+static int apply_pc(Ctx *c, const float *v, float *z)
+{
+    const float *vpc = static_cast<const float *>(c->val_pc);
+    const int64_t n = 2 * c->N;
+    if (!c->use_graph || c->wf.sweep_kind != 0 || !c->ga_in || !c->ga_out)
+        return direct_solve<float>(c->pat, c->wf, vpc, v, z, c->s);
+    if (!c->age) {
+        V2D_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        const int rc = direct_solve<float>(c->pat, c->wf, vpc, c->ga_in, c->ga_out, c->s);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->s, &g);
+        if (rc || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (!rc) set_error(std::string("graph capture of the block-LU apply failed: ") + hipGetErrorString(ec));
+            return -1;
+        }
+        c->ag = g;
+        V2D_HIP(hipGraphInstantiate(&c->age, c->ag, nullptr, nullptr, 0));
+    }
+    V2D_HIP(hipMemcpyAsync(c->ga_in, v, sizeof(float) * n, hipMemcpyDeviceToDevice, c->s));
+    V2D_HIP(hipGraphLaunch(c->age, c->s));
+    V2D_HIP(hipMemcpyAsync(z, c->ga_out, sizeof(float) * n, hipMemcpyDeviceToDevice, c->s));
+    return 0;
+}
+
 // ------------------------------------------------------------- GMRES ------
 // Right-preconditioned GMRES with the block LU M, classical Gram-Schmidt with
 // one re-orthogonalisation (fused multi-dot / multi-axpy), Givens rotations on
@@ -213,8 +259,7 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
 {
     const int64_t n = 2 * c->N;
     const int m = cap;
-    const float *val = static_cast<const float *>(c->val);        // operator
-    const float *vpc = static_cast<const float *>(c->val_pc);     // preconditioner
+    const float *val = static_cast<const float *>(c->val);        // operator (the preconditioner: apply_pc)
     const unsigned nb = std::min<unsigned>(1024, grid_for(n));
     std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1);
     // v0 = b (unit norm already) -> gV[0]
@@ -225,7 +270,7 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
     double hv = 1.0;   // host source of the async normalisation-scalar copies
     for (; j < m; ++j) {
         float *vj = c->gV + j * n, *zj = c->gZ + j * n, *w = c->gV + (j + 1) * n;
-        if (direct_solve<float>(c->pat, c->wf, vpc, vj, zj, c->s)) return -1;
+        if (apply_pc(c, vj, zj)) return -1;
         hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
                            c->jmax, c->flags, val, zj, w);
         V2D_LAUNCH();
@@ -618,7 +663,8 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);
         c->gZ = c->alloc<float>((int64_t)c->gm_cap * 2 * N, rc);
         c->gb = c->alloc<float>(2 * N, rc);
-        c->gx = c->alloc<float>(2 * N, rc);
+        c->ga_in = c->alloc<float>(2 * N, rc);
+        c->ga_out = c->alloc<float>(2 * N, rc);
         c->gxd = c->alloc<double>(2 * N, rc);
     }
     // pivot-ready flags (2 per precision) and the error word of the dataflow steps
@@ -760,6 +806,14 @@ int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind)
     }
     ctx->c->wd.sweep_kind = kind;
     ctx->c->wf.sweep_kind = kind;
+    ctx->c->drop_graph();
+    return 0;
+}
+
+int v2d_ctx_set_graph(v2d_ctx *ctx, int enable)
+{
+    ctx->c->use_graph = enable != 0;
+    if (!enable) ctx->c->drop_graph();
     return 0;
 }
 

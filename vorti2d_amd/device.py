@@ -93,6 +93,12 @@ class DeviceContext:
         default) or "persistent" (one launch per solve)."""
         check(lib.v2d_ctx_set_sweep_kernel(self._h, SWEEP_KERNELS.get(kind, kind)))
 
+    def set_graph(self, enable: bool):
+        """Mixed precision: replay the block-LU apply of each GMRES iteration
+        as one captured HIP graph (the default) or launch its kernels one by
+        one."""
+        check(lib.v2d_ctx_set_graph(self._h, 1 if enable else 0))
+
     def autotune_update_kernel(self, re: float, kinds=("mfma64", "valu64")) -> dict:
         """Time one block-LU factorisation of the current state's Jacobian with
         each update kernel and keep the fastest; returns {kind: seconds}."""
