@@ -205,7 +205,8 @@ def run_precision(v, xg, yg, prec, a):
     if prec == "mixed":
         ctx.set_solver(refactor_every=a.refactor_every, gmres_rtol=a.gmres_rtol,
                        gmres_max_iter=a.gmres_max)
-        ctx.set_forcing(a.forcing)
+        # residual target: the final-state acceptance ||R||_2 / (2N) < TOL
+        ctx.set_forcing(a.forcing, TOL * 2 * xg.size if a.forcing_target else 0.0)
     ctx.init_state()
     tune = ctx.autotune_update_kernel(a.re)
     setup = time.perf_counter() - t0
@@ -259,6 +260,8 @@ def main():
     ap.add_argument("--gmres-rtol", type=float, default=1e-6)
     ap.add_argument("--forcing", type=float, default=0.1,
                     help="mixed: Eisenstat-Walker GMRES tolerance capped at this value (0: off)")
+    ap.add_argument("--no-forcing-target", dest="forcing_target", action="store_false",
+                    help="mixed: no residual-target safeguard on the forcing term")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the sharded design "
@@ -313,6 +316,8 @@ def main():
                    "linear_solver": (f"FP32 block-LU preconditioned FP32 GMRES (rtol "
                                      f"{a.gmres_rtol:g}"
                                      + (f", forcing <= {a.forcing:g}" if a.forcing > 0 else "")
+                                     + (", forcing >= 0.5 (1e-10 * 2N) / ||R_k|| (residual target)"
+                                        if a.forcing > 0 and a.forcing_target else "")
                                      + ", refactor "
                                      + ("on GMRES miss" if a.refactor_every == 0 else
                                         f"every {a.refactor_every}") + ")")},

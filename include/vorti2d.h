@@ -115,10 +115,13 @@ int v2d_ctx_init_state(v2d_ctx *ctx);
 int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int gmres_max_iter);
 /* Inexact-Newton forcing for V2D_MIXED (no counterpart in the reference, whose
  * Krylov solves use a fixed ksp_rtol): the GMRES tolerance of Newton step k is
- * max(gmres_rtol, min(eta_max, 0.9 (||R_k|| / ||R_k-1||)^2)) with the
- * Eisenstat-Walker safeguard; eta_max <= 0 restores the fixed tolerance.  The
- * history restarts with every new state. */
-int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max);
+ * max(gmres_rtol, eta_k) with eta_k = min(eta_max, 0.9 (||R_k|| /
+ * ||R_k-1||)^2) and the Eisenstat-Walker safeguard; with r_target > 0 (an
+ * absolute ||R||_2 the caller is content with), eta_k >= 0.5 r_target /
+ * ||R_k|| as well (up to eta_max), so that steps close to the target do not
+ * solve more accurately than the target needs.  eta_max <= 0 restores the
+ * fixed tolerance.  The history restarts with every new state. */
+int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max, double r_target);
 /* Kernel of the block-LU Gauss-Jordan update: VALU on 64- or 128-wide tiles,
  * or the matrix cores (MFMA 16x16x4, FP32 or FP64) on 64-wide tiles (the
  * default).  A tuning knob with no counterpart in the reference: results do not

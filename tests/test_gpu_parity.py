@@ -250,6 +250,36 @@ def test_persistent_sweep_equals_level_sweeps(v, C, oracle_mod, prec, imax, jmax
     assert rel <= (1e-10 if prec == "fp64" else 5e-2)
 
 
+def test_forcing_residual_target(v, C, oracle_mod):
+    """With a residual target (the final-state acceptance ||R||_2 / (2N) <
+    1e-10) on the forcing term, the mixed solve still meets both stopping
+    rules (res = ||delta||_2 / (2N) <= 1e-10 and the final residual) and
+    spends no more GMRES iterations than without it (on the 1025^2 benchmark
+    with a stored factorisation it saves about 25).  Refactoring every step
+    keeps the iteration counts independent of measured timings."""
+    imax = jmax = 181
+    xg, yg = oracle_mod.generate_annulus(imax, jmax)
+    n2 = 2 * imax * jmax
+    tot = {}
+    for tgt in (0.0, 1e-10 * n2):
+        ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, "mixed")
+        ctx.set_solver(refactor_every=1, gmres_rtol=1e-6, gmres_max_iter=30)
+        ctx.set_forcing(0.1, tgt)
+        ctx.init_state()
+        res, its, k = 1.0, [], 0
+        while res > 1e-10 and k < 20:
+            res, rn, li = ctx.newton_step(40.0)
+            its.append(li)
+            k += 1
+        ctx.assemble(40.0)
+        rfin = ctx.rhs_norm()
+        say(f"  target {tgt:.1e}: {k} Newton steps, GMRES {its}, res {res:.2e}, ||R||/2N {rfin / n2:.2e}")
+        assert res <= 1e-10 and rfin / n2 < 1e-10
+        tot[tgt] = sum(its)
+        ctx.close()
+    assert tot[1e-10 * n2] <= tot[0.0]
+
+
 def test_graph_replayed_gmres_equals_eager(v, C, oracle_mod):
     """The mixed Newton step with the block-LU apply replayed from a captured
     HIP graph equals the step with the apply's kernels launched one by one, bit

@@ -44,7 +44,7 @@ _SIGS = {
     "v2d_ctx_get_state": ([vp, vp, vp, vp, vp], i32),
     "v2d_ctx_init_state": ([vp], i32),
     "v2d_ctx_set_solver": ([vp, i32, f64, i32], i32),
-    "v2d_ctx_set_forcing": ([vp, f64], i32),
+    "v2d_ctx_set_forcing": ([vp, f64, f64], i32),
     "v2d_ctx_set_update_kernel": ([vp, i32], i32),
     "v2d_ctx_set_sweep_kernel": ([vp, i32], i32),
     "v2d_ctx_set_graph": ([vp, i32], i32),

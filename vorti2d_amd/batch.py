@@ -92,7 +92,7 @@ def solve_case(case: Case, precision="mixed", tol=1e-10, max_iter=50, forcing=0.
     ctx = DeviceContext(xg, yg, 0, math.cos(a), math.sin(a), precision)
     if precision == "mixed":
         ctx.set_solver(refactor_every=0, gmres_rtol=1e-6, gmres_max_iter=30)
-        ctx.set_forcing(forcing)
+        ctx.set_forcing(forcing, tol * 2 * case.imax * case.jmax)
     ctx.init_state()
     t1 = time.perf_counter()
     hist = []

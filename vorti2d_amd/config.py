@@ -74,6 +74,9 @@ class Config:
     gmres_forcing: float = 0.0
     """mixed only: > 0 enables the Eisenstat-Walker GMRES tolerance per Newton
     step, capped at this value (gmres_rtol stays the floor)."""
+    gmres_forcing_target: float = 0.0
+    """mixed only, with gmres_forcing: > 0 is an absolute ||R||_2 the solve is
+    content with; steps close to it solve no more accurately than it needs."""
 
     def inv_dtau(self) -> float:
         return 0.0 if math.isinf(self.dtau) else 1.0 / self.dtau

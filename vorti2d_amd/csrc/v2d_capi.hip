@@ -137,6 +137,7 @@ struct Ctx {
     int fresh_its = 0;
     // inexact Newton forcing (Eisenstat-Walker choice 2); eta_max <= 0: off
     double eta_max = 0.0, eta_prev = 0.0, rn_prev = 0.0;
+    double r_target = 0.0;     // residual norm the solve aims at (forcing safeguard; 0: off)
     double rtol_step = 1e-6;   // GMRES tolerance of the current Newton step
     double last_re = 0, last_invdtau = 0, last_inv2dt = 0, last_urot = 0;
     // timers: ev[0..3] bracket a Newton step (start, assembled, preconditioner
@@ -771,14 +772,19 @@ int v2d_ctx_set_solver(v2d_ctx *ctx, int refactor_every, double gmres_rtol, int 
     return 0;
 }
 
-int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max)
+int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max, double r_target)
 {
     Ctx *c = ctx->c.get();
     if (!(eta_max < 1.0)) {
         set_error("forcing eta_max must be < 1 (<= 0 disables it)");
         return -1;
     }
+    if (!(r_target >= 0.0)) {
+        set_error("forcing r_target must be >= 0 (0 disables it)");
+        return -1;
+    }
     c->eta_max = eta_max > 0.0 ? eta_max : 0.0;
+    c->r_target = r_target;
     c->rn_prev = c->eta_prev = 0.0;
     return 0;
 }
@@ -1081,6 +1087,10 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
                     if (sg > 0.1) eta = std::max(eta, sg);
                 }
                 eta = std::min(eta, c->eta_max);
+                // no more accuracy than the residual target needs: eta_k >=
+                // 0.5 r_target / ||R_k|| (Kelley, Iterative Methods for Linear
+                // and Nonlinear Equations, 1995, section 6.3.2)
+                if (c->r_target > 0.0) eta = std::min(c->eta_max, std::max(eta, 0.5 * c->r_target / bn));
                 c->rtol_step = std::max(eta, c->gm_rtol);
                 c->eta_prev = eta;
             }

@@ -78,10 +78,12 @@ class DeviceContext:
         check(lib.v2d_ctx_set_solver(self._h, int(refactor_every), float(gmres_rtol),
                                      int(gmres_max_iter)))
 
-    def set_forcing(self, eta_max: float):
+    def set_forcing(self, eta_max: float, r_target: float = 0.0):
         """Eisenstat-Walker inexact-Newton tolerance for the mixed GMRES
-        (eta_max <= 0: fixed gmres_rtol)."""
-        check(lib.v2d_ctx_set_forcing(self._h, float(eta_max)))
+        (eta_max <= 0: fixed gmres_rtol); r_target > 0: an absolute ||R||_2
+        the solve is content with (no step solves more accurately than it
+        needs)."""
+        check(lib.v2d_ctx_set_forcing(self._h, float(eta_max), float(r_target)))
 
     def set_update_kernel(self, kind: str | int):
         """Gauss-Jordan update kernel of the block LU: "valu64", "valu128" or

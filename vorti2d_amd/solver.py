@@ -67,7 +67,7 @@ class Solver:
                                  cfg.precision_code())
         if cfg.precision_code() == 1:
             self.ctx.set_solver(cfg.refactor_every, cfg.gmres_rtol, cfg.gmres_max_iter)
-            self.ctx.set_forcing(cfg.gmres_forcing)
+            self.ctx.set_forcing(cfg.gmres_forcing, cfg.gmres_forcing_target)
         for name, arr in zip(_METRIC_NAMES, self.ctx.metrics()):
             setattr(self, name, arr)
         self._init_state()
