@@ -262,7 +262,7 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
     const int m = cap;
     const float *val = static_cast<const float *>(c->val);        // operator (the preconditioner: apply_pc)
     const unsigned nb = std::min<unsigned>(1024, grid_for(n));
-    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1);
+    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1), hist(m, 1.0);
     // v0 = b (unit norm already) -> gV[0]
     V2D_HIP(hipMemcpyAsync(c->gV, c->gb, sizeof(float) * n, hipMemcpyDeviceToDevice, c->s));
     g[0] = 1.0;
@@ -322,6 +322,7 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
         g[j + 1] = -sn[j] * g[j];
         g[j] = cs[j] * g[j];
         est = std::fabs(g[j + 1]);
+        hist[j] = est;
         if (!std::isfinite(est) || !(d > 0.0)) {
             *iters = j + 1;
             *relres = INFINITY;
@@ -329,10 +330,19 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
         }
         if (est <= c->rtol_step || hn == 0.0) { ++j; break; }
         if (budget > 0 && j >= 2) {
-            // geometric projection of the iterations this preconditioner needs
+            // geometric projection of the iterations this preconditioner needs,
+            // from the average rate so far, and of the iterations still needed
+            // from the rate of the last two (a stale factorisation often gives
+            // one large first reduction and then stalls, which only the recent
+            // rate shows)
             const double rate = std::pow(est, 1.0 / (j + 1));
             const double need = rate < 1.0 ? std::log(c->rtol_step) / std::log(rate) : INFINITY;
-            if (need > budget) {
+            double rest = 0.0;
+            if (j >= 3) {
+                const double rr = std::sqrt(est / hist[j - 2]);
+                rest = rr < 1.0 ? std::log(c->rtol_step / est) / std::log(rr) : INFINITY;
+            }
+            if (need > budget || rest > budget) {
                 *iters = j + 1;
                 *relres = est;
                 return 2;
