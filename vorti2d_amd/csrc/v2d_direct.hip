@@ -733,10 +733,10 @@ template <> struct Mfma<double> {
 // block updates its 64 x 64 tile with the step-k0 panels (read buffers), and
 // the blocks that touch the next pivot block k1 = k0 + 64 also prepare step k1
 // (write buffers), so no row-panel launch sits between two steps:
-//   * tile (k1, k1), dispatched first: inverts its updated block in registers
-//     (gj_regs), stores the inverse as the new pivot block of A, into dinv and
-//     into the next transposed row panel, then publishes flags[job] = epoch
-//     (release, agent scope);
+//   * tile (k1, k1), dispatched first: inverts its updated block (gj_tile),
+//     stores the inverse into dinv, publishes flags[job] = epoch (release,
+//     agent scope), then stores it as the new pivot block of A and into the
+//     next transposed row panel;
 //   * tiles (k1, c), c != k1: stage their updated tile in LDS, wait for the
 //     flag (acquire; a bounded spin that sets *err and carries on if the flag
 //     never comes, so every wave exits), and store D^-1 * tile (matrix cores,
@@ -909,6 +909,11 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
         const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
         T g[16];
         gj_tile(sT, LT, kb2, g, J.dinv);
+        // publish the inverse first: the pivot-row workgroups read only dinv;
+        // the pivot block of A and the next row panel are read by later launches
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(flags + z, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         T *rk = J.rowk_w + (int64_t)(k1 + c) * NB + rg * 16;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
@@ -916,9 +921,6 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
             if (r < kb2 && c < kb2) A[(int64_t)(k1 + r) * ld + k1 + c] = g[q];
             rk[q] = g[q];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(flags + z, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
     // ------------------------------------------------ next pivot rows ----
@@ -1278,11 +1280,40 @@ struct SweepJob {
     int ld, glob;
     const T *base;
     T *out;
+    T *lout;         // level-ordered copy of the output (or null)
+    // right-hand side from the padded coupling lists (pterm != 0): bit 1 the
+    // couplings to level l-1, bit 2 to level l+1; source psrc, level-ordered
+    int pterm;
+    const T *psrc;
 };
 template <typename T>
 struct SweepBatch {
     SweepJob<T> j[2];
 };
+
+// The same right-hand side from the padded coupling lists: one load round
+// for the coefficients and source indices, one for the source entries (the
+// padding adds zeros, so the sums equal the list sums above).
+template <typename T>
+__global__ void __launch_bounds__(256) k_level_rhs_pad(SweepBatch<T> B, Levels L, int64_t N, const T *__restrict__ cfL,
+                                                       const T *__restrict__ cfU, const int *__restrict__ ixL,
+                                                       const int *__restrict__ ixU, double sgn_rhs)
+{
+    const SweepJob<T> &J = B.j[blockIdx.y];
+    const RhsJob<T> &R = J.rhs;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R.n) return;
+    const int n = R.n;
+    const int64_t cb = (int64_t)kCouple * lvl_voff(L, R.l);
+    double sum = 0.0;
+    if (J.pterm & 1)
+#pragma unroll
+        for (int e = 0; e < kCouple; ++e) sum += (double)cfL[cb + e * n + r] * (double)J.psrc[ixL[cb + e * n + r]];
+    if (J.pterm & 2)
+#pragma unroll
+        for (int e = 0; e < kCouple; ++e) sum += (double)cfU[cb + e * n + r] * (double)J.psrc[ixU[cb + e * n + r]];
+    J.y[r] = (T)((R.b ? (double)R.b[L.global(r, N, R.l)] : 0.0) + sgn_rhs * sum);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_level_rhs_b(SweepBatch<T> B, Levels L, int64_t N,
@@ -1356,17 +1387,22 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
             const T o = (T)(b + sgn_mv * acc);
             if (J.glob) J.out[L.global(r, N, J.rhs.l)] = o;
             else J.out[r] = o;
+            if (J.lout) J.lout[r] = o;
         }
     }
 }
 
 template <typename T>
 static int launch_sweep(const SweepBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
-                        double sgn_rhs, double sgn_mv, hipStream_t s)
+                        double sgn_rhs, double sgn_mv, hipStream_t s, const DirectWork<T> *pad = nullptr)
 {
     const unsigned rows_per_block = 4 * kSweepRowsPerWave;
-    hipLaunchKernelGGL(k_level_rhs_b<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N, val,
-                       sgn_rhs);
+    if (pad)
+        hipLaunchKernelGGL(k_level_rhs_pad<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N,
+                           (const T *)pad->cfL, (const T *)pad->cfU, P.d_ixL, P.d_ixU, sgn_rhs);
+    else
+        hipLaunchKernelGGL(k_level_rhs_b<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N, val,
+                           sgn_rhs);
     hipLaunchKernelGGL(k_level_gemv_b<T>, dim3((nmax + rows_per_block - 1) / rows_per_block, nj, 1),
                        dim3(256), sizeof(T) * P.ldmax, s, B, P.L, P.N, sgn_mv);
     V2D_LAUNCH();
@@ -1729,6 +1765,10 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
     auto zl = [&](int l) { return W.z + P.voff[l]; };
     auto S = [&](int l) { return (const T *)(W.sinv + P.soff[l]); };
     T *ybuf[2] = {W.colk, W.colk + (int64_t)P.nmax * NB};   // the panels are free during a solve
+    // right-hand sides from the padded coupling lists when the factorisation
+    // filled them (the backward steps then also keep a level-ordered copy of
+    // the solution, in wv, for their sources)
+    const DirectWork<T> *pad = (P.sweep_ok && W.cfL && W.wv) ? &W : nullptr;
     // forward sweeps
     for (int st = 0; st < steps; ++st) {
         SweepBatch<T> G{};
@@ -1747,10 +1787,11 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
             G.j[nj].y = ybuf[nj];
             G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 0;
             G.j[nj].base = nullptr; G.j[nj].out = zl(l);
+            G.j[nj].pterm = st > 0 ? (side == 0 ? 1 : 2) : 0; G.j[nj].psrc = W.z;
             nmax = std::max(nmax, P.n[l]);
             ++nj;
         }
-        if (launch_sweep(G, nj, nmax, P, val, -1.0, 1.0, s)) return -1;
+        if (launch_sweep(G, nj, nmax, P, val, -1.0, 1.0, s, pad)) return -1;
     }
     {   // middle level
         SweepBatch<T> G{};
@@ -1763,7 +1804,9 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
         G.j[0].y = ybuf[0];
         G.j[0].S = S(M); G.j[0].ld = P.ld[M]; G.j[0].glob = 1;
         G.j[0].base = nullptr; G.j[0].out = x;
-        if (launch_sweep(G, 1, P.n[M], P, val, -1.0, 1.0, s)) return -1;
+        G.j[0].pterm = (M > 0 ? 1 : 0) | (M < nlev - 1 ? 2 : 0); G.j[0].psrc = W.z;
+        G.j[0].lout = pad ? W.wv + P.voff[M] : nullptr;
+        if (launch_sweep(G, 1, P.n[M], P, val, -1.0, 1.0, s, pad)) return -1;
     }
     // backward sweeps, outward from the middle
     for (int st = 0; st < steps; ++st) {
@@ -1780,11 +1823,13 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
             G.j[nj].y = ybuf[nj];
             G.j[nj].S = S(l); G.j[nj].ld = P.ld[l]; G.j[nj].glob = 1;
             G.j[nj].base = zl(l); G.j[nj].out = x;
+            G.j[nj].pterm = side == 0 ? 2 : 1; G.j[nj].psrc = W.wv;
+            G.j[nj].lout = pad ? W.wv + P.voff[l] : nullptr;
             nmax = std::max(nmax, P.n[l]);
             ++nj;
         }
         if (!nj) break;
-        if (launch_sweep(G, nj, nmax, P, val, 1.0, -1.0, s)) return -1;
+        if (launch_sweep(G, nj, nmax, P, val, 1.0, -1.0, s, pad)) return -1;
     }
     return 0;
 }
