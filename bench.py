@@ -207,6 +207,7 @@ def run_precision(v, xg, yg, prec, a):
                        gmres_max_iter=a.gmres_max)
         # residual target: the final-state acceptance ||R||_2 / (2N) < TOL
         ctx.set_forcing(a.forcing, TOL * 2 * xg.size if a.forcing_target else 0.0)
+        ctx.set_graph(a.graph)
     ctx.init_state()
     tune = ctx.autotune_update_kernel(a.re)
     setup = time.perf_counter() - t0
@@ -262,6 +263,9 @@ def main():
                     help="mixed: Eisenstat-Walker GMRES tolerance capped at this value (0: off)")
     ap.add_argument("--no-forcing-target", dest="forcing_target", action="store_false",
                     help="mixed: no residual-target safeguard on the forcing term")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="mixed: launch the block-LU apply kernels one by one instead of replaying a "
+                         "captured graph (same kernels; for kernel-trace profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the sharded design "
