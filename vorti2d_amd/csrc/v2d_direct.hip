@@ -1355,14 +1355,37 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
     for (int q = 0; q < RW; ++q) {
         const int r = min(row0 + wave * RW + q, n - 1);
         const V *row = reinterpret_cast<const V *>(J.S + (int64_t)r * ld);
+        // unconditional loads of clamped columns (no branch between them),
+        // the entries past the row zeroed afterwards
 #pragma unroll
-        for (int u = 0; u < kSweepPre; ++u) {
-            const int c = lane + 64 * u;
-            if (c < nv) pre[q][u] = row[c];
-            else pre[q][u] = V{};
+        for (int u = 0; u < kSweepPre; ++u) pre[q][u] = row[min(lane + 64 * u, nv - 1)];
+#pragma unroll
+        for (int u = 0; u < kSweepPre; ++u)
+            if (lane + 64 * u >= nv) pre[q][u] = V{};
+    }
+    // y into LDS as 16-byte vectors, every load issued before the first
+    // store (a loop of load-store pairs would wait for each load in turn);
+    // entries past n are zeroed
+    {
+        constexpr int YP = 3;
+        const V *yv = reinterpret_cast<const V *>(J.y);
+        V yr[YP];
+#pragma unroll
+        for (int p = 0; p < YP; ++p) yr[p] = yv[min((int)threadIdx.x + 256 * p, nv - 1)];
+#pragma unroll
+        for (int p = 0; p < YP; ++p) {
+            const int c = threadIdx.x + 256 * p;
+            if (c < nv)
+#pragma unroll
+                for (int e = 0; e < W; ++e)
+                    vs[c * W + e] = c * W + e < n ? reinterpret_cast<const T *>(&yr[p])[e] : T(0);
+        }
+        for (int c = threadIdx.x + 256 * YP; c < nv; c += 256) {
+            const V x = yv[c];
+#pragma unroll
+            for (int e = 0; e < W; ++e) vs[c * W + e] = c * W + e < n ? reinterpret_cast<const T *>(&x)[e] : T(0);
         }
     }
-    for (int r = threadIdx.x; r < ld; r += blockDim.x) vs[r] = r < n ? J.y[r] : T(0);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
