@@ -655,6 +655,7 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->wd.wv = c->alloc<double>(2 * N, rc);
         c->wd.cfL = c->alloc<double>(kCouple * 2 * N, rc);
         c->wd.cfU = c->alloc<double>(kCouple * 2 * N, rc);
+        c->wd.cfC = c->alloc<double>(std::max<int64_t>(1, c->pat.ncx), rc);
         c->wd.yv = c->alloc<double>(2 * N, rc);
         c->wd.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
     } else {
@@ -669,6 +670,7 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
         c->wf.wv = c->alloc<float>(2 * N, rc);
         c->wf.cfL = c->alloc<float>(kCouple * 2 * N, rc);
         c->wf.cfU = c->alloc<float>(kCouple * 2 * N, rc);
+        c->wf.cfC = c->alloc<float>(std::max<int64_t>(1, c->pat.ncx), rc);
         c->wf.yv = c->alloc<float>(2 * N, rc);
         c->wf.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
         c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);

@@ -159,6 +159,85 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
         up(&P.d_Uptr, &P.d_Ur, cU, eU) || up(&P.d_UCptr, &P.d_Uc, cUC, eUC) ||
         up(&P.d_LCptr, &P.d_Lc, cLC, eLC))
         return -1;
+    auto upi = [s](int **d, const std::vector<int> &h) -> int {
+        V2D_HIP(hipMallocAsync(d, sizeof(int) * std::max<size_t>(h.size(), 1), s));
+        if (!h.empty())
+            V2D_HIP(hipMemcpyAsync(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s));
+        return 0;
+    };
+    // Padded coupling rows: every level row's couplings into level l-1 (L) and
+    // l+1 (U), padded to kCouple entries, stored [kCouple][n_l] at
+    // kCouple * voff[l] (level-ordered source index, value index or -1).
+    const int64_t tot = (int64_t)kCouple * n2;
+    std::vector<int> ixL(tot, 0), ixU(tot, 0), vxL(tot, -1), vxU(tot, -1);
+    auto pad = [&](const std::vector<int> &ptr, const std::vector<int2> &ent, int l, int lc,
+                   std::vector<int> &ix, std::vector<int> &vx) -> bool {
+        const int nl = P.n[l];
+        const int64_t base = (int64_t)kCouple * P.voff[l];
+        for (int r = 0; r < nl; ++r) {
+            const int64_t R = P.voff[l] + r;
+            const int e0 = ptr[R], e1 = ptr[R + 1];
+            if (e1 - e0 > kCouple) return false;
+            if (e1 > e0 && (lc < 0 || lc >= nlev)) return false;
+            for (int t = 0; t < kCouple; ++t) {
+                const int64_t q = base + (int64_t)t * nl + r;
+                if (e0 + t < e1) {
+                    ix[q] = (int)(P.voff[lc] + ent[e0 + t].x);
+                    vx[q] = ent[e0 + t].y;
+                } else {
+                    ix[q] = (lc >= 0 && lc < nlev) ? (int)P.voff[lc] : 0;   // any valid source
+                }
+            }
+        }
+        return true;
+    };
+    for (int l = 0; l < nlev; ++l)
+        if (!pad(cL, eL, l, l - 1, ixL, vxL) || !pad(cU, eU, l, l + 1, ixU, vxU)) {
+            set_error("direct solver: a row couples to more than " + std::to_string(kCouple) +
+                      " unknowns of a neighbouring level");
+            return -1;
+        }
+    // Padded coupling columns: column q of level l in U_{l-1} (side 0) and in
+    // L_{l+1} (side 1), padded to cx_w in {4, 8} entries, stored [cx_w][n_l] at
+    // cx_off (row in the neighbouring level, value index or -1).
+    std::vector<int> jxC, vxC;
+    P.ncx = 0;
+    for (int side = 0; side < 2; ++side) {
+        P.cx_off[side].assign(nlev, 0);
+        P.cx_w[side].assign(nlev, 4);
+    }
+    for (int side = 0; side < 2; ++side)
+        for (int l = 0; l < nlev; ++l) {
+            const std::vector<int> &ptr = side ? cLC : cUC;
+            int w = 0;
+            for (int q = 0; q < P.n[l]; ++q) w = std::max(w, ptr[P.voff[l] + q + 1] - ptr[P.voff[l] + q]);
+            if (w > 8) {
+                set_error("direct solver: a column couples to more than 8 unknowns of a neighbouring level");
+                return -1;
+            }
+            P.cx_w[side][l] = w <= 4 ? 4 : 8;
+            P.cx_off[side][l] = P.ncx;
+            P.ncx += (int64_t)P.cx_w[side][l] * P.n[l];
+        }
+    jxC.assign(P.ncx, 0);
+    vxC.assign(P.ncx, -1);
+    for (int side = 0; side < 2; ++side)
+        for (int l = 0; l < nlev; ++l) {
+            const std::vector<int> &ptr = side ? cLC : cUC;
+            const std::vector<int2> &ent = side ? eLC : eUC;
+            const int nl = P.n[l];
+            for (int q = 0; q < nl; ++q) {
+                const int e0 = ptr[P.voff[l] + q], e1 = ptr[P.voff[l] + q + 1];
+                for (int t = 0; t < e1 - e0; ++t) {
+                    jxC[P.cx_off[side][l] + (int64_t)t * nl + q] = ent[e0 + t].x;
+                    vxC[P.cx_off[side][l] + (int64_t)t * nl + q] = ent[e0 + t].y;
+                }
+            }
+        }
+    if (upi(&P.d_ixL, ixL) || upi(&P.d_ixU, ixU) || upi(&P.d_vxL, vxL) || upi(&P.d_vxU, vxU) ||
+        upi(&P.d_jxC, jxC) || upi(&P.d_vxC, vxC))
+        return -1;
+    P.pad_ok = true;
     // tables of the persistent solve sweeps
     P.sweep_ok = false;
     bool closed = nlev >= 2;
@@ -167,37 +246,7 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
     const int M = (nlev - 1) / 2;
     const int steps = std::max(M, nlev - 1 - M);
     P.nsteps = 2 * steps + 1;
-    std::vector<int> ixL, ixU, vxL, vxU;
     if (closed) {
-        const int64_t tot = (int64_t)kCouple * n2;
-        ixL.assign(tot, 0); ixU.assign(tot, 0); vxL.assign(tot, -1); vxU.assign(tot, -1);
-        auto pad = [&](const std::vector<int> &ptr, const std::vector<int2> &ent, int l, int lc,
-                       std::vector<int> &ix, std::vector<int> &vx) -> bool {
-            const int nl = P.n[l];
-            const int64_t base = (int64_t)kCouple * P.voff[l];
-            for (int r = 0; r < nl; ++r) {
-                const int64_t R = P.voff[l] + r;
-                const int e0 = ptr[R], e1 = ptr[R + 1];
-                if (e1 - e0 > kCouple) return false;
-                if (e1 > e0 && (lc < 0 || lc >= nlev)) return false;
-                for (int t = 0; t < kCouple; ++t) {
-                    const int64_t q = base + (int64_t)t * nl + r;
-                    if (e0 + t < e1) {
-                        ix[q] = (int)(P.voff[lc] + ent[e0 + t].x);
-                        vx[q] = ent[e0 + t].y;
-                    } else {
-                        ix[q] = (lc >= 0 && lc < nlev) ? (int)P.voff[lc] : 0;   // any valid source
-                    }
-                }
-            }
-            return true;
-        };
-        for (int l = 0; l < nlev; ++l)
-            if (!pad(cL, eL, l, l - 1, ixL, vxL) || !pad(cU, eU, l, l + 1, ixU, vxU)) {
-                set_error("direct solver: a row couples to more than " + std::to_string(kCouple) +
-                          " unknowns of a neighbouring level");
-                return -1;
-            }
         // C workgroups per chain, one per compute unit: at most 16 rows of an
         // interior level each, at most 128 rows of any level (8 lanes per row
         // build the right-hand side)
@@ -208,13 +257,6 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
         P.sweep_rows = 1;
         for (int l = 0; l < nlev; ++l) P.sweep_rows = std::max(P.sweep_rows, (P.n[l] + P.sweep_C - 1) / P.sweep_C);
         P.sweep_grid = 2 * P.sweep_C;
-        auto upi = [s](int **d, const std::vector<int> &h) -> int {
-            V2D_HIP(hipMallocAsync(d, sizeof(int) * h.size(), s));
-            V2D_HIP(hipMemcpyAsync(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s));
-            return 0;
-        };
-        if (upi(&P.d_ixL, ixL) || upi(&P.d_ixU, ixU) || upi(&P.d_vxL, vxL) || upi(&P.d_vxU, vxU))
-            return -1;
         P.sweep_ok = P.sweep_rows <= kSwThreads / 8 && ncu >= 2;
     }
     V2D_HIP(hipStreamSynchronize(s));   // the host lists above are locals
@@ -224,14 +266,14 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
 void direct_free_pattern(DirectPattern &P, hipStream_t s)
 {
     int *ptrs[] = {P.d_Dptr, P.d_Lptr, P.d_Uptr, P.d_UCptr, P.d_LCptr,
-                   P.d_ixL, P.d_ixU, P.d_vxL, P.d_vxU};
+                   P.d_ixL, P.d_ixU, P.d_vxL, P.d_vxU, P.d_jxC, P.d_vxC};
     int2 *ents[] = {P.d_D, P.d_Lr, P.d_Ur, P.d_Uc, P.d_Lc};
     for (int *p : ptrs) if (p) (void)hipFreeAsync(p, s);
     for (int2 *p : ents) if (p) (void)hipFreeAsync(p, s);
     P.d_Dptr = P.d_Lptr = P.d_Uptr = P.d_UCptr = P.d_LCptr = nullptr;
-    P.d_ixL = P.d_ixU = P.d_vxL = P.d_vxU = nullptr;
+    P.d_ixL = P.d_ixU = P.d_vxL = P.d_vxU = P.d_jxC = P.d_vxC = nullptr;
     P.d_D = P.d_Lr = P.d_Ur = P.d_Uc = P.d_Lc = nullptr;
-    P.sweep_ok = false;
+    P.sweep_ok = P.pad_ok = false;
 }
 
 
@@ -248,88 +290,180 @@ struct Job {
     T *colk_w, *rowk_w;    // panels of the next step (MFMA dataflow step only)
 };
 
-// One Schur-update source: the neighbouring level's inverse on the side the
-// chain comes from, the column lists of the coupling into this level
-// (entries: row in the neighbour, value index) and the row lists of this level
-// into the neighbour (entries: column in the neighbour, value index).
-template <typename T>
-struct Src {
-    const T *Sprev;
-    int np, ldp;
-    const int *cptr;
-    const int2 *cent;
-    const int *rptr;
-    const int2 *rent;
-};
-
 template <typename T>
 struct Batch {
     Job<T> j[2];
-    Src<T> src[2];
     int nj;
     int kind;   // Gauss-Jordan update kernel (UpdateKind)
 };
 
 // ------------------------------------------------------- level build ------
+// One Schur source of a level:  Tm = S_p^-1 C  (np x n, row stride ld), where
+// column q of the coupling C holds at most kc entries (padded columns: row of
+// the neighbouring level p, coefficient).
 template <typename T>
-__global__ void __launch_bounds__(256) k_fill_b(Batch<T> B)
+struct SchurSrc {
+    const T *Sprev;        // S_p^-1 (np x ldp)
+    int np, ldp;
+    const int *jx;         // [kc][n] rows of level p
+    const T *cf;           // [kc][n] coefficients (0 on padding)
+    int kc;
+    T *tm;
+    int n, ld;
+};
+
+// A level's dense block before its inversion:
+//   S = D - sum_k R_k Tm_k      (n x ld, zero beyond column n)
+// where row r of the coupling R_k holds kCouple padded entries (level-ordered
+// column, coefficient) and D is the level's own CSR row.
+template <typename T>
+struct LevelBuild {
+    T *S;
+    int n, ld;
+    const int *dptr;       // CSR rows of D (this level)
+    int nsrc;
+    const T *tm[2];
+    const int *ix[2];      // [kCouple][n] level-ordered columns
+    const T *cf[2];        // [kCouple][n] coefficients
+    int ixoff[2];          // level-ordered offset of the source level
+};
+
+template <typename T>
+struct BuildBatch {
+    SchurSrc<T> src[2];
+    LevelBuild<T> lv[2];
+};
+
+constexpr int kTmRows = 4;   // rows of Tm per wavefront (Tm kernel)
+constexpr int kSbRows = 4;   // rows of S per wavefront (S kernel)
+
+// Tm[c][q] = sum_t S_p^-1[c][jx_t(q)] cf_t(q): a workgroup's four wavefronts
+// each take 64 columns of kTmRows rows; the padded column lists are read once
+// per lane and every row load of S_p^-1 is issued before the first product.
+template <typename T, int KC>
+__global__ void __launch_bounds__(256) k_schur_tm(BuildBatch<T> B)
 {
-    const Job<T> &J = B.j[blockIdx.z];
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < (int64_t)J.n * J.ld) J.S[t] = T(0);
+    const SchurSrc<T> &A = B.src[blockIdx.z];
+    const int lane = threadIdx.x & 63;
+    const int c0 = (blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kTmRows;
+    const int q = blockIdx.x * 64 + lane;
+    if (c0 >= A.np || q >= A.n) return;
+    int p[KC];
+    double u[KC];
+#pragma unroll
+    for (int t = 0; t < KC; ++t) {   // clamped loads, no branches: entries past kc read as 0
+        const int64_t o = (int64_t)min(t, A.kc - 1) * A.n + q;
+        p[t] = A.jx[o];
+        const T c = A.cf[o];
+        u[t] = t < A.kc ? (double)c : 0.0;
+    }
+    T v[kTmRows][KC];
+#pragma unroll
+    for (int r = 0; r < kTmRows; ++r) {
+        const T *row = A.Sprev + (int64_t)min(c0 + r, A.np - 1) * A.ldp;
+#pragma unroll
+        for (int t = 0; t < KC; ++t) v[r][t] = row[p[t]];
+    }
+#pragma unroll
+    for (int r = 0; r < kTmRows; ++r) {
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < KC; ++t) acc += (double)v[r][t] * u[t];
+        if (c0 + r < A.np) A.tm[(int64_t)(c0 + r) * A.ld + q] = (T)acc;
+    }
 }
 
-// S[r][c] = D entries of the level rows
+__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rdlane(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rdlane(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// S[r][q] = D[r][q] - sum_k sum_t cf_k,t(r) Tm_k[ix_k,t(r)][q] for q < n and
+// 0 for n <= q < ld: every entry of the block written once.  Rows are
+// wavefront-uniform: one vector load brings the kSbRows x kCouple row-list
+// entries of a source (lane 4 t + rr), another the D entries of the rows (at
+// most 64 in all, else one row at a time), and readlane hands them out.  All
+// Tm loads of the kSbRows rows are issued before the first sum.
+template <typename T, int NS>
+__device__ __forceinline__ void build_rows(const LevelBuild<T> &J, int r0, int q, const int2 *__restrict__ dent,
+                                           const T *__restrict__ val)
+{
+    static_assert(kSbRows * kCouple <= 64, "row lists of a wavefront exceed its lanes");
+    const int lane = threadIdx.x & 63;
+    const bool in = q < J.n;
+    const int qc = min(q, J.n - 1);
+    int ixl[NS > 0 ? NS : 1];
+    T cfl[NS > 0 ? NS : 1];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int t = (lane >> 2) & (kCouple - 1), rr = lane & 3;
+        const int64_t o = (int64_t)t * J.n + min(r0 + rr, J.n - 1);
+        ixl[k] = J.ix[k][o] - J.ixoff[k];
+        cfl[k] = J.cf[k][o];
+    }
+    double tv[kSbRows][NS > 0 ? NS : 1][kCouple];
+#pragma unroll
+    for (int rr = 0; rr < kSbRows; ++rr)
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+#pragma unroll
+            for (int t = 0; t < kCouple; ++t)
+                tv[rr][k][t] = (double)J.tm[k][(int64_t)rdlane(ixl[k], 4 * t + rr) * J.ld + qc];
+    int rb[kSbRows + 1];
+#pragma unroll
+    for (int rr = 0; rr <= kSbRows; ++rr) rb[rr] = J.dptr[min(r0 + rr, J.n)];
+    double acc[kSbRows];
+#pragma unroll
+    for (int rr = 0; rr < kSbRows; ++rr) acc[rr] = 0.0;
+    if (rb[kSbRows] - rb[0] <= 64) {
+        int dc = -1;
+        T dv = T(0);
+        if (rb[0] + lane < rb[kSbRows]) {
+            const int2 x = dent[rb[0] + lane];
+            dc = x.x;
+            dv = val[x.y];
+        }
+#pragma unroll
+        for (int rr = 0; rr < kSbRows; ++rr)
+            for (int i = rb[rr] - rb[0]; i < rb[rr + 1] - rb[0]; ++i)
+                if (rdlane(dc, i) == q) acc[rr] = (double)rdlane(dv, i);
+    } else {
+#pragma unroll
+        for (int rr = 0; rr < kSbRows; ++rr)
+            for (int e = rb[rr]; e < rb[rr + 1]; ++e) {
+                const int2 x = dent[e];
+                if (x.x == q) acc[rr] = (double)val[x.y];
+            }
+    }
+#pragma unroll
+    for (int rr = 0; rr < kSbRows; ++rr) {
+        if (r0 + rr >= J.n) break;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+#pragma unroll
+            for (int t = 0; t < kCouple; ++t) acc[rr] -= (double)rdlane(cfl[k], 4 * t + rr) * tv[rr][k][t];
+        J.S[(int64_t)(r0 + rr) * J.ld + q] = in ? (T)acc[rr] : T(0);
+    }
+}
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_scatter_D_b(Batch<T> B, const int *__restrict__ dptr,
-                                                     const int2 *__restrict__ dent,
+__global__ void __launch_bounds__(256) k_build_level(BuildBatch<T> B, const int2 *__restrict__ dent,
                                                      const T *__restrict__ val)
 {
-    const Job<T> &J = B.j[blockIdx.z];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= J.n) return;
-    const int *ptr = dptr + J.voff;
-    for (int e = ptr[r]; e < ptr[r + 1]; ++e) {
-        const int2 x = dent[e];
-        J.S[(int64_t)r * J.ld + x.x] = val[x.y];
-    }
-}
-
-// Tm[c][q] = sum_{(p,u) in col(q)} Sprev[c][p] * u      (np x n, stride ld)
-template <typename T>
-__global__ void __launch_bounds__(256) k_schur_T_b(Batch<T> B, const T *__restrict__ val)
-{
-    const Job<T> &J = B.j[blockIdx.z];
-    const Src<T> &S = B.src[blockIdx.z];
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (q >= J.n || c >= S.np) return;
-    double acc = 0.0;
-    const T *row = S.Sprev + (int64_t)c * S.ldp;
-    for (int e = S.cptr[q]; e < S.cptr[q + 1]; ++e) {
-        const int2 x = S.cent[e];
-        acc += (double)row[x.x] * (double)val[x.y];
-    }
-    J.tmat[(int64_t)c * J.ld + q] = (T)acc;
-}
-
-// S[r][q] -= sum_{(c,v) in row(r)} v * Tm[c][q]
-template <typename T>
-__global__ void __launch_bounds__(256) k_schur_S_b(Batch<T> B, const T *__restrict__ val)
-{
-    const Job<T> &J = B.j[blockIdx.z];
-    const Src<T> &S = B.src[blockIdx.z];
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
-    if (q >= J.n || r >= J.n) return;
-    const int e0 = S.rptr[r], e1 = S.rptr[r + 1];
-    if (e0 == e1) return;
-    double acc = (double)J.S[(int64_t)r * J.ld + q];
-    for (int e = e0; e < e1; ++e) {
-        const int2 x = S.rent[e];
-        acc -= (double)val[x.y] * (double)J.tmat[(int64_t)x.x * J.ld + q];
-    }
-    J.S[(int64_t)r * J.ld + q] = (T)acc;
+    const LevelBuild<T> J = B.lv[blockIdx.z];
+    const int r0 = (blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kSbRows;
+    const int q = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (r0 >= J.n || blockIdx.x * 64 >= J.ld) return;
+    if (J.nsrc == 2) build_rows<T, 2>(J, r0, q, dent, val);
+    else if (J.nsrc == 1) build_rows<T, 1>(J, r0, q, dent, val);
+    else build_rows<T, 0>(J, r0, q, dent, val);
 }
 
 // --------------------------------------------------- Gauss-Jordan ---------
@@ -1099,49 +1233,68 @@ static Job<T> make_job(const DirectPattern &P, DirectWork<T> &W, int l, int slot
     return J;
 }
 
-// Schur source of level l coming from level p = l-1 (bottom) or l+1 (top).
-template <typename T>
-static Src<T> make_src(const DirectPattern &P, const DirectWork<T> &W, int l, int p)
-{
-    Src<T> S;
-    S.Sprev = W.sinv + P.soff[p];
-    S.np = P.n[p];
-    S.ldp = P.ld[p];
-    if (p == l - 1) {   // coupling U_{l-1} (columns in l), rows L_l
-        S.cptr = P.d_UCptr + P.voff[l]; S.cent = P.d_Uc;
-        S.rptr = P.d_Lptr + P.voff[l];  S.rent = P.d_Lr;
-    } else {            // coupling L_{l+1} (columns in l), rows U_l
-        S.cptr = P.d_LCptr + P.voff[l]; S.cent = P.d_Lc;
-        S.rptr = P.d_Uptr + P.voff[l];  S.rent = P.d_Ur;
-    }
-    return S;
-}
-
+// Dense block of every job of B (Schur sources: level srcp[z][k] where
+// schur[z][k]), then its inverse.  A source p = l-1 couples through U_{l-1}
+// (columns of side 0) and the rows L_l; a source p = l+1 through L_{l+1}
+// (side 1) and the rows U_l.  At most two sources in all: the two chains take
+// one each, the middle level both.
 template <typename T>
 static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, bool schur[2][2],
                             int srcp[2][2], DirectWork<T> &W, hipStream_t s)
 {
-    int nmax = 0, npmax = 0;
-    for (int z = 0; z < B.nj; ++z) nmax = std::max(nmax, B.j[z].n);
-    const unsigned nz = (unsigned)B.nj;
-    hipLaunchKernelGGL(k_fill_b<T>, dim3(grid_for((int64_t)nmax * P.ldmax), 1, nz), dim3(256), 0, s, B);
-    hipLaunchKernelGGL(k_scatter_D_b<T>, dim3(grid_for(nmax), 1, nz), dim3(256), 0, s, B, P.d_Dptr,
-                       P.d_D, val);
-    for (int k = 0; k < 2; ++k) {   // up to two Schur sources (the middle level has two)
-        Batch<T> Bk = B;
-        int nj = 0;
-        for (int z = 0; z < B.nj; ++z)
-            if (schur[z][k]) {
-                Bk.j[nj] = B.j[z];
-                Bk.src[nj] = make_src(P, W, B.j[z].l, srcp[z][k]);
-                npmax = std::max(npmax, Bk.src[nj].np);
-                ++nj;
-            }
-        if (!nj) continue;
-        Bk.nj = nj;
-        hipLaunchKernelGGL(k_schur_T_b<T>, dim3(grid_for(nmax), npmax, nj), dim3(256), 0, s, Bk, val);
-        hipLaunchKernelGGL(k_schur_S_b<T>, dim3(grid_for(nmax), nmax, nj), dim3(256), 0, s, Bk, val);
+    if (!P.pad_ok || !W.cfL || !W.cfU || !W.cfC) {
+        set_error("block LU: padded coupling tables or their values are missing");
+        return -1;
     }
+    BuildBatch<T> bb{};
+    int nt = 0, nmax = 0, ldmax = 0, npmax = 0, kcmax = 4;
+    for (int z = 0; z < B.nj; ++z) {
+        const Job<T> &J = B.j[z];
+        LevelBuild<T> &lv = bb.lv[z];
+        lv.S = J.S;
+        lv.n = J.n;
+        lv.ld = J.ld;
+        lv.dptr = P.d_Dptr + J.voff;
+        lv.nsrc = 0;
+        nmax = std::max(nmax, J.n);
+        ldmax = std::max(ldmax, J.ld);
+        for (int k = 0; k < 2; ++k) {
+            if (!schur[z][k]) continue;
+            const int l = J.l, p = srcp[z][k], side = p == l - 1 ? 0 : 1;
+            if (nt == 2 || (p != l - 1 && p != l + 1)) {
+                set_error("block LU: invalid Schur source");
+                return -1;
+            }
+            SchurSrc<T> &A = bb.src[nt];
+            A.Sprev = W.sinv + P.soff[p];
+            A.np = P.n[p];
+            A.ldp = P.ld[p];
+            A.jx = P.d_jxC + P.cx_off[side][l];
+            A.cf = W.cfC + P.cx_off[side][l];
+            A.kc = P.cx_w[side][l];
+            A.tm = W.tmat + (int64_t)nt * P.nmax * P.ldmax;
+            A.n = J.n;
+            A.ld = J.ld;
+            lv.tm[lv.nsrc] = A.tm;
+            lv.ix[lv.nsrc] = (side == 0 ? P.d_ixL : P.d_ixU) + (int64_t)kCouple * J.voff;
+            lv.cf[lv.nsrc] = (side == 0 ? W.cfL : W.cfU) + (int64_t)kCouple * J.voff;
+            lv.ixoff[lv.nsrc] = (int)P.voff[p];
+            ++lv.nsrc;
+            npmax = std::max(npmax, A.np);
+            kcmax = std::max(kcmax, A.kc);
+            ++nt;
+        }
+    }
+    if (nt) {
+        const dim3 g((unsigned)((nmax + 63) / 64), (unsigned)((npmax + 4 * kTmRows - 1) / (4 * kTmRows)),
+                     (unsigned)nt);
+        if (kcmax <= 4) hipLaunchKernelGGL((k_schur_tm<T, 4>), g, dim3(256), 0, s, bb);
+        else hipLaunchKernelGGL((k_schur_tm<T, 8>), g, dim3(256), 0, s, bb);
+    }
+    hipLaunchKernelGGL(k_build_level<T>,
+                       dim3((unsigned)(ldmax / 64), (unsigned)((nmax + 4 * kSbRows - 1) / (4 * kSbRows)),
+                            (unsigned)B.nj),
+                       dim3(256), 0, s, bb, P.d_D, val);
     V2D_LAUNCH();
     return gj_invert_b(B, W, W.probe, s);
 }
@@ -1161,8 +1314,8 @@ int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStr
 {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, W.err, 0);
     V2D_LAUNCH();
-    if (direct_factor_levels(P, W, val, s)) return -1;
     if (direct_couplings(P, W, val, s)) return -1;
+    if (direct_factor_levels(P, W, val, s)) return -1;
     if (W.update_kind == kUpdateMfma64) {
         int e = 0;
         V2D_HIP(hipMemcpyAsync(&e, W.err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1674,16 +1827,15 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
     }
 }
 
+// Coefficients of the padded coupling lists: out[q] = val[vx[q]] (0 on padding).
 template <typename T>
-__global__ void __launch_bounds__(256) k_fill_couplings(int64_t n, const int *__restrict__ vxL,
-                                                        const int *__restrict__ vxU, const T *__restrict__ val,
-                                                        T *__restrict__ cfL, T *__restrict__ cfU)
+__global__ void __launch_bounds__(256) k_gather_couplings(int64_t n, const int *__restrict__ vx,
+                                                          const T *__restrict__ val, T *__restrict__ out)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const int a = vxL[q], b = vxU[q];
-    cfL[q] = a >= 0 ? val[a] : T(0);
-    cfU[q] = b >= 0 ? val[b] : T(0);
+    const int a = vx[q];
+    out[q] = a >= 0 ? val[a] : T(0);
 }
 
 __global__ void k_zero_ints(int *p, int n)
@@ -1694,10 +1846,17 @@ __global__ void k_zero_ints(int *p, int n)
 template <typename T>
 int direct_couplings(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s)
 {
-    if (!P.sweep_ok || !W.cfL) return 0;
+    if (!P.pad_ok || !W.cfL || !W.cfU || !W.cfC) {
+        set_error("block LU: padded coupling tables or their values are missing");
+        return -1;
+    }
     const int64_t n = (int64_t)kCouple * 2 * P.N;
-    hipLaunchKernelGGL(k_fill_couplings<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, P.d_vxL,
-                       P.d_vxU, val, W.cfL, W.cfU);
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_gather_couplings<T>, dim3(g), dim3(256), 0, s, n, P.d_vxL, val, W.cfL);
+    hipLaunchKernelGGL(k_gather_couplings<T>, dim3(g), dim3(256), 0, s, n, P.d_vxU, val, W.cfU);
+    if (P.ncx)
+        hipLaunchKernelGGL(k_gather_couplings<T>, dim3((unsigned)((P.ncx + 255) / 256)), dim3(256), 0, s,
+                           P.ncx, P.d_vxC, val, W.cfC);
     V2D_LAUNCH();
     return 0;
 }
@@ -1791,7 +1950,7 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
     // right-hand sides from the padded coupling lists when the factorisation
     // filled them (the backward steps then also keep a level-ordered copy of
     // the solution, in wv, for their sources)
-    const DirectWork<T> *pad = (P.sweep_ok && W.cfL && W.wv) ? &W : nullptr;
+    const DirectWork<T> *pad = (P.pad_ok && W.cfL && W.wv) ? &W : nullptr;
     // forward sweeps
     for (int st = 0; st < steps; ++st) {
         SweepBatch<T> G{};

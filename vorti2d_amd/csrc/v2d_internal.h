@@ -153,11 +153,20 @@ struct DirectPattern {
     // device CSR (rows / columns indexed in level order), entries (local, vidx)
     int *d_Dptr = nullptr, *d_Lptr = nullptr, *d_Uptr = nullptr, *d_UCptr = nullptr, *d_LCptr = nullptr;
     int2 *d_D = nullptr, *d_Lr = nullptr, *d_Ur = nullptr, *d_Uc = nullptr, *d_Lc = nullptr;
-    // Solve sweeps in one persistent launch: the couplings of every level row
-    // into level l-1 (L) and l+1 (U), padded to kCouple entries, stored
-    // [kCouple][n_l] at kCouple * voff[l]: the level-ordered source index and
-    // the value index (-1: padding).
+    // The couplings of every level row into level l-1 (L) and l+1 (U), padded
+    // to kCouple entries, stored [kCouple][n_l] at kCouple * voff[l]: the
+    // level-ordered source index and the value index (-1: padding).
     int *d_ixL = nullptr, *d_ixU = nullptr, *d_vxL = nullptr, *d_vxU = nullptr;
+    // The couplings into every level column q of level l from the rows of level
+    // l-1 (side 0, entries of U_{l-1}) and l+1 (side 1, entries of L_{l+1}),
+    // padded to cx_w[side][l] in {4, 8} entries, stored [cx_w][n_l] at
+    // cx_off[side][l]: the row in the neighbouring level and the value index
+    // (-1: padding).  ncx entries in all.
+    int *d_jxC = nullptr, *d_vxC = nullptr;
+    std::vector<int64_t> cx_off[2];
+    std::vector<int> cx_w[2];
+    int64_t ncx = 0;
+    bool pad_ok = false;     // padded row and column tables built
     int nsteps = 0;          // sweep steps: forward, middle level, backward
     int sweep_C = 1;         // workgroups per elimination chain
     int sweep_grid = 0;      // persistent workgroups: 2 sweep_C, one per compute unit
@@ -205,7 +214,8 @@ struct DirectWork {
     LaunchProbe *probe = nullptr;   // set to time the update launches
     // persistent solve sweeps (sweep_kind 1; 0: one launch pair per level step)
     int sweep_kind = 1;
-    T *cfL = nullptr, *cfU = nullptr;   // coupling values of the factored Jacobian (padded lists)
+    T *cfL = nullptr, *cfU = nullptr;   // coupling values of the factored Jacobian (padded rows)
+    T *cfC = nullptr;                   // the same, padded columns (ncx values)
     T *wv = nullptr;                    // level-ordered backward-sweep results
     T *yv = nullptr;                    // level-ordered right-hand sides of the sweep steps
     int *ctr = nullptr;                 // progress word of every sweep workgroup
