@@ -198,13 +198,14 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
             return -1;
         }
     // Padded coupling columns: column q of level l in U_{l-1} (side 0) and in
-    // L_{l+1} (side 1), padded to cx_w in {4, 8} entries, stored [cx_w][n_l] at
-    // cx_off (row in the neighbouring level, value index or -1).
+    // L_{l+1} (side 1), padded to the level's widest column (cx_w, 1 to 8
+    // entries), stored [cx_w][n_l] at cx_off (row in the neighbouring level,
+    // value index or -1).
     std::vector<int> jxC, vxC;
     P.ncx = 0;
     for (int side = 0; side < 2; ++side) {
         P.cx_off[side].assign(nlev, 0);
-        P.cx_w[side].assign(nlev, 4);
+        P.cx_w[side].assign(nlev, 1);
     }
     for (int side = 0; side < 2; ++side)
         for (int l = 0; l < nlev; ++l) {
@@ -215,7 +216,7 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
                 set_error("direct solver: a column couples to more than 8 unknowns of a neighbouring level");
                 return -1;
             }
-            P.cx_w[side][l] = w <= 4 ? 4 : 8;
+            P.cx_w[side][l] = std::max(w, 1);
             P.cx_off[side][l] = P.ncx;
             P.ncx += (int64_t)P.cx_w[side][l] * P.n[l];
         }
@@ -1247,7 +1248,7 @@ static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, b
         return -1;
     }
     BuildBatch<T> bb{};
-    int nt = 0, nmax = 0, ldmax = 0, npmax = 0, kcmax = 4;
+    int nt = 0, nmax = 0, ldmax = 0, npmax = 0, kcmax = 1;
     for (int z = 0; z < B.nj; ++z) {
         const Job<T> &J = B.j[z];
         LevelBuild<T> &lv = bb.lv[z];
@@ -1288,8 +1289,12 @@ static int build_and_invert(const DirectPattern &P, Batch<T> &B, const T *val, b
     if (nt) {
         const dim3 g((unsigned)((nmax + 63) / 64), (unsigned)((npmax + 4 * kTmRows - 1) / (4 * kTmRows)),
                      (unsigned)nt);
-        if (kcmax <= 4) hipLaunchKernelGGL((k_schur_tm<T, 4>), g, dim3(256), 0, s, bb);
-        else hipLaunchKernelGGL((k_schur_tm<T, 8>), g, dim3(256), 0, s, bb);
+        switch (kcmax) {   // the widest column of the launch (narrower ones read as 0 past theirs)
+        case 1: case 2: case 3: case 4: hipLaunchKernelGGL((k_schur_tm<T, 4>), g, dim3(256), 0, s, bb); break;
+        case 5: hipLaunchKernelGGL((k_schur_tm<T, 5>), g, dim3(256), 0, s, bb); break;
+        case 6: hipLaunchKernelGGL((k_schur_tm<T, 6>), g, dim3(256), 0, s, bb); break;
+        default: hipLaunchKernelGGL((k_schur_tm<T, 8>), g, dim3(256), 0, s, bb); break;
+        }
     }
     hipLaunchKernelGGL(k_build_level<T>,
                        dim3((unsigned)(ldmax / 64), (unsigned)((nmax + 4 * kSbRows - 1) / (4 * kSbRows)),
@@ -1499,45 +1504,47 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
     const int n = J.rhs.n, ld = J.ld;
     const int row0 = blockIdx.x * (4 * RW);
     if (row0 >= n) return;
-    extern __shared__ unsigned char smem[];
-    T *vs = reinterpret_cast<T *>(smem);
+    extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nv = (n + W - 1) / W;
+    // y's loads go out first and the S rows' after them: vector memory
+    // returns in order, so the LDS stores of y wait for y alone while the
+    // rows are still in flight
+    constexpr int YP = W == 4 ? 3 : 5;
+    const V *yv = reinterpret_cast<const V *>(J.y);
+    V yr[YP];
+#pragma unroll
+    for (int p = 0; p < YP; ++p) yr[p] = yv[min((int)threadIdx.x + 256 * p, nv - 1)];
+    __builtin_amdgcn_sched_barrier(0);   // keep y's loads ahead of the rows'
+    // unconditional loads of clamped columns (no branch between them); the
+    // columns past the row are skipped in the sums
     V pre[RW][kSweepPre];
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
         const int r = min(row0 + wave * RW + q, n - 1);
         const V *row = reinterpret_cast<const V *>(J.S + (int64_t)r * ld);
-        // unconditional loads of clamped columns (no branch between them),
-        // the entries past the row zeroed afterwards
 #pragma unroll
         for (int u = 0; u < kSweepPre; ++u) pre[q][u] = row[min(lane + 64 * u, nv - 1)];
-#pragma unroll
-        for (int u = 0; u < kSweepPre; ++u)
-            if (lane + 64 * u >= nv) pre[q][u] = V{};
     }
-    // y into LDS as 16-byte vectors, every load issued before the first
-    // store (a loop of load-store pairs would wait for each load in turn);
-    // entries past n are zeroed
-    {
-        constexpr int YP = 3;
-        const V *yv = reinterpret_cast<const V *>(J.y);
-        V yr[YP];
+    // y into LDS as 16-byte vectors, entries past n zeroed
+    V *vsv = reinterpret_cast<V *>(smem);
 #pragma unroll
-        for (int p = 0; p < YP; ++p) yr[p] = yv[min((int)threadIdx.x + 256 * p, nv - 1)];
+    for (int p = 0; p < YP; ++p) {
+        const int c = threadIdx.x + 256 * p;
+        if (c < nv) {
+            V x = yr[p];
 #pragma unroll
-        for (int p = 0; p < YP; ++p) {
-            const int c = threadIdx.x + 256 * p;
-            if (c < nv)
-#pragma unroll
-                for (int e = 0; e < W; ++e)
-                    vs[c * W + e] = c * W + e < n ? reinterpret_cast<const T *>(&yr[p])[e] : T(0);
+            for (int e = 0; e < W; ++e)
+                if (c * W + e >= n) reinterpret_cast<T *>(&x)[e] = T(0);
+            vsv[c] = x;
         }
-        for (int c = threadIdx.x + 256 * YP; c < nv; c += 256) {
-            const V x = yv[c];
+    }
+    for (int c = threadIdx.x + 256 * YP; c < nv; c += 256) {
+        V x = yv[c];
 #pragma unroll
-            for (int e = 0; e < W; ++e) vs[c * W + e] = c * W + e < n ? reinterpret_cast<const T *>(&x)[e] : T(0);
-        }
+        for (int e = 0; e < W; ++e)
+            if (c * W + e >= n) reinterpret_cast<T *>(&x)[e] = T(0);
+        vsv[c] = x;
     }
     __syncthreads();
 #pragma unroll
@@ -1549,13 +1556,18 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
         for (int u = 0; u < kSweepPre; ++u) {
             const int c = lane + 64 * u;
             if (c < nv) {
-                if (u & 1) acc1 += dot16<T>(pre[q][u], vs + c * W);
-                else acc0 += dot16<T>(pre[q][u], vs + c * W);
+                const V yc = vsv[c];
+                if (u & 1) acc1 += dot16<T>(pre[q][u], reinterpret_cast<const T *>(&yc));
+                else acc0 += dot16<T>(pre[q][u], reinterpret_cast<const T *>(&yc));
             }
         }
         for (int c = lane + 64 * kSweepPre; c < nv; c += 128) {
-            acc0 += dot16<T>(row[c], vs + c * W);
-            if (c + 64 < nv) acc1 += dot16<T>(row[c + 64], vs + (c + 64) * W);
+            const V y0 = vsv[c];
+            acc0 += dot16<T>(row[c], reinterpret_cast<const T *>(&y0));
+            if (c + 64 < nv) {
+                const V y1 = vsv[c + 64];
+                acc1 += dot16<T>(row[c + 64], reinterpret_cast<const T *>(&y1));
+            }
         }
         const double acc = wsum(acc0 + acc1);
         if (lane == 0 && r < n) {

@@ -159,7 +159,7 @@ struct DirectPattern {
     int *d_ixL = nullptr, *d_ixU = nullptr, *d_vxL = nullptr, *d_vxU = nullptr;
     // The couplings into every level column q of level l from the rows of level
     // l-1 (side 0, entries of U_{l-1}) and l+1 (side 1, entries of L_{l+1}),
-    // padded to cx_w[side][l] in {4, 8} entries, stored [cx_w][n_l] at
+    // padded to cx_w[side][l] (1 to 8) entries, stored [cx_w][n_l] at
     // cx_off[side][l]: the row in the neighbouring level and the value index
     // (-1: padding).  ncx entries in all.
     int *d_jxC = nullptr, *d_vxC = nullptr;
