@@ -476,6 +476,59 @@ __device__ __forceinline__ double recip(double x)
     return fma(fma(-x, r, 1.0), r, r);
 }
 
+// Value of lane N of the lane's own 16-lane row (DPP row_newbcast:N, no LDS
+// round trip), and of lane L of the wavefront (readlane, uniform).
+template <int N>
+__device__ __forceinline__ float row_bcast(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + N, 0xf, 0xf, false));
+}
+template <int N>
+__device__ __forceinline__ double row_bcast(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + N, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + N, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+template <int P, typename T>
+__device__ __forceinline__ void gj16_pivot(T (&d)[4], T &sc, int g, int c)
+{
+    // pivot P of the in-register 16 x 16 inverse (lane 16 g + c holds column
+    // c of rows 4 g + q): the pivot row crosses 16-lane rows (a shuffle), the
+    // pivot value is one lane's (readlane), the pivot-column entries of the
+    // lane's rows sit in lane P of its own 16-lane row (DPP)
+    constexpr int pg = P >> 2, pq = P & 3;
+    const T prow = __shfl(d[pq], pg * 16 + c, 64);
+    const T piv = lane_value(d[pq], pg * 16 + P);
+    T pc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pc[q] = row_bcast<P>(d[q]);
+    const T ip = recip(piv);
+    const bool isp = c == P;
+    const T rp = isp ? T(0) : prow * ip;
+    sc = isp ? -ip : sc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] = fma(-pc[q], rp, d[q]);
+    if (g == pg) d[pq] = isp ? T(-1) : rp;
+}
+template <int P, typename T>
+__device__ __forceinline__ void gj16_pivots(T (&d)[4], T &sc, int g, int c)
+{
+    if constexpr (P < 16) {
+        gj16_pivot<P>(d, sc, g, c);
+        gj16_pivots<P + 1>(d, sc, g, c);
+    }
+}
+
 template <typename T> struct Mfma;
 
 // Inverse of the leading kb x kb block (kb <= 64) of a 64 x 64 tile held in
@@ -483,8 +536,9 @@ template <typename T> struct Mfma;
 // pivoting over 16-wide steps K = [16 k, 16 k + 16):
 //   1. wavefront 0 inverts the diagonal block D = A[K, K] in registers: lane l
 //      holds column l % 16 of rows 4 (l / 16) + q, q < 4; per pivot p the
-//      pivot row, pivot value and the lane's pivot-column entries arrive by
-//      cross-lane shuffles (no barrier); the pivot column is scaled by
+//      pivot row arrives by a cross-lane shuffle, the pivot value by readlane
+//      and the lane's pivot-column entries by a DPP broadcast within its
+//      16-lane row (no barrier, one LDS round trip); the pivot column is scaled by
 //      -1 / a[p][p] once at the end, its diagonal entry set to -1 on the way
 //      so that it ends as 1 / a[p][p];
 //   2. on the matrix cores (16x16x4 blocks, operands straight from LDS),
@@ -531,22 +585,7 @@ __device__ void gj_tile(T *sa, int lda, int kb, T (&a)[16], T *out)
 #pragma unroll
             for (int q = 0; q < 4; ++q) d[q] = sa[(K0 + 4 * g + q) * lda + K0 + c];
             T sc = T(0);
-#pragma unroll
-            for (int p = 0; p < 16; ++p) {
-                const int pg = p >> 2, pq = p & 3;
-                const T prow = __shfl(d[pq], pg * 16 + c, 64);
-                const T piv = __shfl(d[pq], pg * 16 + p, 64);
-                T pc[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) pc[q] = __shfl(d[q], g * 16 + p, 64);
-                const T ip = recip(piv);
-                const bool isp = c == p;
-                const T rp = isp ? T(0) : prow * ip;
-                sc = isp ? -ip : sc;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) d[q] = fma(-pc[q], rp, d[q]);
-                if (g == pg) d[pq] = isp ? T(-1) : rp;
-            }
+            gj16_pivots<0>(d, sc, g, c);
 #pragma unroll
             for (int q = 0; q < 4; ++q) sa[(K0 + 4 * g + q) * lda + K0 + c] = d[q] * sc;
         }
