@@ -499,14 +499,14 @@ __device__ __forceinline__ double lane_value(double v, int l)
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                             __builtin_amdgcn_readlane(__double2loint(v), l));
 }
-template <int P, typename T>
+// Pivot P of the in-register 16 x 16 inverse: lane 16 g + c holds column c
+// of rows 4 g + q (RG) or 4 q + g, q < 4.  The pivot row crosses 16-lane rows
+// (a shuffle), the pivot value is one lane's (readlane), the pivot-column
+// entries of the lane's rows sit in lane P of its own 16-lane row (DPP).
+template <bool RG, int P, typename T>
 __device__ __forceinline__ void gj16_pivot(T (&d)[4], T &sc, int g, int c)
 {
-    // pivot P of the in-register 16 x 16 inverse (lane 16 g + c holds column
-    // c of rows 4 g + q): the pivot row crosses 16-lane rows (a shuffle), the
-    // pivot value is one lane's (readlane), the pivot-column entries of the
-    // lane's rows sit in lane P of its own 16-lane row (DPP)
-    constexpr int pg = P >> 2, pq = P & 3;
+    constexpr int pg = RG ? P >> 2 : P & 3, pq = RG ? P & 3 : P >> 2;
     const T prow = __shfl(d[pq], pg * 16 + c, 64);
     const T piv = lane_value(d[pq], pg * 16 + P);
     T pc[4];
@@ -520,34 +520,44 @@ __device__ __forceinline__ void gj16_pivot(T (&d)[4], T &sc, int g, int c)
     for (int q = 0; q < 4; ++q) d[q] = fma(-pc[q], rp, d[q]);
     if (g == pg) d[pq] = isp ? T(-1) : rp;
 }
-template <int P, typename T>
+template <bool RG, int P, typename T>
 __device__ __forceinline__ void gj16_pivots(T (&d)[4], T &sc, int g, int c)
 {
     if constexpr (P < 16) {
-        gj16_pivot<P>(d, sc, g, c);
-        gj16_pivots<P + 1>(d, sc, g, c);
+        gj16_pivot<RG, P>(d, sc, g, c);
+        gj16_pivots<RG, P + 1>(d, sc, g, c);
     }
+}
+// In-register inverse of a 16 x 16 block held by one wavefront (layout as
+// above): Gauss-Jordan without pivoting; the pivot column is scaled by
+// -1 / a[p][p] once at the end, its diagonal entry set to -1 on the way so
+// that it ends as 1 / a[p][p].
+template <bool RG, typename T>
+__device__ __forceinline__ void gj16_invert(T (&d)[4], int g, int c)
+{
+    T sc = T(0);
+    gj16_pivots<RG, 0>(d, sc, g, c);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] *= sc;
 }
 
 template <typename T> struct Mfma;
 
 // Inverse of the leading kb x kb block (kb <= 64) of a 64 x 64 tile held in
 // LDS (sa, row stride lda), in place, by blocked Gauss-Jordan without
-// pivoting over 16-wide steps K = [16 k, 16 k + 16):
-//   1. wavefront 0 inverts the diagonal block D = A[K, K] in registers: lane l
-//      holds column l % 16 of rows 4 (l / 16) + q, q < 4; per pivot p the
-//      pivot row arrives by a cross-lane shuffle, the pivot value by readlane
-//      and the lane's pivot-column entries by a DPP broadcast within its
-//      16-lane row (no barrier, one LDS round trip); the pivot column is scaled by
-//      -1 / a[p][p] once at the end, its diagonal entry set to -1 on the way
-//      so that it ends as 1 / a[p][p];
-//   2. on the matrix cores (16x16x4 blocks, operands straight from LDS),
-//      wavefronts 0..2 form the new pivot rows R = D^-1 A[K, J] and pivot
-//      columns C = -A[I, K] D^-1 of the three 16-blocks I, J outside K;
-//   3. C replaces A[I, K]; the four wavefronts form the nine trailing blocks
-//      A[I, J] + C_I A[K, J] with the pivot rows still old;
-//   4. the trailing blocks and R are stored.
-// Five barriers per step.  Padding rows and columns (kb .. 63) are zero on
+// pivoting over 16-wide steps K = [16 k, 16 k + 16), the diagonal block D of
+// step k already inverted in place (wavefront 0 inverts the first one in
+// registers, gj16_invert):
+//   1. on the matrix cores (16x16x4 blocks, operands straight from LDS):
+//      wavefront w < 3 takes the row block I = ob(w) outside K: C = A[I, K] D^-1
+//      into A[I, K] (negated; its own rows, so no barrier before it reads them
+//      back), then its three trailing blocks A[I, J] + C A[K, J], J outside K,
+//      with the pivot rows still old; the wavefront whose rows hold the next
+//      diagonal block inverts that block in registers right away (the matrix
+//      core output layout is the inverse's), overlapping the others' work;
+//      wavefront 3 forms the pivot rows R = D^-1 A[K, J];
+//   2. the trailing blocks and R are stored.
+// Two barriers per step.  Padding rows and columns (kb .. 63) are zero on
 // entry; they are set to the identity for the sweep (so the tile is block
 // diagonal) and to zero again at the end.  256 threads.  On exit
 // a[q] = inverse[16 (t / 64) + q][t % 64] and out (row stride 64) holds the
@@ -570,66 +580,59 @@ __device__ void gj_tile(T *sa, int lda, int kb, T (&a)[16], T *out)
 {
     using M = Mfma<T>;
     using C = typename M::C;
+    constexpr bool RG = M::kRowsByGroup;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, lk = lane >> 4;
     __syncthreads();
     for (int r = kb + t; r < NB; r += 256) sa[r * lda + r] = T(1);
     __syncthreads();
     const int nk = (kb + 15) / 16;
+    if (wave == 0) {
+        T d[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = sa[M::drow(lk, q) * lda + li];
+        gj16_invert<RG>(d, lk, li);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sa[M::drow(lk, q) * lda + li] = d[q];
+    }
+    __syncthreads();
     for (int k = 0; k < nk; ++k) {
         const int K0 = 16 * k;
         auto ob = [k](int o) { return 16 * (o < k ? o : o + 1); };   // the three blocks outside K
-        if (wave == 0) {
-            const int c = lane & 15, g = lane >> 4;
-            T d[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) d[q] = sa[(K0 + 4 * g + q) * lda + K0 + c];
-            T sc = T(0);
-            gj16_pivots<0>(d, sc, g, c);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) sa[(K0 + 4 * g + q) * lda + K0 + c] = d[q] * sc;
-        }
-        __syncthreads();
-        C rt = {}, ct = {};
-        if (wave < 3) {
-            const int B0 = ob(wave);
-            rt = mm16<T>(sa, lda, K0, K0, K0, B0, rt);   // D^-1 A[K, B]
-            ct = mm16<T>(sa, lda, B0, K0, K0, K0, ct);   // A[B, K] D^-1
-        }
-        __syncthreads();
-        if (wave < 3) {
-            const int B0 = ob(wave);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) sa[(B0 + M::drow(lk, v)) * lda + K0 + li] = -ct[v];
-        }
-        __syncthreads();
         C tt[3];
+        if (wave < 3) {
+            const int I0 = ob(wave);
+            const C ct = mm16<T>(sa, lda, I0, K0, K0, K0, C{});   // A[I, K] D^-1
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            const int ti = wave + 4 * u;
-            if (ti < 9) {
-                const int I0 = ob(ti / 3), J0 = ob(ti % 3);
+            for (int v = 0; v < 4; ++v) sa[(I0 + M::drow(lk, v)) * lda + K0 + li] = -ct[v];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int J0 = ob(j);
                 C acc;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[v] = sa[(I0 + M::drow(lk, v)) * lda + J0 + li];
-                tt[u] = mm16<T>(sa, lda, I0, K0, K0, J0, acc);
+                tt[j] = mm16<T>(sa, lda, I0, K0, K0, J0, acc);
             }
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (j == k && wave == k && k + 1 < nk) {   // block (k + 1, k + 1): the next diagonal
+                    T d[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) d[q] = tt[j][q];
+                    gj16_invert<RG>(d, lk, li);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) tt[j][q] = d[q];
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tt[j] = mm16<T>(sa, lda, K0, K0, K0, ob(j), C{});   // D^-1 A[K, J]
         }
         __syncthreads();
+        const int R0 = wave < 3 ? ob(wave) : K0;
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            const int ti = wave + 4 * u;
-            if (ti < 9) {
-                const int I0 = ob(ti / 3), J0 = ob(ti % 3);
+        for (int j = 0; j < 3; ++j)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) sa[(I0 + M::drow(lk, v)) * lda + J0 + li] = tt[u][v];
-            }
-        }
-        if (wave < 3) {
-            const int B0 = ob(wave);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) sa[(K0 + M::drow(lk, v)) * lda + B0 + li] = rt[v];
-        }
+            for (int v = 0; v < 4; ++v) sa[(R0 + M::drow(lk, v)) * lda + ob(j) + li] = tt[j][v];
         __syncthreads();
     }
     const int c = lane, rg = wave;
@@ -889,6 +892,7 @@ template <> struct Mfma<float> {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ int drow(int lk, int v) { return 4 * lk + v; }
+    static constexpr bool kRowsByGroup = true;   // drow = 4 lk + v
 };
 template <> struct Mfma<double> {
     using C = double __attribute__((ext_vector_type(4)));
@@ -901,6 +905,7 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ int col(int li, int lk, int v) { return li; }
     static __device__ __forceinline__ C opd(double a, double b, C c) { return op(a, b, c); }
     static __device__ __forceinline__ int drow(int lk, int v) { return 4 * v + lk; }
+    static constexpr bool kRowsByGroup = false;   // drow = 4 v + lk
 };
 
 // One Gauss-Jordan step of the MFMA form as a single launch (dataflow): every
