@@ -1466,12 +1466,13 @@ __device__ __forceinline__ double dot16<double>(const double2 &a, const double *
 // One solve step of up to two levels, as two launches:
 //   k_level_rhs_b:  y = [b_l] + sgn_rhs * sum_t list_t(r) . vec_t   (one row per thread)
 //   k_level_gemv_b: out = [base] + sgn_mv * S y
-// The GEMV issues the loads of its own S rows first (two rows per wavefront,
-// up to kSweepPre 16-byte vectors per lane held in registers), stages y in
-// LDS while they are in flight, then forms the dot products (row parts past
-// the prefetch are streamed).  Output level-ordered (glob = 0) or scattered
-// to the global vector.
-constexpr int kSweepRowsPerWave = 2;
+// The GEMV issues the loads of y and then of its S rows (one row per
+// wavefront, up to kSweepPre 16-byte vectors per lane held in registers; one
+// row measured faster than two or three), stages y in LDS while the rows are
+// in flight, then forms the dot products (row parts past the prefetch are
+// streamed).  Output level-ordered (glob = 0) or scattered to the global
+// vector.
+constexpr int kSweepRowsPerWave = 1;
 constexpr int kSweepPre = 9;
 
 template <typename T>
