@@ -1474,6 +1474,7 @@ __device__ __forceinline__ double dot16<double>(const double2 &a, const double *
 // vector.
 constexpr int kSweepRowsPerWave = 1;
 constexpr int kSweepPre = 9;
+constexpr int kGemvThreads = 256;   // four wavefronts per workgroup (measured faster than two or eight)
 
 template <typename T>
 struct SweepJob {
@@ -1539,7 +1540,7 @@ __global__ void __launch_bounds__(256) k_level_rhs_b(SweepBatch<T> B, Levels L, 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L, int64_t N,
+__global__ void __launch_bounds__(kGemvThreads) k_level_gemv_b(SweepBatch<T> B, Levels L, int64_t N,
                                                       double sgn_mv)
 {
     using V = typename Vec16<T>::type;
@@ -1547,7 +1548,7 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
     constexpr int RW = kSweepRowsPerWave;
     const SweepJob<T> &J = B.j[blockIdx.y];
     const int n = J.rhs.n, ld = J.ld;
-    const int row0 = blockIdx.x * (4 * RW);
+    const int row0 = blockIdx.x * (kGemvThreads / 64 * RW);
     if (row0 >= n) return;
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1555,11 +1556,11 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
     // y's loads go out first and the S rows' after them: vector memory
     // returns in order, so the LDS stores of y wait for y alone while the
     // rows are still in flight
-    constexpr int YP = W == 4 ? 3 : 5;
+    constexpr int YP = (W == 4 ? 768 : 1280) / kGemvThreads;   // y vectors per thread loaded ahead
     const V *yv = reinterpret_cast<const V *>(J.y);
     V yr[YP];
 #pragma unroll
-    for (int p = 0; p < YP; ++p) yr[p] = yv[min((int)threadIdx.x + 256 * p, nv - 1)];
+    for (int p = 0; p < YP; ++p) yr[p] = yv[min((int)threadIdx.x + kGemvThreads * p, nv - 1)];
     __builtin_amdgcn_sched_barrier(0);   // keep y's loads ahead of the rows'
     // unconditional loads of clamped columns (no branch between them); the
     // columns past the row are skipped in the sums
@@ -1575,7 +1576,7 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
     V *vsv = reinterpret_cast<V *>(smem);
 #pragma unroll
     for (int p = 0; p < YP; ++p) {
-        const int c = threadIdx.x + 256 * p;
+        const int c = threadIdx.x + kGemvThreads * p;
         if (c < nv) {
             V x = yr[p];
 #pragma unroll
@@ -1584,7 +1585,7 @@ __global__ void __launch_bounds__(256) k_level_gemv_b(SweepBatch<T> B, Levels L,
             vsv[c] = x;
         }
     }
-    for (int c = threadIdx.x + 256 * YP; c < nv; c += 256) {
+    for (int c = threadIdx.x + kGemvThreads * YP; c < nv; c += kGemvThreads) {
         V x = yv[c];
 #pragma unroll
         for (int e = 0; e < W; ++e)
@@ -1629,7 +1630,7 @@ template <typename T>
 static int launch_sweep(const SweepBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
                         double sgn_rhs, double sgn_mv, hipStream_t s, const DirectWork<T> *pad = nullptr)
 {
-    const unsigned rows_per_block = 4 * kSweepRowsPerWave;
+    const unsigned rows_per_block = kGemvThreads / 64 * kSweepRowsPerWave;
     if (pad)
         hipLaunchKernelGGL(k_level_rhs_pad<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N,
                            (const T *)pad->cfL, (const T *)pad->cfU, P.d_ixL, P.d_ixU, sgn_rhs);
@@ -1637,7 +1638,7 @@ static int launch_sweep(const SweepBatch<T> &B, int nj, int nmax, const DirectPa
         hipLaunchKernelGGL(k_level_rhs_b<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N, val,
                            sgn_rhs);
     hipLaunchKernelGGL(k_level_gemv_b<T>, dim3((nmax + rows_per_block - 1) / rows_per_block, nj, 1),
-                       dim3(256), sizeof(T) * P.ldmax, s, B, P.L, P.N, sgn_mv);
+                       dim3(kGemvThreads), sizeof(T) * P.ldmax, s, B, P.L, P.N, sgn_mv);
     V2D_LAUNCH();
     return 0;
 }
