@@ -1472,6 +1472,19 @@ __device__ __forceinline__ double dot16<double>(const double2 &a, const double *
 // in flight, then forms the dot products (row parts past the prefetch are
 // streamed).  Output level-ordered (glob = 0) or scattered to the global
 // vector.
+// A 16-byte vector of a block-inverse row, loaded non-temporally: the rows
+// stream through once per solve (35 MB per step at 1025^2), so they should
+// not displace the right-hand sides and coupling lists from the caches.
+template <typename T>
+__device__ __forceinline__ typename Vec16<T>::type load_row_nt(const typename Vec16<T>::type *p)
+{
+    using N = T __attribute__((ext_vector_type(Vec16<T>::w)));
+    const N x = __builtin_nontemporal_load(reinterpret_cast<const N *>(p));
+    typename Vec16<T>::type v;
+    __builtin_memcpy(&v, &x, sizeof(v));
+    return v;
+}
+
 constexpr int kSweepRowsPerWave = 1;
 constexpr int kSweepPre = 9;
 constexpr int kGemvThreads = 256;   // four wavefronts per workgroup (measured faster than two or eight)
@@ -1570,7 +1583,7 @@ __global__ void __launch_bounds__(kGemvThreads) k_level_gemv_b(SweepBatch<T> B, 
         const int r = min(row0 + wave * RW + q, n - 1);
         const V *row = reinterpret_cast<const V *>(J.S + (int64_t)r * ld);
 #pragma unroll
-        for (int u = 0; u < kSweepPre; ++u) pre[q][u] = row[min(lane + 64 * u, nv - 1)];
+        for (int u = 0; u < kSweepPre; ++u) pre[q][u] = load_row_nt<T>(row + min(lane + 64 * u, nv - 1));
     }
     // y into LDS as 16-byte vectors, entries past n zeroed
     V *vsv = reinterpret_cast<V *>(smem);
@@ -1609,10 +1622,10 @@ __global__ void __launch_bounds__(kGemvThreads) k_level_gemv_b(SweepBatch<T> B, 
         }
         for (int c = lane + 64 * kSweepPre; c < nv; c += 128) {
             const V y0 = vsv[c];
-            acc0 += dot16<T>(row[c], reinterpret_cast<const T *>(&y0));
+            acc0 += dot16<T>(load_row_nt<T>(row + c), reinterpret_cast<const T *>(&y0));
             if (c + 64 < nv) {
                 const V y1 = vsv[c + 64];
-                acc1 += dot16<T>(row[c + 64], reinterpret_cast<const T *>(&y1));
+                acc1 += dot16<T>(load_row_nt<T>(row + c + 64), reinterpret_cast<const T *>(&y1));
             }
         }
         const double acc = wsum(acc0 + acc1);
