@@ -178,7 +178,7 @@ def test_ctx_export_matches_oracle(v, C, oracle_mod, prec, ff, ca, sa):
 
 
 @pytest.mark.parametrize("imax,jmax,kind,ff", [(41, 41, "cyl", 0), (57, 33, "ann", 2), (20, 4, "cyl", 0),
-                                               (25, 5, "cyl", 1), (33, 8, "ann", 2),
+                                               (25, 5, "cyl", 1), (33, 8, "ann", 2), (7, 6, "ann", 0),
                                                (181, 181, "ann", 1)])
 @pytest.mark.parametrize("upd", ["mfma64", "valu64", "valu128"])
 def test_direct_solve_fp64(v, C, oracle_mod, imax, jmax, kind, ff, upd):
