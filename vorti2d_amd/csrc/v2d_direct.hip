@@ -1188,8 +1188,11 @@ static int probe_flush(LaunchProbe &p)
     return 0;
 }
 
+// An event pair around the update launches of steps k0 = kbeg, kbeg + 64, ..
+// < kend (back to back on the stream: the pair times the launches and the
+// gaps between them, not a dispatch per launch).
 template <typename T>
-static int probe_mark(LaunchProbe *p, const Batch<T> &B, int k0, bool begin, hipStream_t s)
+static int probe_mark(LaunchProbe *p, const Batch<T> &B, int kbeg, int kend, bool begin, hipStream_t s)
 {
     if (!p) return 0;
     if (begin) {
@@ -1201,13 +1204,15 @@ static int probe_mark(LaunchProbe *p, const Batch<T> &B, int k0, bool begin, hip
     }
     V2D_HIP(hipEventRecord(p->ev[2 * p->used + 1], s));
     ++p->used;
-    ++p->launches;
-    for (int z = 0; z < B.nj; ++z) {
-        const double n = B.j[z].n;
-        if (k0 >= B.j[z].n) continue;
-        const double kb = std::min(NB, B.j[z].n - k0);
-        p->flops += 2.0 * (n - kb) * n * kb;
-        p->bytes += sizeof(T) * (2.0 * (n - kb) * n + (n - kb) * kb + kb * n);
+    for (int k0 = kbeg; k0 < kend; k0 += NB) {
+        ++p->launches;
+        for (int z = 0; z < B.nj; ++z) {
+            const double n = B.j[z].n;
+            if (k0 >= B.j[z].n) continue;
+            const double kb = std::min(NB, B.j[z].n - k0);
+            p->flops += 2.0 * (n - kb) * n * kb;
+            p->bytes += sizeof(T) * (2.0 * (n - kb) * n + (n - kb) * kb + kb * n);
+        }
     }
     return 0;
 }
@@ -1228,27 +1233,27 @@ static int gj_invert_b(const Batch<T> &B, DirectWork<T> &W, LaunchProbe *probe, 
         // per step, the panel buffers alternating
         hipLaunchKernelGGL(k_gj_rowpanel_b<T>, dim3(np, 2, nz), dim3(256), 0, s, B, 0);
         Batch<T> Bs = B;
+        if (probe_mark(probe, B, 0, nmax, true, s)) return -1;
         for (int k0 = 0; k0 < nmax; k0 += NB) {
-            if (probe_mark(probe, B, k0, true, s)) return -1;
             hipLaunchKernelGGL(k_gj_step_mfma_b<T>, dim3(nt * nt * nz), dim3(256), 0, s, Bs, k0,
                                ++W.epoch, W.flags, W.err);
-            if (probe_mark(probe, B, k0, false, s)) return -1;
             for (int z = 0; z < B.nj; ++z) {
                 std::swap(Bs.j[z].colk, Bs.j[z].colk_w);
                 std::swap(Bs.j[z].rowk, Bs.j[z].rowk_w);
             }
         }
+        if (probe_mark(probe, B, 0, nmax, false, s)) return -1;
         V2D_LAUNCH();
         return 0;
     }
     for (int k0 = 0; k0 < nmax; k0 += NB) {
         hipLaunchKernelGGL(k_gj_rowpanel_b<T>, dim3(np, 2, nz), dim3(256), 0, s, B, k0);
-        if (probe_mark(probe, B, k0, true, s)) return -1;
+        if (probe_mark(probe, B, k0, k0 + 1, true, s)) return -1;
         if (ut == 128)
             hipLaunchKernelGGL((k_gj_update_b<T, 128>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
         else
             hipLaunchKernelGGL((k_gj_update_b<T, 64>), dim3(nt * nt * nz), dim3(256), 0, s, B, k0);
-        if (probe_mark(probe, B, k0, false, s)) return -1;
+        if (probe_mark(probe, B, k0, k0 + 1, false, s)) return -1;
     }
     V2D_LAUNCH();
     return 0;

@@ -184,11 +184,12 @@ void direct_free_pattern(DirectPattern &P, hipStream_t s);
 // MFMA 16x16x4 (FP32 / FP64 matrix cores) on 64-wide tiles.
 enum UpdateKind { kUpdateValu64 = 0, kUpdateValu128 = 1, kUpdateMfma64 = 2 };
 
-// Optional per-launch timing of the Gauss-Jordan update kernel: an event pair
-// brackets every launch on the factorisation stream; pairs are read back in
-// groups of kPairs.  flops / bytes are the algorithmic counts of the timed
-// launches (2 (n-kb) n kb flops per job; the updated rows read and written
-// once, both panels read once).
+// Optional timing of the Gauss-Jordan update kernel on the factorisation
+// stream: an event pair brackets each level's run of back-to-back update
+// launches (MFMA steps; each launch with the VALU kernels, whose row-panel
+// launches sit in between); pairs are read back in groups of kPairs.
+// flops / bytes are the algorithmic counts of the timed launches (2 (n-kb) n kb
+// flops per job; the updated rows read and written once, both panels read once).
 struct LaunchProbe {
     static constexpr int kPairs = 64;
     hipEvent_t ev[2 * kPairs] = {};
