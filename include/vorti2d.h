@@ -135,9 +135,14 @@ int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind);
  * slice of every level, synchronised through per-workgroup progress words).  The persistent launch needs all
  * its workgroups resident at once; if another persistent launch holds the GPU
  * its bounded waits expire and the solve is redone with per-level launches,
- * which the context then keeps.  A tuning knob with no counterpart in the
- * reference: both compute the same sums in the same order. */
-enum { V2D_SWEEP_LEVELS = 0, V2D_SWEEP_PERSISTENT = 1 };
+ * which the context then keeps.  V2D_SWEEP_FUSED runs every FP32 level step
+ * whose right-hand side couples one neighbouring level as one launch (each
+ * workgroup forms the right-hand side itself) and the other steps as launch
+ * pairs; FP64 solves use launch pairs.  A tuning knob with no counterpart in
+ * the reference: all compute the same right-hand sides, and the same sums in
+ * the same order except for the rows of a level past 16 per compute unit,
+ * which the fused step sums in sixteenths. */
+enum { V2D_SWEEP_LEVELS = 0, V2D_SWEEP_PERSISTENT = 1, V2D_SWEEP_FUSED = 2 };
 int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind);
 /* Mixed precision: run the FP32 block-LU apply of every GMRES iteration as one
  * replay of a HIP graph captured over fixed device buffers (enable != 0, the

@@ -250,6 +250,72 @@ def test_persistent_sweep_equals_level_sweeps(v, C, oracle_mod, prec, imax, jmax
     assert rel <= (1e-10 if prec == "fp64" else 5e-2)
 
 
+@pytest.mark.parametrize("imax,jmax,kind,ff", [(17, 4, "cyl", 0), (41, 23, "cyl", 1), (73, 40, "ann", 2),
+                                               (1025, 8, "ann", 0)])
+def test_fused_sweep_matches_level_sweeps(v, C, oracle_mod, imax, jmax, kind, ff):
+    """FP32 solve with one fused launch per level step (every workgroup forms
+    the level right-hand side itself): the right-hand sides are the launch
+    pair's and every row but the split rows (a level's rows past 16 per
+    compute unit: the 2050-unknown levels of imax 1025 on two elimination
+    chains) is summed in the same order, so the solution is bit-identical
+    without split rows and agrees to FP32 rounding with them (their FP64 sums
+    differ in order only, so their FP32 roundings mostly coincide).  The
+    4100-unknown end levels of imax 1025 run as launch pairs."""
+    say(f"fused sweep {imax}x{jmax} {kind} ff={ff}")
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "mixed", ff)
+    ctx.assemble(40.0, 0.0, 2.5, 0.1)
+    ci, cj, cv, b = ctx.export_coo()
+    ctx.factor()
+    ctx.set_sweep_kernel("levels")
+    x_lv = ctx.solve(b)
+    ctx.set_sweep_kernel("fused")
+    x_fu = ctx.solve(b)
+    x_fu2 = ctx.solve(b)
+    assert np.all(np.isfinite(x_fu))
+    assert np.array_equal(x_fu, x_fu2)
+    N = imax * jmax
+    A = sp.csc_matrix((cv, (ci, cj)), shape=(2 * N, 2 * N))
+    if imax < 1025:
+        assert np.array_equal(x_fu, x_lv)
+    else:
+        xs = spla.spsolve(A, b)
+        err_lv = np.linalg.norm(x_lv - xs) / np.linalg.norm(xs)
+        d = np.linalg.norm(x_fu - x_lv) / np.linalg.norm(x_lv)
+        say(f"  levels error {err_lv:.2e}, fused - levels {d:.2e}")
+        assert d <= max(4.0 * err_lv, 1e-6)
+    rel = np.linalg.norm(b - A @ x_fu) / np.linalg.norm(b)
+    say(f"  relative residual {rel:.2e}")
+    assert rel <= 5e-2
+
+
+def test_fused_sweep_newton_equals_level_sweeps(v, oracle_mod):
+    """Mixed Newton-GMRES on 181^2 (refactoring every step, so no measured
+    timing steers it) with the fused solve steps, captured in the HIP graph of
+    the preconditioner apply: every level has 362 unknowns (no split rows), so
+    the residual history and the state equal the launch-pair run bit for bit."""
+    imax = jmax = 181
+    xg, yg = oracle_mod.generate_annulus(imax, jmax)
+    out = {}
+    for kind in ("levels", "fused"):
+        say(f"fused sweep newton: {kind}")
+        ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, "mixed")
+        ctx.set_solver(refactor_every=1, gmres_rtol=1e-6, gmres_max_iter=30)
+        ctx.set_sweep_kernel(kind)
+        ctx.init_state()
+        hist = []
+        res, k = 1.0, 0
+        while res > 1e-10 and k < 20:
+            res, rn, li = ctx.newton_step(40.0)
+            hist.append((res, rn, li))
+            k += 1
+        out[kind] = (hist, ctx.get_state(("psi", "ome")))
+        ctx.close()
+    assert out["fused"][0] == out["levels"][0]
+    assert out["fused"][0][-1][0] <= 1e-10
+    for a, b in zip(out["fused"][1], out["levels"][1]):
+        assert np.array_equal(a, b)
+
+
 def test_forcing_residual_target(v, C, oracle_mod):
     """With a residual target (the final-state acceptance ||R||_2 / (2N) <
     1e-10) on the forcing term, the mixed solve still meets both stopping

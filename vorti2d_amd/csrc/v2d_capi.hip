@@ -228,7 +228,7 @@ static int apply_pc(Ctx *c, const float *v, float *z)
 {
     const float *vpc = static_cast<const float *>(c->val_pc);
     const int64_t n = 2 * c->N;
-    if (!c->use_graph || c->wf.sweep_kind != 0 || !c->ga_in || !c->ga_out)
+    if (!c->use_graph || c->wf.sweep_kind == V2D_SWEEP_PERSISTENT || !c->ga_in || !c->ga_out)
         return direct_solve<float>(c->pat, c->wf, vpc, v, z, c->s);
     if (!c->age) {
         V2D_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
@@ -688,10 +688,12 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     c->wd.err = sync_words + 2;
     c->wf.flags = sync_words + 4;
     c->wf.err = sync_words + 6;
-    // Per-level sweep launches by default: on a 1025^2 mesh they take 17 us
-    // per level step against 25 us for the persistent launch, whose every step
-    // pays two cross-die release / acquire round trips.
-    c->wd.sweep_kind = c->wf.sweep_kind = 0;
+    // Per-level sweep launch pairs by default: on a 1025^2 mesh they take
+    // 12.2 us per level step against 25 us for the persistent launch, whose
+    // every step pays two cross-die release / acquire round trips.  The fused
+    // FP32 step (11.6 us) gains nothing measurable once the apply is a graph
+    // replay; it halves the solve time when launches are issued one by one.
+    c->wd.sweep_kind = c->wf.sweep_kind = V2D_SWEEP_LEVELS;
     V2D_HIP(hipHostMalloc((void **)&c->err_host, 2 * sizeof(int), hipHostMallocDefault));
     c->err_host[0] = c->err_host[1] = 0;
     c->wd.err_host = c->err_host;
@@ -818,8 +820,8 @@ static int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double
 
 int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind)
 {
-    if (kind != V2D_SWEEP_LEVELS && kind != V2D_SWEEP_PERSISTENT) {
-        set_error("sweep kernel must be V2D_SWEEP_LEVELS or V2D_SWEEP_PERSISTENT");
+    if (kind != V2D_SWEEP_LEVELS && kind != V2D_SWEEP_PERSISTENT && kind != V2D_SWEEP_FUSED) {
+        set_error("sweep kernel must be V2D_SWEEP_LEVELS, V2D_SWEEP_PERSISTENT or V2D_SWEEP_FUSED");
         return -1;
     }
     ctx->c->wd.sweep_kind = kind;

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "v2d_internal.h"
@@ -247,13 +248,14 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
     const int M = (nlev - 1) / 2;
     const int steps = std::max(M, nlev - 1 - M);
     P.nsteps = 2 * steps + 1;
+    int dev = 0, ncu = 0;
+    V2D_HIP(hipGetDevice(&dev));
+    V2D_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    P.ncu = ncu;
     if (closed) {
         // C workgroups per chain, one per compute unit: at most 16 rows of an
         // interior level each, at most 128 rows of any level (8 lanes per row
         // build the right-hand side)
-        int dev = 0, ncu = 0;
-        V2D_HIP(hipGetDevice(&dev));
-        V2D_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         P.sweep_C = std::max(1, std::min(ncu / 2, (2 * imax + kSwRows - 1) / kSwRows));
         P.sweep_rows = 1;
         for (int l = 0; l < nlev; ++l) P.sweep_rows = std::max(P.sweep_rows, (P.n[l] + P.sweep_C - 1) / P.sweep_C);
@@ -1507,6 +1509,7 @@ struct SweepJob {
     // couplings to level l-1, bit 2 to level l+1; source psrc, level-ordered
     int pterm;
     const T *psrc;
+    int fg;          // workgroups of the fused step (k_level_step_fused)
 };
 template <typename T>
 struct SweepBatch {
@@ -1644,11 +1647,212 @@ __global__ void __launch_bounds__(kGemvThreads) k_level_gemv_b(SweepBatch<T> B, 
     }
 }
 
+// One FP32 solve step as a single launch, for steps whose right-hand side
+// takes the couplings of one neighbouring level (every forward step but the
+// first, every backward step).  Every workgroup forms the whole level
+// right-hand side in LDS itself, from the padded coupling lists, with the
+// operations of k_level_rhs_pad, so it holds the same values bit for bit.
+// 16 wavefronts per workgroup; the G workgroups of a level own 16 rows each
+// (row 16 g + wave, prefetched in registers and summed as in k_level_gemv_b,
+// so these rows equal the launch pair's bit for bit).  The rows past 16 G
+// are split rows: workgroup g takes rows 16 G + E g .. 16 G + E g + E - 1,
+// each wavefront one sixteenth of their columns, summed through LDS in
+// wavefront order.  G is at most the compute units per job, so the whole
+// grid is resident at once and each right-hand side is formed once per
+// compute unit.  Loads go out in the order they are needed: the thread's
+// coefficients, source indices and b first, then the matrix rows, then the
+// source gathers, which wait for the indices only (vector memory returns in
+// order).
+constexpr int kFusedThreads = 1024;
+constexpr int kFusedRpt = 3;                             // right-hand-side rows per thread, at most
+constexpr int kFusedMaxRows = kFusedRpt * kFusedThreads;
+constexpr int kFusedSplit = 2;                           // split rows per workgroup, at most
+
+template <int RPT, int E>
+__global__ void __launch_bounds__(kFusedThreads) k_level_step_fused(SweepBatch<float> B, Levels L, int64_t N,
+                                                                    const float *__restrict__ cfL,
+                                                                    const float *__restrict__ cfU,
+                                                                    const int *__restrict__ ixL,
+                                                                    const int *__restrict__ ixU, double sgn_rhs,
+                                                                    double sgn_mv)
+{
+    using V = float4;
+    constexpr int W = 4;
+    const SweepJob<float> &J = B.j[blockIdx.y];
+    const RhsJob<float> &R = J.rhs;
+    const int n = R.n, ld = J.ld, G = J.fg, g = blockIdx.x;
+    if (g >= G) return;
+    __shared__ __align__(16) float ys[kFusedMaxRows];
+    __shared__ double part[kFusedSplit][16];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int nv = (n + W - 1) / W;
+    // 1. the thread's right-hand-side inputs (rows tid + 1024 q, clamped)
+    const int64_t cb = (int64_t)kCouple * lvl_voff(L, R.l);
+    const float *cf = J.pterm == 2 ? cfU : cfL;
+    const int *ix = J.pterm == 2 ? ixU : ixL;
+    // loaded unconditionally (a branch would wait for the load at its join):
+    // without b, any vector of 2N entries stands in and its value is unused
+    const float *bsrc = R.b ? R.b : J.psrc;
+    float cv[RPT][kCouple], bv[RPT];
+    int iv[RPT][kCouple];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = min(tid + kFusedThreads * q, n - 1);
+#pragma unroll
+        for (int e = 0; e < kCouple; ++e) {
+            cv[q][e] = cf[cb + e * n + r];
+            iv[q][e] = ix[cb + e * n + r];
+        }
+        bv[q] = bsrc[L.global(r, N, R.l)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // 2. the wavefront's row and its slices of the split rows
+    const V *row = reinterpret_cast<const V *>(J.S + (int64_t)min(16 * g + wave, n - 1) * ld);
+    V pre[kSweepPre];
+#pragma unroll
+    for (int u = 0; u < kSweepPre; ++u) pre[u] = load_row_nt<float>(row + min(lane + 64 * u, nv - 1));
+    const int sl = (nv + 15) / 16;                   // vectors per wavefront slice (at most 64)
+    const int sc = min(wave * sl + min(lane, sl - 1), nv - 1);
+    V sp[E > 0 ? E : 1];
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const int sr = min(16 * G + E * g + t, n - 1);
+        sp[t] = load_row_nt<float>(reinterpret_cast<const V *>(J.S + (int64_t)sr * ld) + sc);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // 3. the source gathers; the empty asm keeps the address arithmetic on
+    // the indices (and so the wait for them) behind the row loads
+    float sv[RPT][kCouple];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+#pragma unroll
+        for (int e = 0; e < kCouple; ++e) {
+            asm volatile("" : "+v"(iv[q][e]));
+            sv[q][e] = J.psrc[iv[q][e]];
+        }
+    // 4. the right-hand side into LDS: every row r < RPT * 1024 is written,
+    // so the entries from n up to the last 16-byte vector are zeros (an
+    // unconditional store: a conditional one lets the compiler sink the
+    // gathers behind the other loads)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = tid + kFusedThreads * q;
+        double sum = 0.0;
+#pragma unroll
+        for (int e = 0; e < kCouple; ++e) sum += (double)cv[q][e] * (double)sv[q][e];
+        const float y = (float)((R.b ? (double)bv[q] : 0.0) + sgn_rhs * sum);
+        ys[r] = r < n ? y : 0.f;
+    }
+    __syncthreads();
+    const V *vsv = reinterpret_cast<const V *>(ys);
+    auto store = [&](int r, double acc) {
+        const double b = J.base ? (double)J.base[r] : 0.0;
+        const float o = (float)(b + sgn_mv * acc);
+        if (J.glob) J.out[L.global(r, N, R.l)] = o;
+        else J.out[r] = o;
+        if (J.lout) J.lout[r] = o;
+    };
+    // 5. the wavefront's row
+    {
+        const int r = 16 * g + wave;
+        double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+        for (int u = 0; u < kSweepPre; ++u) {
+            const int c = lane + 64 * u;
+            if (c < nv) {
+                const V yc = vsv[c];
+                if (u & 1) acc1 += dot16<float>(pre[u], reinterpret_cast<const float *>(&yc));
+                else acc0 += dot16<float>(pre[u], reinterpret_cast<const float *>(&yc));
+            }
+        }
+        for (int c = lane + 64 * kSweepPre; c < nv; c += 128) {
+            const V y0 = vsv[c];
+            acc0 += dot16<float>(load_row_nt<float>(row + c), reinterpret_cast<const float *>(&y0));
+            if (c + 64 < nv) {
+                const V y1 = vsv[c + 64];
+                acc1 += dot16<float>(load_row_nt<float>(row + c + 64), reinterpret_cast<const float *>(&y1));
+            }
+        }
+        const double acc = wsum(acc0 + acc1);
+        if (lane == 0 && r < n) store(r, acc);
+    }
+    // 6. the split rows
+    if constexpr (E > 0) {
+#pragma unroll
+        for (int t = 0; t < E; ++t) {
+            double a = 0.0;
+            if (lane < sl && wave * sl + lane < nv) {
+                const V yc = vsv[wave * sl + lane];
+                a = dot16<float>(sp[t], reinterpret_cast<const float *>(&yc));
+            }
+            a = wsum(a);
+            if (lane == 0) part[t][wave] = a;
+        }
+        __syncthreads();
+        if (wave < E && lane == 0) {
+            const int sr = 16 * G + E * g + wave;
+            if (sr < n) {
+                double t = 0.0;
+#pragma unroll
+                for (int w = 0; w < 16; ++w) t += part[wave][w];
+                store(sr, t);
+            }
+        }
+    }
+}
+
+// Grid of the fused step: per job G = min(ceil(n / 16), compute units / jobs)
+// workgroups and E split rows per workgroup; false when the step does not fit
+// the fused kernel (then the launch pair runs).
 template <typename T>
-static int launch_sweep(const SweepBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
-                        double sgn_rhs, double sgn_mv, hipStream_t s, const DirectWork<T> *pad = nullptr)
+static bool fused_plan(SweepBatch<T> &B, int nj, int nmax, const DirectPattern &P, int &grid, int &rpt, int &e)
+{
+    if (!std::is_same<T, float>::value || nmax > kFusedMaxRows || P.ncu < nj) return false;
+    grid = 0; e = 0;
+    for (int j = 0; j < nj; ++j) {
+        const int pt = B.j[j].pterm;
+        if (pt != 1 && pt != 2) return false;
+        const int n = B.j[j].rhs.n;
+        const int G = std::min((n + 15) / 16, P.ncu / nj);
+        const int extra = std::max(0, n - 16 * G);
+        e = std::max(e, (extra + G - 1) / G);
+        B.j[j].fg = G;
+        grid = std::max(grid, G);
+    }
+    rpt = (nmax + kFusedThreads - 1) / kFusedThreads;
+    return e <= kFusedSplit;
+}
+
+template <int RPT>
+static void launch_fused_e(int e, dim3 grid, const SweepBatch<float> &B, const DirectPattern &P,
+                           const DirectWork<float> &W, double sgn_rhs, double sgn_mv, hipStream_t s)
+{
+    auto go = [&](auto k) {
+        hipLaunchKernelGGL(k, grid, dim3(kFusedThreads), 0, s, B, P.L, P.N, (const float *)W.cfL,
+                           (const float *)W.cfU, (const int *)P.d_ixL, (const int *)P.d_ixU, sgn_rhs, sgn_mv);
+    };
+    if (e == 0) go(k_level_step_fused<RPT, 0>);
+    else if (e == 1) go(k_level_step_fused<RPT, 1>);
+    else go(k_level_step_fused<RPT, 2>);
+}
+
+template <typename T>
+static int launch_sweep(SweepBatch<T> &B, int nj, int nmax, const DirectPattern &P, const T *val,
+                        double sgn_rhs, double sgn_mv, hipStream_t s, const DirectWork<T> *pad = nullptr,
+                        bool fused = false)
 {
     const unsigned rows_per_block = kGemvThreads / 64 * kSweepRowsPerWave;
+    int fgrid = 0, rpt = 0, e = 0;
+    if constexpr (std::is_same<T, float>::value) {
+        if (fused && pad && fused_plan(B, nj, nmax, P, fgrid, rpt, e)) {
+            const dim3 grid(fgrid, nj, 1);
+            if (rpt == 1) launch_fused_e<1>(e, grid, B, P, *pad, sgn_rhs, sgn_mv, s);
+            else if (rpt == 2) launch_fused_e<2>(e, grid, B, P, *pad, sgn_rhs, sgn_mv, s);
+            else launch_fused_e<3>(e, grid, B, P, *pad, sgn_rhs, sgn_mv, s);
+            V2D_LAUNCH();
+            return 0;
+        }
+    }
     if (pad)
         hipLaunchKernelGGL(k_level_rhs_pad<T>, dim3(grid_for(nmax), nj, 1), dim3(256), 0, s, B, P.L, P.N,
                            (const T *)pad->cfL, (const T *)pad->cfU, P.d_ixL, P.d_ixU, sgn_rhs);
@@ -2027,6 +2231,7 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
     // filled them (the backward steps then also keep a level-ordered copy of
     // the solution, in wv, for their sources)
     const DirectWork<T> *pad = (P.pad_ok && W.cfL && W.wv) ? &W : nullptr;
+    const bool fused = W.sweep_kind == 2;
     // forward sweeps
     for (int st = 0; st < steps; ++st) {
         SweepBatch<T> G{};
@@ -2049,7 +2254,7 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
             nmax = std::max(nmax, P.n[l]);
             ++nj;
         }
-        if (launch_sweep(G, nj, nmax, P, val, -1.0, 1.0, s, pad)) return -1;
+        if (launch_sweep(G, nj, nmax, P, val, -1.0, 1.0, s, pad, fused)) return -1;
     }
     {   // middle level
         SweepBatch<T> G{};
@@ -2087,7 +2292,7 @@ static int direct_solve_levels(const DirectPattern &P, const DirectWork<T> &W, c
             ++nj;
         }
         if (!nj) break;
-        if (launch_sweep(G, nj, nmax, P, val, 1.0, -1.0, s, pad)) return -1;
+        if (launch_sweep(G, nj, nmax, P, val, 1.0, -1.0, s, pad, fused)) return -1;
     }
     return 0;
 }

@@ -172,6 +172,7 @@ struct DirectPattern {
     int sweep_grid = 0;      // persistent workgroups: 2 sweep_C, one per compute unit
     int sweep_rows = 1;      // most rows of one workgroup at one level
     bool sweep_ok = false;   // closed-form level offsets verified, tables built
+    int ncu = 0;             // compute units of the device (grid of the fused solve steps)
 };
 
 constexpr int kCouple = 4;   // widest inter-level coupling of a row (host-checked)
@@ -213,7 +214,8 @@ struct DirectWork {
     int epoch = 0;          // value the current step's pivot block publishes
     int update_kind = kUpdateMfma64;
     LaunchProbe *probe = nullptr;   // set to time the update launches
-    // persistent solve sweeps (sweep_kind 1; 0: one launch pair per level step)
+    // solve sweeps: 0 one launch pair per level step, 1 one persistent launch,
+    // 2 one fused launch per FP32 level step (launch pairs elsewhere)
     int sweep_kind = 1;
     T *cfL = nullptr, *cfU = nullptr;   // coupling values of the factored Jacobian (padded rows)
     T *cfC = nullptr;                   // the same, padded columns (ncx values)

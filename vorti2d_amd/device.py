@@ -14,7 +14,7 @@ from ._lib import check, f64c, lib, ptr
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
 UPDATE_KERNELS = {"valu64": 0, "valu128": 1, "mfma64": 2}
-SWEEP_KERNELS = {"levels": 0, "persistent": 1}
+SWEEP_KERNELS = {"levels": 0, "persistent": 1, "fused": 2}
 
 
 class DeviceContext:
@@ -92,7 +92,9 @@ class DeviceContext:
 
     def set_sweep_kernel(self, kind: str | int):
         """Block-LU solve sweeps: "levels" (two launches per level step, the
-        default) or "persistent" (one launch per solve)."""
+        default), "persistent" (one launch per solve) or "fused" (one launch
+        per FP32 level step whose right-hand side couples one neighbouring
+        level, launch pairs for the others and in FP64)."""
         check(lib.v2d_ctx_set_sweep_kernel(self._h, SWEEP_KERNELS.get(kind, kind)))
 
     def set_graph(self, enable: bool):
