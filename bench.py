@@ -268,7 +268,7 @@ def main():
                          "captured graph (same kernels; for kernel-trace profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the sharded design "
+        print(json.dumps({"error": "vorti2d_amd runs on one GPU only; the sharded design "
                           "(circumferential slabs) is not built", "n_gpus": a.gpus}))
         sys.exit(2)
 

@@ -133,7 +133,7 @@ class Solver:
         if cfg.compute_forces and (not self.restarted or not os.path.exists(forces_path)):
             fields_io.init_force_history(cfg.out_dir, FORCE_COLUMNS)
         if cfg.write_xdmf:
-            warnings.warn("write_xdmf: XDMF/HDF5 output is not part of this build; "
+            warnings.warn("write_xdmf: vorti2d_amd has no XDMF/HDF5 output; "
                           "fields are written as CSV only", stacklevel=2)
 
         invdtau = cfg.inv_dtau()
