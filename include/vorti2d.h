@@ -97,6 +97,13 @@ enum { V2D_FP64 = 0, V2D_MIXED = 1 };
  * FP64 residual and state). */
 int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const double *yg,
                    int ff_bc, double ca, double sa, int precision);
+/* A new context in the given precision on src's mesh and far-field setting,
+ * with src's state (psi, omega and the two older omega planes), after src's
+ * queued work.  Mesh, state and the Jacobian pattern are copied on the device
+ * (no host pattern build); nothing is shared, so either context may be
+ * destroyed first.  No counterpart in the reference: for an FP64 adjoint
+ * context beside a mixed-precision solve. */
+int v2d_ctx_create_like(v2d_ctx **out, v2d_ctx *src, int precision);
 int v2d_ctx_destroy(v2d_ctx *ctx);
 /* Bytes of device memory held by the context. */
 int v2d_ctx_device_bytes(v2d_ctx *ctx, int64_t *bytes);

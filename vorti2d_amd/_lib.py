@@ -37,6 +37,7 @@ _SIGS = {
     "v2d_force_coeffs": ([i32, i32, f64, f64, f64, vp, vp, vp, vp, vp, vp, vp, f64, f64, f64, vp],
                          i32),
     "v2d_ctx_create": ([P(vp), i32, i32, vp, vp, i32, f64, f64, i32], i32),
+    "v2d_ctx_create_like": ([P(vp), vp, i32], i32),
     "v2d_ctx_destroy": ([vp], i32),
     "v2d_ctx_device_bytes": ([vp, P(i64)], i32),
     "v2d_ctx_metrics": ([vp, vp], i32),

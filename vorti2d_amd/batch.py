@@ -110,9 +110,7 @@ def solve_case(case: Case, precision="mixed", tol=1e-10, max_iter=50, forcing=0.
         if precision == "fp64":
             adj = ctx.adjoint_forces(case.re, ref_length, 1.0, 0.0)
         else:
-            psi, ome = ctx.get_state(("psi", "ome"))
-            c64 = DeviceContext(xg, yg, 0, math.cos(a), math.sin(a), "fp64")
-            c64.set_state(psi, ome)
+            c64 = ctx.like("fp64")   # mesh, state and pattern copied on the device
             adj = c64.adjoint_forces(case.re, ref_length, 1.0, 0.0)
             c64.close()
         out.adjoint_s = time.perf_counter() - t3

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/vorti2d.h"
@@ -597,8 +598,26 @@ int v2d_force_coeffs(int imax, int jmax, double re, double dksi, double deta,
 }
 
 // ------------------------------------------------------------ context -----
-int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const double *yg,
-                   int ff_bc, double ca, double sa, int precision)
+// A context on the same mesh, far-field setting and state as src: the mesh
+// and state are copied on the device and the block-LU pattern is copied, not
+// rebuilt on the host.
+static int ctx_bc_copy(Ctx *c, const Ctx *src)
+{
+    c->ff_bc = src->ff_bc; c->ca = src->ca; c->sa = src->sa;
+    c->hflags = src->hflags;
+    c->npsi_out = src->npsi_out;
+    c->nnz = src->nnz;
+    V2D_HIP(hipMemcpyAsync(c->flags, src->flags, c->imax, hipMemcpyDeviceToDevice, c->s));
+    V2D_HIP(hipMemcpyAsync(c->pre_psi, src->pre_psi, sizeof(int) * c->imax, hipMemcpyDeviceToDevice, c->s));
+    V2D_HIP(hipMemcpyAsync(c->pre_ome, src->pre_ome, sizeof(int) * c->imax, hipMemcpyDeviceToDevice, c->s));
+    c->drop_graph();
+    if (direct_copy_pattern(c->pat, src->pat, c->s)) return -1;
+    c->factored = false;
+    return 0;
+}
+
+static int ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const double *yg, int ff_bc, double ca,
+                      double sa, int precision, const Ctx *src)
 {
     *out = nullptr;
     if (check_grid(imax, jmax)) return -1;
@@ -630,15 +649,24 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     c->flags = c->alloc<uint8_t>(imax, rc);
     c->pre_psi = c->alloc<int>(imax, rc); c->pre_ome = c->alloc<int>(imax, rc);
     if (rc) return -1;
-    V2D_HIP(hipMemcpyAsync(c->x, xg, sizeof(double) * N, hipMemcpyHostToDevice, c->s));
-    V2D_HIP(hipMemcpyAsync(c->y, yg, sizeof(double) * N, hipMemcpyHostToDevice, c->s));
-    V2D_HIP(hipMemsetAsync(c->psi, 0, sizeof(double) * N, c->s));
-    V2D_HIP(hipMemsetAsync(c->ome, 0, sizeof(double) * N, c->s));
-    V2D_HIP(hipMemsetAsync(c->omeold, 0, sizeof(double) * N, c->s));
-    V2D_HIP(hipMemsetAsync(c->omeoldold, 0, sizeof(double) * N, c->s));
+    if (src) {
+        V2D_HIP(hipStreamSynchronize(src->s));   // the source's queued work (state updates) first
+        const std::pair<double *, const double *> cp[] = {{c->x, src->x}, {c->y, src->y}, {c->psi, src->psi},
+                                                          {c->ome, src->ome}, {c->omeold, src->omeold},
+                                                          {c->omeoldold, src->omeoldold}};
+        for (const auto &q : cp)
+            V2D_HIP(hipMemcpyAsync(q.first, q.second, sizeof(double) * N, hipMemcpyDeviceToDevice, c->s));
+    } else {
+        V2D_HIP(hipMemcpyAsync(c->x, xg, sizeof(double) * N, hipMemcpyHostToDevice, c->s));
+        V2D_HIP(hipMemcpyAsync(c->y, yg, sizeof(double) * N, hipMemcpyHostToDevice, c->s));
+        V2D_HIP(hipMemsetAsync(c->psi, 0, sizeof(double) * N, c->s));
+        V2D_HIP(hipMemsetAsync(c->ome, 0, sizeof(double) * N, c->s));
+        V2D_HIP(hipMemsetAsync(c->omeold, 0, sizeof(double) * N, c->s));
+        V2D_HIP(hipMemsetAsync(c->omeoldold, 0, sizeof(double) * N, c->s));
+    }
     if (v2d_compute_metrics_dev(imax, jmax, c->dksi, c->deta, c->x, c->y, c->m, c->s)) return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
-    if (ctx_bc(c, ff_bc, ca, sa)) return -1;
+    if (src ? ctx_bc_copy(c, src) : ctx_bc(c, ff_bc, ca, sa)) return -1;
     const size_t tsz = precision == V2D_FP64 ? sizeof(double) : sizeof(float);
     c->val = c->alloc<unsigned char>((int64_t)kSlots * N * tsz, rc);
     c->val_pc = c->alloc<unsigned char>((int64_t)kSlots * N * tsz, rc);
@@ -701,6 +729,23 @@ int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const do
     V2D_HIP(hipStreamSynchronize(c->s));
     *out = h.release();
     return 0;
+}
+
+int v2d_ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const double *yg,
+                   int ff_bc, double ca, double sa, int precision)
+{
+    return ctx_create(out, imax, jmax, xg, yg, ff_bc, ca, sa, precision, nullptr);
+}
+
+int v2d_ctx_create_like(v2d_ctx **out, v2d_ctx *src, int precision)
+{
+    if (out) *out = nullptr;
+    if (!out || !src || !src->c) {
+        set_error("create_like: null context");
+        return -1;
+    }
+    const Ctx *sc = src->c.get();
+    return ctx_create(out, sc->imax, sc->jmax, nullptr, nullptr, sc->ff_bc, sc->ca, sc->sa, precision, sc);
 }
 
 int v2d_ctx_destroy(v2d_ctx *ctx)

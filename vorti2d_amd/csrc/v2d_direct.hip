@@ -156,6 +156,7 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
             V2D_HIP(hipMemcpyAsync(*dent, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice, s));
         return 0;
     };
+    P.nent[0] = cD[n2]; P.nent[1] = cL[n2]; P.nent[2] = cU[n2]; P.nent[3] = cUC[n2]; P.nent[4] = cLC[n2];
     if (up(&P.d_Dptr, &P.d_D, cD, eD) || up(&P.d_Lptr, &P.d_Lr, cL, eL) ||
         up(&P.d_Uptr, &P.d_Ur, cU, eU) || up(&P.d_UCptr, &P.d_Uc, cUC, eUC) ||
         up(&P.d_LCptr, &P.d_Lc, cLC, eLC))
@@ -263,6 +264,35 @@ int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *fl
         P.sweep_ok = P.sweep_rows <= kSwThreads / 8 && ncu >= 2;
     }
     V2D_HIP(hipStreamSynchronize(s));   // the host lists above are locals
+    return 0;
+}
+
+int direct_copy_pattern(DirectPattern &D, const DirectPattern &S, hipStream_t s)
+{
+    direct_free_pattern(D, s);
+    D = S;
+    // the device lists are D's own: allocate and copy every one
+    D.d_Dptr = D.d_Lptr = D.d_Uptr = D.d_UCptr = D.d_LCptr = nullptr;
+    D.d_ixL = D.d_ixU = D.d_vxL = D.d_vxU = D.d_jxC = D.d_vxC = nullptr;
+    D.d_D = D.d_Lr = D.d_Ur = D.d_Uc = D.d_Lc = nullptr;
+    D.sweep_ok = D.pad_ok = false;
+    auto dup = [s](auto **dst, const auto *src, int64_t count) -> int {
+        const size_t bytes = sizeof(*src) * (size_t)std::max<int64_t>(count, 1);
+        V2D_HIP(hipMallocAsync((void **)dst, bytes, s));
+        if (count > 0)
+            V2D_HIP(hipMemcpyAsync(*dst, src, sizeof(*src) * (size_t)count, hipMemcpyDeviceToDevice, s));
+        return 0;
+    };
+    const int64_t n2 = 2 * S.N, tot = (int64_t)kCouple * n2;
+    if (dup(&D.d_Dptr, S.d_Dptr, n2 + 1) || dup(&D.d_Lptr, S.d_Lptr, n2 + 1) || dup(&D.d_Uptr, S.d_Uptr, n2 + 1) ||
+        dup(&D.d_UCptr, S.d_UCptr, n2 + 1) || dup(&D.d_LCptr, S.d_LCptr, n2 + 1) ||
+        dup(&D.d_D, S.d_D, S.nent[0]) || dup(&D.d_Lr, S.d_Lr, S.nent[1]) || dup(&D.d_Ur, S.d_Ur, S.nent[2]) ||
+        dup(&D.d_Uc, S.d_Uc, S.nent[3]) || dup(&D.d_Lc, S.d_Lc, S.nent[4]) ||
+        dup(&D.d_ixL, S.d_ixL, tot) || dup(&D.d_ixU, S.d_ixU, tot) || dup(&D.d_vxL, S.d_vxL, tot) ||
+        dup(&D.d_vxU, S.d_vxU, tot) || dup(&D.d_jxC, S.d_jxC, S.ncx) || dup(&D.d_vxC, S.d_vxC, S.ncx))
+        return -1;
+    D.sweep_ok = S.sweep_ok;
+    D.pad_ok = S.pad_ok;
     return 0;
 }
 

@@ -173,6 +173,7 @@ struct DirectPattern {
     int sweep_rows = 1;      // most rows of one workgroup at one level
     bool sweep_ok = false;   // closed-form level offsets verified, tables built
     int ncu = 0;             // compute units of the device (grid of the fused solve steps)
+    int64_t nent[5] = {0, 0, 0, 0, 0};   // entries of the D, L, U, UC, LC lists
 };
 
 constexpr int kCouple = 4;   // widest inter-level coupling of a row (host-checked)
@@ -180,6 +181,10 @@ constexpr int kCouple = 4;   // widest inter-level coupling of a row (host-check
 // Device lists are allocated and freed in stream order on s.
 int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags_host, hipStream_t s);
 void direct_free_pattern(DirectPattern &P, hipStream_t s);
+// D becomes a copy of the built pattern S, its device lists copied on stream s
+// (S is not changed after its build, so only S's lifetime matters: keep it
+// until s has run the copies).
+int direct_copy_pattern(DirectPattern &D, const DirectPattern &S, hipStream_t s);
 
 // Gauss-Jordan update kernels: VALU with 64- or 128-wide output tiles, or
 // MFMA 16x16x4 (FP32 / FP64 matrix cores) on 64-wide tiles.

@@ -35,6 +35,19 @@ class DeviceContext:
                                  float(ca), float(sa), self.precision))
         self._h = h
 
+    def like(self, precision: str | int = "fp64") -> "DeviceContext":
+        """A new context in `precision` on this mesh and far-field setting,
+        with this context's state (after its queued work); the mesh, state and
+        Jacobian pattern are copied on the device, not rebuilt."""
+        prec = precision if isinstance(precision, int) else PRECISIONS[precision]
+        h = C.c_void_p()
+        check(lib.v2d_ctx_create_like(C.byref(h), self._h, prec))
+        other = DeviceContext.__new__(DeviceContext)
+        other.imax, other.jmax, other.N = self.imax, self.jmax, self.N
+        other.precision = prec
+        other._h = h
+        return other
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib.v2d_ctx_destroy(self._h)
