@@ -607,7 +607,7 @@ __device__ __forceinline__ typename Mfma<T>::C mm16(const T *sa, int lda, int xr
     return acc;
 }
 
-template <typename T>
+template <typename T, bool kCoherentOut = false>
 __device__ void gj_tile(T *sa, int lda, int kb, T (&a)[16], T *out)
 {
     using M = Mfma<T>;
@@ -672,7 +672,10 @@ __device__ void gj_tile(T *sa, int lda, int kb, T (&a)[16], T *out)
     for (int q = 0; q < 16; ++q) {
         const int r = rg * 16 + q;
         a[q] = (r < kb && c < kb) ? sa[r * lda + c] : T(0);
-        out[r * NB + c] = a[q];
+        if constexpr (kCoherentOut)
+            __hip_atomic_store(out + r * NB + c, a[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            out[r * NB + c] = a[q];
     }
 }
 
@@ -1119,12 +1122,16 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
                 }
         const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
         T g[16];
-        gj_tile(sT, LT, kb2, g, J.dinv);
         // publish the inverse first: the pivot-row workgroups read only dinv;
-        // the pivot block of A and the next row panel are read by later launches
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // the pivot block of A and the next row panel are read by later
+        // launches.  The inverse goes out in agent-coherent stores and the
+        // flag follows their completion (s_waitcnt vmcnt(0)): a release fence
+        // would write back every dirty line of this die's L2, the bulk tiles'
+        // updates included.
+        gj_tile<T, true>(sT, LT, kb2, g, J.dinv);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
         __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(flags + z, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) __hip_atomic_store(flags + z, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         T *rk = J.rowk_w + (int64_t)(k1 + c) * NB + rg * 16;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
