@@ -69,6 +69,7 @@ V2D_HD inline int sweep_jobs(int s, int nlev, int M, int steps, SwJob (&jb)[2], 
 
 constexpr int kSwThreads = 1024;              // persistent sweep workgroup
 constexpr int kSwRows = kSwThreads / 64;      // rows per wavefront pass
+constexpr int kSwRb = 3;                       // right-hand-side rows per thread in one batch
 
 // ------------------------------------------------------------ pattern -----
 int direct_build_pattern(DirectPattern &P, int imax, int jmax, const uint8_t *flags, hipStream_t s)
@@ -1940,7 +1941,8 @@ struct SweepArgs {
 
 // Wait until flag[first .. first + count) >= target: the first waves poll one
 // word per lane; bounded, error word 2 on expiry (every later wait then gives
-// up at once, so a failed launch drains quickly).
+// up at once, so a failed launch drains quickly).  The data behind the words
+// is read with agent-coherent loads (cload), so no acquire fence follows.
 __device__ __forceinline__ void sweep_wait(const int *flag, int first, int count, int target, int *err)
 {
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -1958,9 +1960,24 @@ __device__ __forceinline__ void sweep_wait(const int *flag, int first, int count
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
+}
+
+// Agent-coherent store and load of the vectors the sweep workgroups exchange
+// (level right-hand sides, the chains' z and w): with them the publishing
+// side waits for its stores to complete (s_waitcnt vmcnt(0)) before its
+// progress word, and neither side needs a fence, whose release writes back
+// and whose acquire invalidates the whole L2 of the die.
+template <typename T>
+__device__ __forceinline__ void cstore(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T cload(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // An empty asm that "changes" a register-held vector: the compiler cannot
@@ -2033,46 +2050,60 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
             pre17 = R17[min(lane + 64 * wave, nv - 1)];
             if (lane + 64 * wave >= nv) pre17 = V{};
         }
-        // y part: eight lanes per row, lane k < 4 entry k of the first term,
-        // k >= 4 entry k - 4 of the second (the middle level's U term)
-        const int yr = r0 + (tid >> 3), yk = tid & 7;
-        const bool yon = yr < r1;
-        const int yrc = min(yr, n - 1);
-        const bool first_term = yk < 4;
-        const bool use = term != 0 && (first_term || term == 3);
-        const T *cft = (first_term ? ((term & 1) ? A.cfL : A.cfU) : A.cfU) + (int64_t)kCouple * vo;
-        const int *ixt = (first_term ? ((term & 1) ? A.ixL : A.ixU) : A.ixU) + (int64_t)kCouple * vo;
-        const int e = yk & 3;
-        const T ycf = cft[e * n + yrc];
-        const int yix = ixt[e * n + yrc];
-        const T yb = A.b[A.L.global(yrc, A.N, l)];
+        // The level right-hand side, all of it, formed by every workgroup
+        // itself (nothing exchanged): rows tid + 1024 k, in batches of kSwRb
+        // rows per thread.  The first batch's source indices do not depend on
+        // earlier steps and are loaded before the wait; the coefficients, b
+        // and the source entries (agent-coherent loads) after it, all issued
+        // before the first sum.  Sums in list order: the first term's entries
+        // 0..3, then the second term's (the middle level's U term).
+        const bool has_t = term != 0;
+        const T *cft = ((term & 1) ? A.cfL : A.cfU) + (int64_t)kCouple * vo;
+        const int *ixt = ((term & 1) ? A.ixL : A.ixU) + (int64_t)kCouple * vo;
+        int ixp[kSwRb][kCouple];
+#pragma unroll
+        for (int k = 0; k < kSwRb; ++k) {
+            const int r = min(tid + kSwThreads * k, n - 1);
+#pragma unroll
+            for (int e = 0; e < kCouple; ++e) ixp[k][e] = ixt[e * n + r];
+        }
         // Y. the producing chain has stored step s - 1
         if (s > 0) {
             const int first = (s == A.steps || s == A.steps + 1) ? 0 : chain * C;
             const int count = s == A.steps ? 2 * C : C;
             sweep_wait(A.flag, first, count, 2 * s, A.err);
         }
-        {
-            const T *src = phase == 2 ? A.w : A.z;
-            const double p = use ? (double)ycf * (double)src[yix] : 0.0;
-            // in list order: first term entries 0..3, then the second term's
-            double acc = p;
+        const T *src = phase == 2 ? A.w : A.z;
+        for (int kb = 0; kb * kSwRb * kSwThreads < n; ++kb) {
+            T cv[kSwRb][kCouple], sv[kSwRb][kCouple], bv[kSwRb];
 #pragma unroll
-            for (int k = 1; k < 8; ++k) {
-                const double pk = __shfl(p, (lane & ~7) + k, 64);
-                if (k < 4 || term == 3) acc += pk;
+            for (int k = 0; k < kSwRb; ++k) {
+                const int r = min(tid + kSwThreads * (k + kSwRb * kb), n - 1);
+#pragma unroll
+                for (int e = 0; e < kCouple; ++e) {
+                    if (kb > 0) ixp[k][e] = ixt[e * n + r];
+                    cv[k][e] = cft[e * n + r];
+                    sv[k][e] = cload(src + ixp[k][e]);
+                }
+                bv[k] = A.b[A.L.global(r, A.N, l)];
             }
-            if (yk == 0 && yon) so[yr - r0] = phase < 2 ? (T)((double)yb - acc) : (T)acc;
+#pragma unroll
+            for (int k = 0; k < kSwRb; ++k) {
+                const int r = tid + kSwThreads * (k + kSwRb * kb);
+                if (r >= n) break;
+                double acc = has_t ? (double)cv[k][0] * (double)sv[k][0] : 0.0;
+#pragma unroll
+                for (int e = 1; e < kCouple; ++e)
+                    if (has_t) acc += (double)cv[k][e] * (double)sv[k][e];
+                if (term == 3)
+#pragma unroll
+                    for (int e = 0; e < kCouple; ++e) {
+                        const int64_t o = (int64_t)kCouple * vo + (int64_t)e * n + r;
+                        acc += (double)A.cfU[o] * (double)cload(src + A.ixU[o]);
+                    }
+                ys[r] = phase < 2 ? (T)((double)bv[k] - acc) : (T)acc;
+            }
         }
-        __syncthreads();
-        if (wave == 0) {
-            for (int r = r0 + lane; r < r1; r += 64) A.y[vo + r] = so[r - r0];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            if (lane == 0) __hip_atomic_store(A.flag + blockIdx.x, 2 * s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // D. the chain has stored all of y_l
-        sweep_wait(A.flag, chain * C, C, 2 * s + 1, A.err);
-        for (int r = tid; r < n; r += kSwThreads) ys[r] = A.y[vo + r];
         // zero the tail the dot products read (up to the prefetch width)
         for (int r = n + tid; r < max(nv, 64 * kSweepPre) * W; r += kSwThreads) ys[r] = T(0);
         __syncthreads();
@@ -2124,20 +2155,20 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
                 if (c + 64 < nv) acc1 += dot16<T>(Rr[c + 64], ys + (c + 64) * W);
             }
             const double acc = wsum(acc0 + acc1);
-            if (lane == 0) so[r - r0] = phase == 2 ? (T)((double)A.z[vo + r] - acc) : (T)acc;
+            if (lane == 0) so[r - r0] = phase == 2 ? (T)((double)cload(A.z + vo + r) - acc) : (T)acc;
         }
         __syncthreads();
         if (wave == 0) {
             for (int r = r0 + lane; r < r1; r += 64) {
                 const T o = so[r - r0];
                 if (phase == 0) {
-                    A.z[vo + r] = o;
+                    cstore(A.z + vo + r, o);
                 } else {
-                    A.w[vo + r] = o;
-                    A.x[A.L.global(r, A.N, l)] = o;
+                    cstore(A.w + vo + r, o);
+                    A.x[A.L.global(r, A.N, l)] = o;   // read after the launch only
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the stores are complete
             if (lane == 0) __hip_atomic_store(A.flag + blockIdx.x, 2 * s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();   // so[] and ys[] are rewritten by the next step
