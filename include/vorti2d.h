@@ -137,12 +137,15 @@ int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max, double r_target);
 enum { V2D_UPDATE_VALU64 = 0, V2D_UPDATE_VALU128 = 1, V2D_UPDATE_MFMA64 = 2 };
 int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind);
 /* Kernel of the block-LU solve sweeps: two launches per level step
- * (V2D_SWEEP_LEVELS, the default), or one persistent launch for the whole
- * solve (V2D_SWEEP_PERSISTENT: one workgroup per compute unit, each owning a
- * slice of every level, synchronised through per-workgroup progress words).  The persistent launch needs all
- * its workgroups resident at once; if another persistent launch holds the GPU
- * its bounded waits expire and the solve is redone with per-level launches,
- * which the context then keeps.  V2D_SWEEP_FUSED runs every FP32 level step
+ * (V2D_SWEEP_LEVELS), or one persistent launch for the whole solve
+ * (V2D_SWEEP_PERSISTENT: one workgroup per compute unit, each owning a slice
+ * of every level and forming each level's whole right-hand side itself,
+ * synchronised through per-workgroup progress words).  Default: persistent
+ * for FP32 solves when its two chains fill the compute units, launch pairs
+ * otherwise and for FP64.  The persistent launch needs all its workgroups
+ * resident at once; if another persistent launch holds the GPU its bounded
+ * waits expire and the solve is redone with per-level launches, which the
+ * context then keeps.  V2D_SWEEP_FUSED runs every FP32 level step
  * whose right-hand side couples one neighbouring level as one launch (each
  * workgroup forms the right-hand side itself) and the other steps as launch
  * pairs; FP64 solves use launch pairs.  A tuning knob with no counterpart in

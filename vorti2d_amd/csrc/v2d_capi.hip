@@ -716,12 +716,18 @@ static int ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const
     c->wd.err = sync_words + 2;
     c->wf.flags = sync_words + 4;
     c->wf.err = sync_words + 6;
-    // Per-level sweep launch pairs by default: on a 1025^2 mesh they take
-    // 12.2 us per level step against 25 us for the persistent launch, whose
-    // every step pays two cross-die release / acquire round trips.  The fused
-    // FP32 step (11.6 us) gains nothing measurable once the apply is a graph
-    // replay; it halves the solve time when launches are issued one by one.
-    c->wd.sweep_kind = c->wf.sweep_kind = V2D_SWEEP_LEVELS;
+    // FP32 solve sweeps: the persistent launch when it fills the GPU (two
+    // chains of one workgroup per compute unit), else launch pairs.  On a
+    // 1025^2 mesh the persistent launch takes 11.0 us per level step against
+    // 12.3 us for graph-replayed launch pairs (the fused FP32 step, 11.6 us,
+    // gains nothing once the apply is a graph replay).  A smaller persistent
+    // launch is not used by default: beside other contexts' persistent
+    // launches (a batch of solves) its workgroups may not all be resident,
+    // and its bounded waits would expire.  FP64 solves, one per Newton step,
+    // use launch pairs.
+    c->wd.sweep_kind = V2D_SWEEP_LEVELS;
+    c->wf.sweep_kind = (c->pat.sweep_ok && 2 * c->pat.sweep_C >= c->pat.ncu) ? V2D_SWEEP_PERSISTENT
+                                                                           : V2D_SWEEP_LEVELS;
     V2D_HIP(hipHostMalloc((void **)&c->err_host, 2 * sizeof(int), hipHostMallocDefault));
     c->err_host[0] = c->err_host[1] = 0;
     c->wd.err_host = c->err_host;

@@ -104,10 +104,12 @@ class DeviceContext:
         check(lib.v2d_ctx_set_update_kernel(self._h, UPDATE_KERNELS.get(kind, kind)))
 
     def set_sweep_kernel(self, kind: str | int):
-        """Block-LU solve sweeps: "levels" (two launches per level step, the
-        default), "persistent" (one launch per solve) or "fused" (one launch
-        per FP32 level step whose right-hand side couples one neighbouring
-        level, launch pairs for the others and in FP64)."""
+        """Block-LU solve sweeps: "levels" (two launches per level step),
+        "persistent" (one launch per solve) or "fused" (one launch per FP32
+        level step whose right-hand side couples one neighbouring level,
+        launch pairs for the others and in FP64).  Default: "persistent" for
+        FP32 solves on meshes whose persistent launch fills the GPU (2 imax >=
+        16 x compute units, e.g. imax 1025 on 256 units), "levels" otherwise."""
         check(lib.v2d_ctx_set_sweep_kernel(self._h, SWEEP_KERNELS.get(kind, kind)))
 
     def set_graph(self, enable: bool):
