@@ -2035,12 +2035,11 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
         //    the level: every load in bounds, none branched)
         const V *Sr = reinterpret_cast<const V *>(Sl + (int64_t)min(row, n - 1) * ld);
         const int nv = (n + W - 1) / W;
+        // (vectors past the row are clamped duplicates, left out of the sums:
+        // zeroing them here would wait for the loads at once)
         V pre[kSweepPre];
 #pragma unroll
         for (int u = 0; u < kSweepPre; ++u) pre[u] = Sr[min(lane + 64 * u, nv - 1)];
-#pragma unroll
-        for (int u = 0; u < kSweepPre; ++u)
-            if (lane + 64 * u >= nv) pre[u] = V{};
         // an item of 17 rows (n = 16 C + k): its 17th row is prefetched one
         // 16-byte vector per lane by wavefronts 0..8 (u = wave)
         const bool x17 = r1 - r0 == kSwRows + 1;
@@ -2048,7 +2047,6 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
         if (x17 && wave < kSweepPre) {
             const V *R17 = reinterpret_cast<const V *>(Sl + (int64_t)(r0 + kSwRows) * ld);
             pre17 = R17[min(lane + 64 * wave, nv - 1)];
-            if (lane + 64 * wave >= nv) pre17 = V{};
         }
         // The level right-hand side, all of it, formed by every workgroup
         // itself (nothing exchanged): rows tid + 1024 k, in batches of kSwRb
@@ -2118,7 +2116,8 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
         if (x17) {
             if (wave < kSweepPre) {
                 pin(pre17);
-                x17p[wave * 64 + lane] = dot16<T>(pre17, ys + (lane + 64 * wave) * W);
+                const double d = dot16<T>(pre17, ys + (lane + 64 * wave) * W);
+                x17p[wave * 64 + lane] = lane + 64 * wave < nv ? d : 0.0;
             }
             __syncthreads();
         }
@@ -2137,8 +2136,10 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
 #pragma unroll
                 for (int u = 0; u < kSweepPre; ++u) {
                     pin(pre[u]);
-                    if (u & 1) acc1 += dot16<T>(pre[u], ys + (lane + 64 * u) * W);
-                    else acc0 += dot16<T>(pre[u], ys + (lane + 64 * u) * W);
+                    if (lane + 64 * u < nv) {
+                        if (u & 1) acc1 += dot16<T>(pre[u], ys + (lane + 64 * u) * W);
+                        else acc0 += dot16<T>(pre[u], ys + (lane + 64 * u) * W);
+                    }
                     if (u % 3 == 2) __builtin_amdgcn_sched_barrier(0);
                 }
             } else {
