@@ -2039,14 +2039,14 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
         // zeroing them here would wait for the loads at once)
         V pre[kSweepPre];
 #pragma unroll
-        for (int u = 0; u < kSweepPre; ++u) pre[u] = Sr[min(lane + 64 * u, nv - 1)];
+        for (int u = 0; u < kSweepPre; ++u) pre[u] = load_row_nt<T>(Sr + min(lane + 64 * u, nv - 1));
         // an item of 17 rows (n = 16 C + k): its 17th row is prefetched one
         // 16-byte vector per lane by wavefronts 0..8 (u = wave)
         const bool x17 = r1 - r0 == kSwRows + 1;
         V pre17 = V{};
         if (x17 && wave < kSweepPre) {
             const V *R17 = reinterpret_cast<const V *>(Sl + (int64_t)(r0 + kSwRows) * ld);
-            pre17 = R17[min(lane + 64 * wave, nv - 1)];
+            pre17 = load_row_nt<T>(R17 + min(lane + 64 * wave, nv - 1));
         }
         // The level right-hand side, all of it, formed by every workgroup
         // itself (nothing exchanged): rows tid + 1024 k, in batches of kSwRb
