@@ -1164,7 +1164,11 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
         }
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // The published inverse is read with agent-coherent 16-byte buffer loads
+    // (sc1) in place of an acquire fence, whose L2 invalidate would also drop
+    // the lines the bulk tiles on this die are using.
+    const __amdgpu_buffer_rsrc_t drs =
+        __builtin_amdgcn_make_buffer_rsrc(J.dinv, (short)0, NB * NB * (int)sizeof(T), 0x00020000);
     const int rw2 = (w >> 1) * 32, cw2 = (w & 1) * 32;
     C d[2][2];
 #pragma unroll
@@ -1180,7 +1184,10 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
             for (int u = 0; u < 4 / VW; ++u) {
-                const V x = reinterpret_cast<const V *>(J.dinv + (rw2 + bi * 16 + li) * NB + 16 * h + 4 * lk)[u];
+                const int off = ((rw2 + bi * 16 + li) * NB + 16 * h + 4 * lk + u * VW) * (int)sizeof(T);
+                const auto raw = __builtin_amdgcn_raw_buffer_load_b128(drs, off, 0, 16);   // 16: sc1
+                V x;
+                __builtin_memcpy(&x, &raw, sizeof(x));
 #pragma unroll
                 for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = reinterpret_cast<const T *>(&x)[e];
             }
