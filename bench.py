@@ -259,7 +259,7 @@ def main():
                          "(0 = only when GMRES on the stored one misses its tolerance)")
     ap.add_argument("--gmres-max", type=int, default=30)
     ap.add_argument("--gmres-rtol", type=float, default=1e-6)
-    ap.add_argument("--forcing", type=float, default=0.1,
+    ap.add_argument("--forcing", type=float, default=0.25,
                     help="mixed: Eisenstat-Walker GMRES tolerance capped at this value (0: off)")
     ap.add_argument("--no-forcing-target", dest="forcing_target", action="store_false",
                     help="mixed: no residual-target safeguard on the forcing term")
