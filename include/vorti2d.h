@@ -154,6 +154,11 @@ int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind);
  * which the fused step sums in sixteenths. */
 enum { V2D_SWEEP_LEVELS = 0, V2D_SWEEP_PERSISTENT = 1, V2D_SWEEP_FUSED = 2 };
 int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind);
+/* Bound of each wait of the persistent sweep, in polls (default 2^21, a few
+ * tenths of a second).  A small bound makes the waits expire, so the solve
+ * is redone with launch pairs, which the context then keeps: a way to
+ * exercise that path.  No counterpart in the reference. */
+int v2d_ctx_set_sweep_spin_limit(v2d_ctx *ctx, int polls);
 /* Mixed precision: run the FP32 block-LU apply of every GMRES iteration as one
  * replay of a HIP graph captured over fixed device buffers (enable != 0, the
  * default; per-level sweeps only), instead of launching its 2 (levels - 1)

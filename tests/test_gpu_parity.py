@@ -250,6 +250,29 @@ def test_persistent_sweep_equals_level_sweeps(v, C, oracle_mod, prec, imax, jmax
     assert rel <= (1e-10 if prec == "fp64" else 5e-2)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_persistent_sweep_expired_waits_fall_back(v, C, oracle_mod, prec):
+    """The persistent solve's waits are bounded.  With a bound of one poll
+    they expire, the launch gives up and the solve is redone with launch
+    pairs (which the context then keeps): the solutions equal the launch-pair
+    solve bit for bit, and a later persistent solve with the default bound
+    does too."""
+    say(f"persistent sweep fall-back {prec}")
+    ctx, m, st = _ctx_case(v, oracle_mod, C, 73, 40, "ann", prec, 2)
+    ctx.assemble(40.0, 0.0, 2.5, 0.1)
+    ci, cj, cv, b = ctx.export_coo()
+    ctx.factor()
+    ctx.set_sweep_kernel("levels")
+    x_lv = ctx.solve(b)
+    ctx.set_sweep_kernel("persistent")
+    ctx.set_sweep_spin_limit(1)
+    for _ in range(2):
+        assert np.array_equal(ctx.solve(b), x_lv)
+    ctx.set_sweep_spin_limit(1 << 21)
+    ctx.set_sweep_kernel("persistent")
+    assert np.array_equal(ctx.solve(b), x_lv)
+
+
 @pytest.mark.parametrize("imax,jmax,kind,ff", [(17, 4, "cyl", 0), (41, 23, "cyl", 1), (73, 40, "ann", 2),
                                                (1025, 8, "ann", 0)])
 def test_fused_sweep_matches_level_sweeps(v, C, oracle_mod, imax, jmax, kind, ff):

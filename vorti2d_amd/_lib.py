@@ -48,6 +48,7 @@ _SIGS = {
     "v2d_ctx_set_forcing": ([vp, f64, f64], i32),
     "v2d_ctx_set_update_kernel": ([vp, i32], i32),
     "v2d_ctx_set_sweep_kernel": ([vp, i32], i32),
+    "v2d_ctx_set_sweep_spin_limit": ([vp, i32], i32),
     "v2d_ctx_set_graph": ([vp, i32], i32),
     "v2d_ctx_profile_factor": ([vp, vp], i32),
     "v2d_ctx_assemble": ([vp, f64, f64, f64, f64], i32),

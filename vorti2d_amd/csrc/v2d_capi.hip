@@ -881,6 +881,16 @@ int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind)
     return 0;
 }
 
+int v2d_ctx_set_sweep_spin_limit(v2d_ctx *ctx, int polls)
+{
+    if (polls < 1) {
+        set_error("sweep spin limit must be >= 1");
+        return -1;
+    }
+    ctx->c->wd.sweep_spins = ctx->c->wf.sweep_spins = polls;
+    return 0;
+}
+
 int v2d_ctx_set_graph(v2d_ctx *ctx, int enable)
 {
     ctx->c->use_graph = enable != 0;

@@ -1944,13 +1944,15 @@ struct SweepArgs {
     Levels L;
     int64_t N;
     int M, steps, nsteps, C, ys_len, rows_max;
+    int spin_limit;   // bound of each wait (polls)
 };
 
 // Wait until flag[first .. first + count) >= target: the first waves poll one
 // word per lane; bounded, error word 2 on expiry (every later wait then gives
 // up at once, so a failed launch drains quickly).  The data behind the words
 // is read with agent-coherent loads (cload), so no acquire fence follows.
-__device__ __forceinline__ void sweep_wait(const int *flag, int first, int count, int target, int *err)
+__device__ __forceinline__ void sweep_wait(const int *flag, int first, int count, int target, int *err,
+                                           int spin_limit)
 {
     const int tid = threadIdx.x, wave = tid >> 6;
     if (wave * 64 < count) {
@@ -1960,7 +1962,7 @@ __device__ __forceinline__ void sweep_wait(const int *flag, int first, int count
             const bool mine =
                 i >= count || __hip_atomic_load(flag + first + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
             if (__all(mine)) break;
-            if (++spins > kSpinLimit ||
+            if (++spins > spin_limit ||
                 ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
@@ -2076,7 +2078,7 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(SweepArgs<T> A)
         if (s > 0) {
             const int first = (s == A.steps || s == A.steps + 1) ? 0 : chain * C;
             const int count = s == A.steps ? 2 * C : C;
-            sweep_wait(A.flag, first, count, 2 * s, A.err);
+            sweep_wait(A.flag, first, count, 2 * s, A.err, A.spin_limit);
         }
         const T *src = phase == 2 ? A.w : A.z;
         for (int kb = 0; kb * kSwRb * kSwThreads < n; ++kb) {
@@ -2257,6 +2259,7 @@ static int direct_solve_persistent(const DirectPattern &P, DirectWork<T> &W, con
     A.C = P.sweep_C;
     A.ys_len = sweep_ys<T>(P);
     A.rows_max = (P.sweep_rows + 1) / 2 * 2;
+    A.spin_limit = std::max(1, W.sweep_spins);
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(1024), 0, s, W.ctr, P.sweep_grid);
     hipLaunchKernelGGL(k_sweep<T>, dim3(P.sweep_grid), dim3(kSwThreads), sweep_lds<T>(P), s, A);
     V2D_LAUNCH();

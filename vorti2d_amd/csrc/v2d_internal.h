@@ -228,6 +228,7 @@ struct DirectWork {
     T *yv = nullptr;                    // level-ordered right-hand sides of the sweep steps
     int *ctr = nullptr;                 // progress word of every sweep workgroup
     int *err_host = nullptr;            // pinned copy of err, refreshed after every solve
+    int sweep_spins = 1 << 21;          // bound of each persistent-sweep wait (polls)
 };
 
 template <typename T>

@@ -112,6 +112,12 @@ class DeviceContext:
         16 x compute units, e.g. imax 1025 on 256 units), "levels" otherwise."""
         check(lib.v2d_ctx_set_sweep_kernel(self._h, SWEEP_KERNELS.get(kind, kind)))
 
+    def set_sweep_spin_limit(self, polls: int):
+        """Bound of each persistent-sweep wait, in polls (default 2**21).  A
+        small bound makes the waits expire: the solve is redone with launch
+        pairs, which the context then keeps."""
+        check(lib.v2d_ctx_set_sweep_spin_limit(self._h, int(polls)))
+
     def set_graph(self, enable: bool):
         """Mixed precision: replay the block-LU apply of each GMRES iteration
         as one captured HIP graph (the default) or launch its kernels one by
