@@ -155,10 +155,26 @@ int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind);
 enum { V2D_SWEEP_LEVELS = 0, V2D_SWEEP_PERSISTENT = 1, V2D_SWEEP_FUSED = 2 };
 int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind);
 /* Bound of each wait of the persistent sweep, in polls (default 2^21, a few
- * tenths of a second).  A small bound makes the waits expire, so the solve
- * is redone with launch pairs, which the context then keeps: a way to
- * exercise that path.  No counterpart in the reference. */
+ * tenths of a second).  When a wait expires the solve is redone with launch
+ * pairs, which the context then keeps; 0 makes every wait expire at once,
+ * without polling: a way to exercise that path.  No counterpart in the
+ * reference. */
 int v2d_ctx_set_sweep_spin_limit(v2d_ctx *ctx, int polls);
+/* The sweep kernel the context's solves use now (it changes from
+ * V2D_SWEEP_PERSISTENT to V2D_SWEEP_LEVELS after a persistent solve whose
+ * waits expired) and the number of persistent solves redone with launch pairs
+ * so far.  No counterpart in the reference. */
+int v2d_ctx_get_sweep_kernel(v2d_ctx *ctx, int *kind, int64_t *fallbacks);
+/* Mixed precision, refactor_every = 0: when GMRES on a stored FP32
+ * factorisation gives up.  V2D_REFACTOR_FIXED (the default): once the
+ * geometric projection of its convergence exceeds gmres_max_iter iterations,
+ * a function of the iterates alone, so the Newton trajectory does not depend
+ * on GPU load.  V2D_REFACTOR_MEASURED: once it exceeds (last factorisation
+ * time / mean GMRES iteration time) + the iterations a fresh factorisation
+ * needed, both measured on the context's stream (cheaper where refactoring is
+ * cheap, but load-dependent).  No counterpart in the reference. */
+enum { V2D_REFACTOR_FIXED = 0, V2D_REFACTOR_MEASURED = 1 };
+int v2d_ctx_set_refactor_policy(v2d_ctx *ctx, int policy);
 /* Mixed precision: run the FP32 block-LU apply of every GMRES iteration as one
  * replay of a HIP graph captured over fixed device buffers (enable != 0, the
  * default; per-level sweeps only), instead of launching its 2 (levels - 1)

@@ -252,11 +252,12 @@ def test_persistent_sweep_equals_level_sweeps(v, C, oracle_mod, prec, imax, jmax
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_persistent_sweep_expired_waits_fall_back(v, C, oracle_mod, prec):
-    """The persistent solve's waits are bounded.  With a bound of one poll
-    they expire, the launch gives up and the solve is redone with launch
-    pairs (which the context then keeps): the solutions equal the launch-pair
-    solve bit for bit, and a later persistent solve with the default bound
-    does too."""
+    """The persistent solve's waits are bounded.  With a bound of 0 polls
+    every wait expires at once, the launch gives up and the solve is redone
+    with launch pairs, which the context then keeps: the first solve falls
+    back (counted), the second runs launch pairs directly, both equal the
+    launch-pair solve bit for bit, and a later persistent solve with the
+    default bound does too, without a fall-back."""
     say(f"persistent sweep fall-back {prec}")
     ctx, m, st = _ctx_case(v, oracle_mod, C, 73, 40, "ann", prec, 2)
     ctx.assemble(40.0, 0.0, 2.5, 0.1)
@@ -265,12 +266,16 @@ def test_persistent_sweep_expired_waits_fall_back(v, C, oracle_mod, prec):
     ctx.set_sweep_kernel("levels")
     x_lv = ctx.solve(b)
     ctx.set_sweep_kernel("persistent")
-    ctx.set_sweep_spin_limit(1)
-    for _ in range(2):
-        assert np.array_equal(ctx.solve(b), x_lv)
+    assert ctx.sweep_kernel() == ("persistent", 0)
+    ctx.set_sweep_spin_limit(0)
+    assert np.array_equal(ctx.solve(b), x_lv)
+    assert ctx.sweep_kernel() == ("levels", 1)
+    assert np.array_equal(ctx.solve(b), x_lv)
+    assert ctx.sweep_kernel() == ("levels", 1)
     ctx.set_sweep_spin_limit(1 << 21)
     ctx.set_sweep_kernel("persistent")
     assert np.array_equal(ctx.solve(b), x_lv)
+    assert ctx.sweep_kernel() == ("persistent", 1)
 
 
 @pytest.mark.parametrize("imax,jmax,kind,ff", [(17, 4, "cyl", 0), (41, 23, "cyl", 1), (73, 40, "ann", 2),

@@ -49,6 +49,8 @@ _SIGS = {
     "v2d_ctx_set_update_kernel": ([vp, i32], i32),
     "v2d_ctx_set_sweep_kernel": ([vp, i32], i32),
     "v2d_ctx_set_sweep_spin_limit": ([vp, i32], i32),
+    "v2d_ctx_get_sweep_kernel": ([vp, P(i32), P(i64)], i32),
+    "v2d_ctx_set_refactor_policy": ([vp, i32], i32),
     "v2d_ctx_set_graph": ([vp, i32], i32),
     "v2d_ctx_profile_factor": ([vp, vp], i32),
     "v2d_ctx_assemble": ([vp, f64, f64, f64, f64], i32),

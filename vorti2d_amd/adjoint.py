@@ -62,6 +62,10 @@ def steady_solve(xg, yg, re, alpha_deg=0.0, ff_bc=0, ref_length=1.0, tol=1e-13, 
             raise RuntimeError("steady solve diverged")
         ctx.assemble(re)
         rn = ctx.rhs_norm()
+    if not (res <= tol and invdtau == 0.0):
+        ctx.close()
+        raise RuntimeError(f"steady solve: no convergence in {it} Newton steps (res {res:.3e} > "
+                           f"{tol:g}); the adjoint needs a converged state")
     f = ctx.forces(re, ref_length)
     return SteadyResult(ctx, float(f[0]), float(f[1]), it, res)
 

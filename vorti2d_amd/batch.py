@@ -3,12 +3,12 @@
 Every case owns a device context and therefore its own HIP stream; a host
 thread pool drives up to `workers` contexts at once (the ctypes calls release
 the GIL), so the small solves of a sweep overlap on the GPU instead of each
-leaving most of it idle.  A case runs exactly the kernels of a single solve.
-In FP64 its results equal the single-solve results bit for bit.  The mixed
-solver decides when to refactor from timings it measures on its own stream
-(refactorisation cost against iteration cost), and concurrency changes those
-timings, so a mixed case can take a different refactorisation path and agree
-with its single solve to the converged tolerance only.
+leaving most of it idle.  A case runs exactly the kernels of a single solve,
+and its results equal the single-solve results bit for bit in both precisions:
+the mixed solver's refactorisation decision is, by default, a function of the
+iterates alone (DeviceContext.set_refactor_policy("fixed")), so concurrency
+does not change a trajectory.  A case that misses its stopping rule within
+max_iter Newton steps raises.
 
 The reference has no batched driver; a case is the steady loop of
 vorti2d/solver.py:136-145 (stopping rule res = ||delta||_2 / (2N) <= tol) on an
@@ -80,31 +80,42 @@ class CaseResult:
     adjoint_s: float = 0.0
     dcd_dalpha: float = float("nan")
     history: list = field(default_factory=list)
+    rnorm: float = float("nan")      # FP64 ||R||_2 at the converged state
+    cm: float = float("nan")
+    psi: np.ndarray | None = None
+    ome: np.ndarray | None = None
 
 
 def solve_case(case: Case, precision="mixed", tol=1e-10, max_iter=50, forcing=0.1, ref_length=1.0,
-               adjoint=False):
+               adjoint=False, rms_tol=1e-10, keep_state=False):
     """One steady solve (and, with adjoint=True, dCd/dalpha per degree from an
-    FP64 context at the converged state)."""
+    FP64 context at the converged state).  Stopping rules: res = ||delta||_2 /
+    (2N) <= tol (vorti2d/solver.py:141-144) and, unless rms_tol is None, the
+    final FP64 residual ||R||_2 / sqrt(2N) < rms_tol; the mixed GMRES forcing
+    aims at that residual (eta_k >= 0.5 rms_tol sqrt(2N) / ||R_k||).
+    keep_state: the result also holds the converged psi, omega."""
     xg, yg = elliptic_grid(case.imax, case.jmax, case.aspect)
     a = math.radians(case.alpha_deg)
     t0 = time.perf_counter()
     ctx = DeviceContext(xg, yg, 0, math.cos(a), math.sin(a), precision)
     if precision == "mixed":
         ctx.set_solver(refactor_every=0, gmres_rtol=1e-6, gmres_max_iter=30)
-        ctx.set_forcing(forcing, tol * 2 * case.imax * case.jmax)
+        n2 = 2 * case.imax * case.jmax
+        ctx.set_forcing(forcing, (rms_tol if rms_tol is not None else tol * math.sqrt(n2))
+                        * math.sqrt(n2))
     ctx.init_state()
     t1 = time.perf_counter()
-    hist = []
-    res = 1e5
-    it = 0
-    while res > tol and it < max_iter:      # vorti2d/solver.py:141-144
-        res, _, _ = ctx.newton_step(case.re)
-        hist.append(res)
-        it += 1
+    r = ctx.steady_newton(case.re, tol=tol, rms_tol=rms_tol, max_iter=max_iter, record=True)
     t2 = time.perf_counter()
+    if not r.converged:
+        ctx.close()
+        raise RuntimeError(f"case {case}: no convergence in {r.iters} Newton steps "
+                           f"(res {r.res:.3e}, ||R||_2 {r.rnorm:.3e})")
     f = ctx.forces(case.re, ref_length)
-    out = CaseResult(case, float(f[0]), float(f[1]), it, res, t1 - t0, t2 - t1, history=hist)
+    out = CaseResult(case, float(f[0]), float(f[1]), r.iters, r.res, t1 - t0, t2 - t1,
+                     history=r.res_hist, rnorm=r.rnorm, cm=float(f[2]))
+    if keep_state:
+        out.psi, out.ome = ctx.get_state(("psi", "ome"))
     if adjoint:
         t3 = time.perf_counter()
         if precision == "fp64":

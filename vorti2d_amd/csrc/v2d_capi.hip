@@ -133,7 +133,11 @@ struct Ctx {
     double *gxd = nullptr;
     bool factored = false;
     int since_factor = 0;
-    // measured costs for the refactorisation decision (milliseconds)
+    // refactorisation decision: V2D_REFACTOR_FIXED allows gm_max iterations on a
+    // stored factorisation (a function of the iterates only, so trajectories do
+    // not depend on GPU load); V2D_REFACTOR_MEASURED weighs the measured costs
+    // below (milliseconds) instead
+    int refactor_policy = V2D_REFACTOR_FIXED;
     double fac_ms_last = 0.0, it_ms_avg = 0.0;
     int fresh_its = 0;
     // inexact Newton forcing (Eisenstat-Walker choice 2); eta_max <= 0: off
@@ -881,10 +885,29 @@ int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind)
     return 0;
 }
 
+int v2d_ctx_get_sweep_kernel(v2d_ctx *ctx, int *kind, int64_t *fallbacks)
+{
+    const Ctx *c = ctx->c.get();
+    const auto &w = c->prec == V2D_FP64 ? c->wd.sweep_kind : c->wf.sweep_kind;
+    *kind = w;
+    *fallbacks = c->prec == V2D_FP64 ? c->wd.sweep_fallbacks : c->wf.sweep_fallbacks;
+    return 0;
+}
+
+int v2d_ctx_set_refactor_policy(v2d_ctx *ctx, int policy)
+{
+    if (policy != V2D_REFACTOR_FIXED && policy != V2D_REFACTOR_MEASURED) {
+        set_error("refactor policy must be V2D_REFACTOR_FIXED or V2D_REFACTOR_MEASURED");
+        return -1;
+    }
+    ctx->c->refactor_policy = policy;
+    return 0;
+}
+
 int v2d_ctx_set_sweep_spin_limit(v2d_ctx *ctx, int polls)
 {
-    if (polls < 1) {
-        set_error("sweep spin limit must be >= 1");
+    if (polls < 0) {
+        set_error("sweep spin limit must be >= 0");
         return -1;
     }
     ctx->c->wd.sweep_spins = ctx->c->wf.sweep_spins = polls;
@@ -1174,7 +1197,8 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
             // they cost less than a refactorisation plus the iterations a
             // fresh one needs (both measured on this context).
             int budget = c->gm_max;
-            if (!need && c->fac_ms_last > 0.0 && c->it_ms_avg > 0.0)
+            if (!need && c->refactor_policy == V2D_REFACTOR_MEASURED && c->fac_ms_last > 0.0 &&
+                c->it_ms_avg > 0.0)
                 budget = std::max(3, std::min(c->gm_max, (int)(c->fac_ms_last / c->it_ms_avg) + c->fresh_its));
             int st = gmres_f32(c, need ? c->gm_cap : c->gm_max, need ? 0 : budget, &its, &rr);
             if (st < 0) return -1;

@@ -1949,13 +1949,17 @@ struct SweepArgs {
 
 // Wait until flag[first .. first + count) >= target: the first waves poll one
 // word per lane; bounded, error word 2 on expiry (every later wait then gives
-// up at once, so a failed launch drains quickly).  The data behind the words
-// is read with agent-coherent loads (cload), so no acquire fence follows.
+// up at once, so a failed launch drains quickly).  A negative spin_limit
+// expires every wait without polling (the host's way to force the fallback).
+// The data behind the words is read with agent-coherent loads (cload), so no
+// acquire fence follows.
 __device__ __forceinline__ void sweep_wait(const int *flag, int first, int count, int target, int *err,
                                            int spin_limit)
 {
     const int tid = threadIdx.x, wave = tid >> 6;
-    if (wave * 64 < count) {
+    if (wave * 64 < count && spin_limit < 0) {
+        __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (wave * 64 < count) {
         int spins = 0;
         for (;;) {
             const int i = tid;
@@ -2259,7 +2263,7 @@ static int direct_solve_persistent(const DirectPattern &P, DirectWork<T> &W, con
     A.C = P.sweep_C;
     A.ys_len = sweep_ys<T>(P);
     A.rows_max = (P.sweep_rows + 1) / 2 * 2;
-    A.spin_limit = std::max(1, W.sweep_spins);
+    A.spin_limit = W.sweep_spins > 0 ? W.sweep_spins : -1;
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(1024), 0, s, W.ctr, P.sweep_grid);
     hipLaunchKernelGGL(k_sweep<T>, dim3(P.sweep_grid), dim3(kSwThreads), sweep_lds<T>(P), s, A);
     V2D_LAUNCH();
@@ -2270,6 +2274,7 @@ static int direct_solve_persistent(const DirectPattern &P, DirectWork<T> &W, con
         V2D_LAUNCH();
         *W.err_host = 0;
         W.sweep_kind = 0;
+        ++W.sweep_fallbacks;
         return direct_solve_levels(P, W, val, b, x, s);
     }
     return 0;

@@ -229,6 +229,7 @@ struct DirectWork {
     int *ctr = nullptr;                 // progress word of every sweep workgroup
     int *err_host = nullptr;            // pinned copy of err, refreshed after every solve
     int sweep_spins = 1 << 21;          // bound of each persistent-sweep wait (polls)
+    int64_t sweep_fallbacks = 0;        // persistent solves redone with launch pairs
 };
 
 template <typename T>

@@ -7,14 +7,28 @@ uploads / downloads.
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass, field
 
 import numpy as np
 
 from ._lib import check, f64c, lib, ptr
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
+REFACTOR_POLICIES = {"fixed": 0, "measured": 1}
 UPDATE_KERNELS = {"valu64": 0, "valu128": 1, "mfma64": 2}
 SWEEP_KERNELS = {"levels": 0, "persistent": 1, "fused": 2}
+
+
+@dataclass
+class NewtonSolve:
+    """Outcome of DeviceContext.steady_newton."""
+    converged: bool
+    iters: int
+    res: float          # ||delta||_2 / (2N) of the last step (vorti2d/solver.py:134)
+    rnorm: float        # FP64 ||R||_2 at the final state
+    res_hist: list = field(default_factory=list)
+    rnorm_hist: list = field(default_factory=list)   # ||R||_2 before each step
+    lin_iters: list = field(default_factory=list)
 
 
 class DeviceContext:
@@ -113,10 +127,26 @@ class DeviceContext:
         check(lib.v2d_ctx_set_sweep_kernel(self._h, SWEEP_KERNELS.get(kind, kind)))
 
     def set_sweep_spin_limit(self, polls: int):
-        """Bound of each persistent-sweep wait, in polls (default 2**21).  A
-        small bound makes the waits expire: the solve is redone with launch
-        pairs, which the context then keeps."""
+        """Bound of each persistent-sweep wait, in polls (default 2**21).  An
+        expired wait makes the solve be redone with launch pairs, which the
+        context then keeps; 0 expires every wait at once (forces that path)."""
         check(lib.v2d_ctx_set_sweep_spin_limit(self._h, int(polls)))
+
+    def sweep_kernel(self) -> tuple[str, int]:
+        """(sweep kernel the solves use now, persistent solves redone with
+        launch pairs so far)."""
+        k, n = C.c_int(), C.c_int64()
+        check(lib.v2d_ctx_get_sweep_kernel(self._h, C.byref(k), C.byref(n)))
+        names = {v: k for k, v in SWEEP_KERNELS.items()}
+        return names[int(k.value)], int(n.value)
+
+    def set_refactor_policy(self, policy: str | int):
+        """Mixed, refactor_every = 0: "fixed" (the default) gives up on a stored
+        FP32 factorisation once GMRES projects more than gmres_max_iter
+        iterations, a function of the iterates only; "measured" weighs the
+        factorisation and iteration times measured on the context instead
+        (load-dependent trajectories)."""
+        check(lib.v2d_ctx_set_refactor_policy(self._h, REFACTOR_POLICIES.get(policy, policy)))
 
     def set_graph(self, enable: bool):
         """Mixed precision: replay the block-LU apply of each GMRES iteration
@@ -247,6 +277,41 @@ class DeviceContext:
         return dict(cd=float(cf[0]), cl=float(cf[1]), dalpha=float(gs[0]), dre=float(gs[1]),
                     gx=gx.reshape(self.jmax, self.imax).T.copy(),
                     gy=gy.reshape(self.jmax, self.imax).T.copy())
+
+    def steady_newton(self, re, tol=1e-10, rms_tol=None, max_iter=50, urot=0.0,
+                      record=False) -> "NewtonSolve":
+        """Pure Newton iterations from the current state (the steady inner loop
+        of vorti2d/solver.py:136-145): step while res = ||delta||_2 / (2N) > tol
+        and fewer than max_iter steps.  rms_tol: also require ||R||_2 / sqrt(2N)
+        < rms_tol at the final state (an FP64 assembly after the last step),
+        stepping on until it holds.  record: keep per-step histories.  The
+        result's `converged` says whether the rules were met."""
+        n2 = 2 * self.N
+        res, it = 1.0e5, 0
+        rs, rns, lin = [], [], []
+        rfinal = float("nan")
+        while it < max_iter:
+            res, rn, li = self.newton_step(re, 0.0, 0.0, urot)
+            it += 1
+            if record:
+                rs.append(res)
+                rns.append(rn)
+                lin.append(li)
+            if not np.isfinite(res):
+                break
+            if res <= tol:
+                if rms_tol is None:
+                    break
+                self.assemble(re, 0.0, 0.0, urot)
+                rfinal = self.rhs_norm()
+                if rfinal / np.sqrt(n2) < rms_tol:
+                    break
+        if np.isnan(rfinal) and np.isfinite(res):
+            self.assemble(re, 0.0, 0.0, urot)
+            rfinal = self.rhs_norm()
+        ok = bool(np.isfinite(res) and res <= tol
+                  and (rms_tol is None or rfinal / np.sqrt(n2) < rms_tol))
+        return NewtonSolve(ok, it, float(res), float(rfinal), rs, rns, lin)
 
     def bdf2_shift(self):
         check(lib.v2d_ctx_bdf2_shift(self._h))
