@@ -77,6 +77,21 @@ int v2d_force_coeffs(int imax, int jmax, double re, double dksi, double deta,
                      const double *detady, const double *xphys, const double *yphys,
                      const double *ome, double ref_length, double x0, double y0, double *out);
 
+/* The force coefficients of v2d_force_coeffs (same arguments, meaning, errors
+ * and results: the same kernel) without any device allocation.  No
+ * counterpart in the reference; it is for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  The wall
+ * rows the integral reads are staged in a fixed device array of the library,
+ * 13 * V2D_NOALLOC_MAX_IMAX + 16 doubles (1.7 MB) on every GPU it runs on,
+ * held until the process ends; imax > V2D_NOALLOC_MAX_IMAX is rejected.
+ * Calls that overlap on one GPU race on that array. */
+#define V2D_NOALLOC_MAX_IMAX 16385
+int v2d_force_coeffs_noalloc(int imax, int jmax, double re, double dksi, double deta,
+                             const double *beta, const double *gama, const double *detadx,
+                             const double *detady, const double *xphys, const double *yphys,
+                             const double *ome, double ref_length, double x0, double y0,
+                             double *out);
+
 /* ------------------------------------------------------------------------
  * Device-resident solver context.  Replaces the per-iteration host work of
  * Solver._assemble_and_solve_once (vorti2d/solver.py:117-134): the assembly

@@ -36,6 +36,8 @@ _SIGS = {
                                 vp, i32, i32, vp, i32, i32, f64, f64, vp, vp, vp, P(i32), vp], i32),
     "v2d_force_coeffs": ([i32, i32, f64, f64, f64, vp, vp, vp, vp, vp, vp, vp, f64, f64, f64, vp],
                          i32),
+    "v2d_force_coeffs_noalloc": ([i32, i32, f64, f64, f64, vp, vp, vp, vp, vp, vp, vp, f64, f64,
+                                  f64, vp], i32),
     "v2d_ctx_create": ([P(vp), i32, i32, vp, vp, i32, f64, f64, i32], i32),
     "v2d_ctx_create_like": ([P(vp), vp, i32], i32),
     "v2d_ctx_destroy": ([vp], i32),
