@@ -1,8 +1,8 @@
 """Fixed workloads for hardware-counter and kernel-trace runs under rocprofv3.
 
-    python -m vorti2d_amd.profile_kernels factor   [--size 1025] [--precision mixed]
-    python -m vorti2d_amd.profile_kernels assembly [--size 1025] [--launches 20]
-    python -m vorti2d_amd.profile_kernels copy     [--mib 256] [--launches 5]
+    python profile_kernels.py factor   [--size 1025] [--precision mixed]
+    python profile_kernels.py assembly [--size 1025] [--launches 20]
+    python profile_kernels.py copy     [--mib 256] [--launches 5]
 
 factor:   one block-LU factorisation of the steady cylinder Jacobian (initial
           state, Re 40) after one warm-up factorisation.
@@ -16,8 +16,7 @@ import argparse
 import os
 import sys
 
-if __package__ in (None, ""):   # run as a file path (rocprofv3 -- python3 .../profile_kernels.py)
-    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))   # the repository root
 
 
 def main(argv=None):

@@ -187,3 +187,17 @@ def test_airfoil_grid_layout(C):
     nh = (65 - 1) // 2
     assert np.all(g.basis[:, [0, nh, 64], 0] == 0.0)          # TE base, nose, seam
     assert np.all(g.basis[:, :, -1] == 0.0)                    # far field fixed
+
+
+def test_ignored_reference_knobs_warn():
+    """The reference's PETSc menu and KSP knobs are accepted (configurations
+    load unchanged) but not used; each one is named in a warning."""
+    from vorti2d_amd import Config
+    assert Config().ignored_reference_knobs() == []
+    msgs = Config(linsolve="gmres_jacobi", ksp_rtol=1e-8, ilu_fill=2,
+                  distributed=True).ignored_reference_knobs()
+    assert len(msgs) == 4
+    assert msgs[0].startswith("linsolve='gmres_jacobi'")
+    assert any(m.startswith("ksp_rtol=") for m in msgs)
+    assert any(m.startswith("ilu_fill=") for m in msgs)
+    assert any(m.startswith("distributed=True") for m in msgs)

@@ -116,5 +116,28 @@ class Config:
         if self.linsolve.lower() not in _LINSOLVE:
             raise ValueError(f"unknown linsolve={self.linsolve!r} (choose one of {_LINSOLVE})")
 
+    def ignored_reference_knobs(self) -> list[str]:
+        """Reference settings this device solver accepts but does not use
+        (the PETSc solver menu of vorti2d/dist_solver.py:127-229 and the DMDA
+        switch), one message each; empty for the reference defaults."""
+        out = []
+        solver = ("exact FP64 block-LU Newton steps" if self.precision.lower() == "fp64" else
+                  "FP32 block-LU preconditioned GMRES under FP64 Newton refinement")
+        if self.linsolve.lower() != "mumps":
+            out.append(f"linsolve={self.linsolve!r}: the PETSc Krylov menu is not built; the "
+                       f"device path runs {solver} (precision={self.precision!r}); GMRES with "
+                       "Jacobi or ILU preconditioning does not converge on the pure-Newton "
+                       "system (docs/parallel.rst:61-64)")
+        defaults = Config()
+        for name in ("ksp_rtol", "ksp_restart", "ilu_fill", "asm_overlap"):
+            if getattr(self, name) != getattr(defaults, name):
+                out.append(f"{name}={getattr(self, name)!r}: a PETSc KSP/PC setting, unused by "
+                           f"the device solver (its knobs are gmres_rtol, gmres_max_iter, "
+                           "gmres_forcing, refactor_every)")
+        if self.distributed:
+            out.append("distributed=True: one GPU holds the whole mesh; the DMDA slab "
+                       "decomposition is not built")
+        return out
+
     def to_dict(self) -> dict:
         return asdict(self)

@@ -23,6 +23,7 @@ from .device import DeviceContext
 from .forces import FORCE_COLUMNS, ForceCoeffs
 
 _PROFILE = bool(os.environ.get("VORTI2D_TIMING"))
+_XDMF_WARNED = False
 _METRIC_NAMES = ("jac", "alfa", "beta", "gama", "pmet", "qmet", "detadx", "detady",
                  "xphys", "yphys")
 
@@ -48,6 +49,8 @@ class Solver:
     def _setup(self, mesh):
         cfg = self.cfg
         cfg.check_linsolve()
+        for msg in cfg.ignored_reference_knobs():
+            warnings.warn(msg, stacklevel=3)
         if mesh is not None:
             xg, yg = mesh
         elif cfg.mesh_cgns:
@@ -132,7 +135,9 @@ class Solver:
         forces_path = os.path.join(cfg.out_dir, "forces.csv")
         if cfg.compute_forces and (not self.restarted or not os.path.exists(forces_path)):
             fields_io.init_force_history(cfg.out_dir, FORCE_COLUMNS)
-        if cfg.write_xdmf:
+        global _XDMF_WARNED
+        if cfg.write_xdmf and not _XDMF_WARNED:
+            _XDMF_WARNED = True   # the reference default is True: say it once per process
             warnings.warn("write_xdmf: vorti2d_amd has no XDMF/HDF5 output; "
                           "fields are written as CSV only", stacklevel=2)
 
