@@ -277,6 +277,27 @@ int v2d_ctx_forces_vjp(v2d_ctx *ctx, double re, double ref_length, double wd, do
 int v2d_ctx_adjoint_forces(v2d_ctx *ctx, double re, double urot, double ref_length, double wd,
                            double wl, double *g_x, double *g_y, double *gscal, double *coef);
 
+/* Discrete adjoint of an unsteady BDF2 run (vorti2d/solver.py:169-190: one
+ * converged level per physical time step, history shift after each).  No
+ * counterpart in the reference.  For J = sum_n (wd_n Cd_n + wl_n Cl_n) over
+ * the levels n of the run, the reverse sweep visits the levels from the last
+ * to the first; at level n,
+ *     A_n^T lambda_n = dJ_n/dU^n + C_{n+1}^T lambda_{n+1} + C_{n+2}^T lambda_{n+2},
+ * with A_n = -db_n/dU^n and C_{n+1}, C_{n+2} the dependence of the later
+ * levels' BDF2 terms on omega^n (4 inv2dt and -inv2dt on the interior omega
+ * rows), and dJ/dtheta = dJ/dtheta|explicit + sum_n lambda_n^T db_n/dtheta.
+ * begin clears the sweep (FP64 contexts only).  step: the context's state
+ * must hold level n's converged psi, omega and its history omega^(n-1),
+ * omega^(n-2) (v2d_ctx_set_state); it refactorises that level's Newton
+ * matrix, solves for lambda_n with the two later levels' lambdas it keeps, and
+ * accumulates the gradients; wd, wl: this level's objective weights (0, 0:
+ * none); coef (may be NULL) = {Cd_n, Cl_n}.  end: g_x, g_y = dJ/d(xg, yg) (N
+ * each, i fastest), gscal = {dJ/dalpha per degree, dJ/dRe}. */
+int v2d_ctx_adjoint_bdf2_begin(v2d_ctx *ctx);
+int v2d_ctx_adjoint_bdf2_step(v2d_ctx *ctx, double re, double inv2dt, double urot, double ref_length,
+                              double wd, double wl, double *coef);
+int v2d_ctx_adjoint_bdf2_end(v2d_ctx *ctx, double *g_x, double *g_y, double *gscal);
+
 #ifdef __cplusplus
 }
 #endif

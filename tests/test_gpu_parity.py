@@ -703,3 +703,25 @@ def test_unsteady_shedding_mixed_vs_fp64(v):
     assert abs(mx["st"]["amp"] - f["st"]["amp"]) < 0.02 * f["st"]["amp"]
     assert abs(mx["cd_mean"] - f["cd_mean"]) < 1e-3 * f["cd_mean"]
     assert 0.14 < f["st"]["st"] < 0.17
+
+
+def test_unsteady_c2_reference_grid_strouhal(v):
+    """C2 on the reference's own grid, generate_cylinder(181, 181): Re 100, BDF2
+    dt 0.2 to t = 80, rotation kick 0.5 until t = 2, outflow far field, mixed
+    precision.  The reference publishes St 0.1535 for Re 100
+    (docs/verification.rst:181-204, run length and grid not stated) and
+    "St ~ 0.16, mean Cd ~ 1.31" for this setting on a 131 x 131 grid to t = 50
+    (examples/cylinder_unsteady.py:9); the saturated cycle here gives St 0.1603
+    and mean Cd 1.315 (FP64 identical to the printed digits)."""
+    from lco import strouhal
+    xg, yg = v.generate_cylinder(181, 181, 0.5, 50.0)
+    with tempfile.TemporaryDirectory() as tmp:
+        say("unsteady C2 on generate_cylinder(181, 181), mixed")
+        _, d = _unsteady_c2(v, "mixed", xg, yg, tmp, t_end=80.0)
+    st = strouhal(d["t"], d["cl"])
+    cd_mean = float(d["cd"][st["i0"]:].mean())
+    say(f"  St {st['st']:.4f} ({st['n_periods']} periods from t={d['t'][st['i0']]:.1f}), "
+        f"mean Cd {cd_mean:.4f}, Cl amplitude {st['amp']:.4f}")
+    assert st["n_periods"] >= 4
+    assert abs(st["st"] - 0.1535) <= 0.05 * 0.1535
+    assert abs(cd_mean - 1.31) <= 0.02 * 1.31

@@ -20,6 +20,7 @@ from .mesh import generate_annulus, generate_cylinder, load_cgns_ogrid, load_mes
 from .solver import Solver, run  # noqa: E402
 from .airfoil import AirfoilGrid  # noqa: E402
 from .adjoint import SteadyForces, force_gradients, metrics_vjp, steady_solve  # noqa: E402
+from .adjoint_unsteady import BDF2Run, unsteady_forces, unsteady_forces_gradient  # noqa: E402
 
 __all__ = [
     "Config", "Solver", "run", "generate_cylinder", "generate_annulus", "load_mesh",

@@ -16,6 +16,23 @@
 
 namespace v2d {
 
+// --------------------------------------------------- BDF2 time coupling ---
+// Right-hand side of the adjoint at time level n of an unsteady run: level
+// n + 1 reads omega^n through -(3 w - 4 w^n + w^(n-1)) inv2dt and level n + 2
+// through + ... w^n inv2dt, on the interior omega rows only (wall and
+// far-field rows carry no BDF2 term; vorti2d_core.f90:340-379), so
+//     g[N + k] += inv2dt (4 lambda_{n+1}[N + k] - lambda_{n+2}[N + k]).
+__global__ void k_bdf2_adjoint_rhs(int imax, int jmax, double inv2dt, const double *__restrict__ l1,
+                                   const double *__restrict__ l2, double *__restrict__ g)
+{
+    const int64_t N = (int64_t)imax * jmax;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int64_t j = k / imax;
+    if (j == 0 || j == jmax - 1) return;
+    g[N + k] += inv2dt * (4.0 * l1[N + k] - l2[N + k]);
+}
+
 // ------------------------------------------------------------ forces ------
 // J = wd Cd + wl Cl with Cd = 2 Fx / d, Cl = 2 Fy / d (body axes, fixed d):
 //     J = s * sum_i [ (wd y_i - wl x_i) dwdn_i ds_i - w_i (wd ny_i - wl nx_i) ds_i ],

@@ -131,6 +131,13 @@ struct Ctx {
         ag = nullptr;
     }
     double *gxd = nullptr;
+    // reverse sweep of the unsteady (BDF2) adjoint: the adjoint vectors of the
+    // two later time levels, the current level's, its right-hand side, the
+    // accumulated metric-plane gradient and (dJ/dRe, dJ/dalpha [rad]) sums;
+    // allocated at the first v2d_ctx_adjoint_bdf2_begin
+    double *adj_l1 = nullptr, *adj_l2 = nullptr, *adj_lam = nullptr, *adj_rhs = nullptr, *adj_gm = nullptr;
+    double adj_gs[2] = {0.0, 0.0};
+    int64_t adj_levels = -1;   // levels swept so far; -1: no sweep begun
     bool factored = false;
     int since_factor = 0;
     // refactorisation decision: V2D_REFACTOR_FIXED allows gm_max iterations on a
@@ -1486,6 +1493,97 @@ int v2d_ctx_forces_vjp(v2d_ctx *ctx, double re, double ref_length, double wd, do
     V2D_HIP(hipMemcpyAsync(g_ome, go.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
     V2D_HIP(hipMemcpyAsync(g_m, gm.p, sizeof(double) * 10 * N, hipMemcpyDeviceToHost, c->s));
     V2D_HIP(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int v2d_ctx_adjoint_bdf2_begin(v2d_ctx *ctx)
+{
+    Ctx *c = ctx->c.get();
+    if (c->prec != V2D_FP64) {
+        set_error("unsteady adjoint: needs an FP64 context (exact transposed solves)");
+        return -1;
+    }
+    const int64_t N = c->N;
+    if (!c->adj_l1) {
+        int rc = 0;
+        c->adj_l1 = c->alloc<double>(2 * N, rc);
+        c->adj_l2 = c->alloc<double>(2 * N, rc);
+        c->adj_lam = c->alloc<double>(2 * N, rc);
+        c->adj_rhs = c->alloc<double>(2 * N, rc);
+        c->adj_gm = c->alloc<double>(10 * N, rc);
+        if (rc) return -1;
+    }
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(2 * N)), dim3(256), 0, c->s, 2 * N, c->adj_l1);
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(2 * N)), dim3(256), 0, c->s, 2 * N, c->adj_l2);
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(10 * N)), dim3(256), 0, c->s, 10 * N, c->adj_gm);
+    V2D_LAUNCH();
+    c->adj_gs[0] = c->adj_gs[1] = 0.0;
+    c->adj_levels = 0;
+    return 0;
+}
+
+int v2d_ctx_adjoint_bdf2_step(v2d_ctx *ctx, double re, double inv2dt, double urot, double ref_length,
+                              double wd, double wl, double *coef)
+{
+    Ctx *c = ctx->c.get();
+    if (c->adj_levels < 0) {
+        set_error("unsteady adjoint: call v2d_ctx_adjoint_bdf2_begin first");
+        return -1;
+    }
+    const int64_t N = c->N;
+    // Newton matrix of this level's converged residual (no pseudo-time shift)
+    if (ctx_assemble(c, re, 0.0, inv2dt, urot)) return -1;
+    if (ctx_factor(c)) return -1;
+    hipLaunchKernelGGL(k_fill_zero_d, dim3(grid_for(2 * N)), dim3(256), 0, c->s, 2 * N, c->adj_rhs);
+    V2D_LAUNCH();
+    double f3[3] = {0.0, 0.0, 0.0};
+    if (wd != 0.0 || wl != 0.0 || coef) {
+        // dJ_n/domega in the omega half; explicit dJ_n/dm and dJ_n/dRe
+        if (forces_vjp_dev(c, re, ref_length, wd, wl, c->adj_rhs + N, c->adj_gm, f3)) return -1;
+        c->adj_gs[0] += f3[2];
+    }
+    if (c->adj_levels > 0 && inv2dt != 0.0) {
+        hipLaunchKernelGGL(k_bdf2_adjoint_rhs, dim3(grid_for(N)), dim3(256), 0, c->s, c->imax, c->jmax, inv2dt,
+                           c->adj_l1, c->adj_l2, c->adj_rhs);
+        V2D_LAUNCH();
+    }
+    if (direct_solve_t<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->adj_rhs, c->adj_lam,
+                               c->s))
+        return -1;
+    double rs[2];
+    if (residual_vjp_dev(c, re, inv2dt, urot, c->adj_lam, c->adj_gm, rs)) return -1;
+    c->adj_gs[0] += rs[0];
+    c->adj_gs[1] += rs[1];
+    std::swap(c->adj_l2, c->adj_l1);    // lambda_{n+2} <- lambda_{n+1}
+    std::swap(c->adj_l1, c->adj_lam);   // lambda_{n+1} <- lambda_n
+    ++c->adj_levels;
+    if (coef) {
+        coef[0] = f3[0];
+        coef[1] = f3[1];
+    }
+    return 0;
+}
+
+int v2d_ctx_adjoint_bdf2_end(v2d_ctx *ctx, double *g_x, double *g_y, double *gscal)
+{
+    Ctx *c = ctx->c.get();
+    if (c->adj_levels < 0) {
+        set_error("unsteady adjoint: no sweep begun");
+        return -1;
+    }
+    const int64_t N = c->N;
+    DBuf gx, gy;
+    V2D_ALLOC_S(gx, sizeof(double) * N, c->s);
+    V2D_ALLOC_S(gy, sizeof(double) * N, c->s);
+    if (metrics_vjp_dev(c->imax, c->jmax, c->dksi, c->deta, c->x, c->y, c->adj_gm, gx.as<double>(),
+                        gy.as<double>(), c->s))
+        return -1;
+    V2D_HIP(hipMemcpyAsync(g_x, gx.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipMemcpyAsync(g_y, gy.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
+    gscal[0] = c->adj_gs[1] * (M_PI / 180.0);   // per degree of alpha
+    gscal[1] = c->adj_gs[0];                    // per unit Re
+    c->adj_levels = -1;
     return 0;
 }
 

@@ -245,6 +245,8 @@ int direct_solve_t(const DirectPattern &P, const DirectWork<T> &W, const T *val,
 __global__ void k_forces_vjp(int imax, double re, double dksi, double deta, const double *m, int64_t N,
                              const double *ome, double d, double wd, double wl, double *g_ome,
                              double *g_m, double *out);
+__global__ void k_bdf2_adjoint_rhs(int imax, int jmax, double inv2dt, const double *l1, const double *l2,
+                                   double *g);
 __global__ void k_residual_vjp(AsmArgs a, const uint8_t *flags, const double *lam, double *g_m,
                                double *partial);
 __global__ void k_metrics_vjp_1(int imax, int jmax, double dksi, double deta, const double *x,

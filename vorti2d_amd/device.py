@@ -313,6 +313,28 @@ class DeviceContext:
                   and (rms_tol is None or rfinal / np.sqrt(n2) < rms_tol))
         return NewtonSolve(ok, it, float(res), float(rfinal), rs, rns, lin)
 
+    def adjoint_bdf2_begin(self):
+        """Start the reverse sweep of an unsteady adjoint (FP64 context)."""
+        check(lib.v2d_ctx_adjoint_bdf2_begin(self._h))
+
+    def adjoint_bdf2_step(self, re, inv2dt, urot, ref_length, wd=0.0, wl=0.0):
+        """One level of the reverse sweep at the current state (the level's
+        converged psi, omega and its history); returns (Cd_n, Cl_n)."""
+        cf = np.zeros(2)
+        check(lib.v2d_ctx_adjoint_bdf2_step(self._h, float(re), float(inv2dt), float(urot),
+                                            float(ref_length), float(wd), float(wl), ptr(cf)))
+        return float(cf[0]), float(cf[1])
+
+    def adjoint_bdf2_end(self):
+        """Finish the sweep: dict(dalpha (per degree), dre, gx, gy (imax x jmax))."""
+        gx = np.empty(self.N)
+        gy = np.empty(self.N)
+        gs = np.zeros(2)
+        check(lib.v2d_ctx_adjoint_bdf2_end(self._h, ptr(gx), ptr(gy), ptr(gs)))
+        return dict(dalpha=float(gs[0]), dre=float(gs[1]),
+                    gx=gx.reshape(self.jmax, self.imax).T.copy(),
+                    gy=gy.reshape(self.jmax, self.imax).T.copy())
+
     def bdf2_shift(self):
         check(lib.v2d_ctx_bdf2_shift(self._h))
 
