@@ -202,18 +202,29 @@ def sweep_timing(workers=8, n=64):
     1e-10 plus the adjoint dCd/dalpha, up to `workers` contexts (streams) in
     flight.  One untimed pass
     of the first `workers` cases warms the allocator and the code objects."""
-    from vorti2d_amd.batch import solve_batch, sweep_cases
+    from vorti2d_amd.batch import solve_batch, solve_batch_lockstep, sweep_cases
     cases = sweep_cases(n, seed=0)
     solve_batch(cases[:workers], workers=workers, adjoint=True)
     res, st = solve_batch(cases, workers=workers, adjoint=True)
     log(f"[sweep] {n} elliptic O-grids, {workers} streams: {st['solves_per_s']:.2f} solves/s, "
         f"p50 {st['p50_s'] * 1e3:.0f} ms, p99 {st['p99_s'] * 1e3:.0f} ms")
+    lock = {}
+    for slots in (8, 16):
+        rl, sl = solve_batch_lockstep(cases, inflight=slots, adjoint=True)
+        same = all(a.cd == b.cd and a.cl == b.cl and a.newton_iters == b.newton_iters
+                   for a, b in zip(res, rl))
+        lock[str(slots)] = {"solves_per_s": sl["solves_per_s"], "p50_s": sl["p50_s"], "p99_s": sl["p99_s"],
+                            "workgroups": sl["blocks"], "equal_to_stream_engine": same}
+        log(f"[sweep lockstep] {slots} slots: {sl['solves_per_s']:.2f} solves/s, p50 "
+            f"{sl['p50_s'] * 1e3:.0f} ms, p99 {sl['p99_s'] * 1e3:.0f} ms, workgroups {sl['blocks']}, "
+            f"results equal to the stream engine's: {same}")
     return {"workload": f"{n} elliptic O-grids (default_rng(0): angular 129..257, radial 129..385, "
                         "aspect 0.5..1, Re 20..45, alpha -10..10 deg), mixed steady solve to "
                         "res <= 1e-10 and ||R||_2/sqrt(2N) < 1e-10 + adjoint dCd/dalpha per case",
             "solves_per_s": st["solves_per_s"], "wall_s": st["wall_s"], "p50_s": st["p50_s"],
             "p99_s": st["p99_s"], "streams": workers,
             "newton_iters_mean": sum(r.newton_iters for r in res) / len(res),
+            "lockstep": lock,
             "max_res": max(r.res for r in res),
             "max_rnorm_rms": max(r.rnorm / math.sqrt(2 * r.case.imax * r.case.jmax) for r in res)}
 

@@ -277,6 +277,44 @@ int v2d_ctx_forces_vjp(v2d_ctx *ctx, double re, double ref_length, double wd, do
 int v2d_ctx_adjoint_forces(v2d_ctx *ctx, double re, double urot, double ref_length, double wd,
                            double wl, double *g_x, double *g_y, double *gscal, double *coef);
 
+/* ---------------------------------------------------------------------
+ * Lockstep batch of contexts of uneven mesh sizes (no counterpart in the
+ * reference, whose steady loop vorti2d/solver.py:136-145 runs one mesh).  A
+ * batch has up to V2D_BATCH_MAX slots of one precision; the caller owns the
+ * contexts and keeps them alive while they sit in a slot.  One batched Newton
+ * step of the active filled slots runs one assembly launch, one residual-norm
+ * launch pair, per case the factorisation (when due) and the preconditioner
+ * applies on the case's own stream, mixed precision the GMRES iterations in
+ * lockstep with one SpMV, two multi-dot / reduce / multi-axpy rounds, one norm
+ * round and one normalisation launch per iteration for all iterating cases,
+ * and one update launch: every case's iterates equal those of
+ * v2d_ctx_newton_step on its own context bit for bit.  Mixed-precision cases
+ * need the fixed refactor policy; a case with the persistent sweep kernel is
+ * switched to launch pairs (same results).
+ * --------------------------------------------------------------------- */
+#define V2D_BATCH_MAX 16
+typedef struct v2d_batch v2d_batch;
+int v2d_batch_create(v2d_batch **out, int precision, int slots);
+int v2d_batch_destroy(v2d_batch *b);
+/* Put ctx into a slot (NULL empties it). */
+int v2d_batch_set_case(v2d_batch *b, int slot, v2d_ctx *ctx);
+/* One Newton step of every filled slot with active[slot] != 0 (active NULL:
+ * all).  re, invdtau, inv2dt, urot: per-slot scalars of v2d_ctx_newton_step
+ * (the last three may be NULL for zeros); res, rnorm, lin_iters: per-slot
+ * outputs as there. */
+int v2d_batch_newton_step(v2d_batch *b, const int *active, const double *re, const double *invdtau,
+                          const double *inv2dt, const double *urot, double *res, double *rnorm,
+                          int *lin_iters);
+/* Workgroup sizes (threads) of the batched pointwise launches: assembly, SpMV,
+ * vector updates.  Results do not depend on them.  autotune times 128, 256,
+ * 512 and 1024 threads on the filled slots (re: the assembly's Reynolds
+ * number; the slots' Jacobians are reassembled at their current states),
+ * keeps the fastest of each and remembers them for batches of the same size
+ * class (total nodes rounded up to a power of two). */
+int v2d_batch_get_blocks(v2d_batch *b, int *threads3);
+int v2d_batch_set_blocks(v2d_batch *b, const int *threads3);
+int v2d_batch_autotune(v2d_batch *b, double re, int *threads3);
+
 /* Discrete adjoint of an unsteady BDF2 run (vorti2d/solver.py:169-190: one
  * converged level per physical time step, history shift after each).  No
  * counterpart in the reference.  For J = sum_n (wd_n Cd_n + wl_n Cl_n) over

@@ -74,6 +74,13 @@ _SIGS = {
     "v2d_metrics_vjp": ([i32, i32, f64, f64, vp, vp, vp, vp, vp], i32),
     "v2d_ctx_forces_vjp": ([vp, f64, f64, f64, f64, vp, vp, vp], i32),
     "v2d_ctx_adjoint_forces": ([vp, f64, f64, f64, f64, f64, vp, vp, vp, vp], i32),
+    "v2d_batch_create": ([P(vp), i32, i32], i32),
+    "v2d_batch_destroy": ([vp], i32),
+    "v2d_batch_set_case": ([vp, i32, vp], i32),
+    "v2d_batch_newton_step": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+    "v2d_batch_get_blocks": ([vp, vp], i32),
+    "v2d_batch_set_blocks": ([vp, vp], i32),
+    "v2d_batch_autotune": ([vp, f64, vp], i32),
     "v2d_ctx_adjoint_bdf2_begin": ([vp], i32),
     "v2d_ctx_adjoint_bdf2_step": ([vp, f64, f64, f64, f64, f64, f64, vp], i32),
     "v2d_ctx_adjoint_bdf2_end": ([vp, vp, vp, vp], i32),

@@ -12,32 +12,12 @@
 
 #include "../../include/vorti2d.h"
 #include "v2d_internal.h"
+#include "v2d_ctx.h"
 
 namespace v2d {
 
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
-
-// RAII device buffer: plain for the stateless host entry points (null
-// stream), stream-ordered (hipMallocAsync / hipFreeAsync) on a context's
-// stream, so that a context never synchronises the whole device.
-struct DBuf {
-    void *p = nullptr;
-    hipStream_t s = nullptr;
-    ~DBuf()
-    {
-        if (!p) return;
-        if (s) (void)hipFreeAsync(p, s);
-        else (void)hipFree(p);
-    }
-    template <typename T> T *as() { return static_cast<T *>(p); }
-};
-#define V2D_ALLOC(buf, bytes) V2D_HIP(hipMalloc(&(buf).p, std::max<size_t>((bytes), 8)))
-#define V2D_ALLOC_S(buf, bytes, st)                                                          \
-    do {                                                                                     \
-        (buf).s = (st);                                                                      \
-        V2D_HIP(hipMallocAsync(&(buf).p, std::max<size_t>((bytes), 8), (st)));               \
-    } while (0)
 
 static int check_grid(int imax, int jmax)
 {
@@ -90,113 +70,6 @@ static int64_t count_rows(int imax, int jmax, int64_t r0, int64_t r1, const std:
     return c;
 }
 
-// ----------------------------------------------------------- context -----
-struct Ctx {
-    int imax = 0, jmax = 0, prec = V2D_FP64, ff_bc = 0;
-    int64_t N = 0;
-    double dksi = 0, deta = 0, ca = 1, sa = 0;
-    hipStream_t s = nullptr;
-    int64_t bytes = 0;
-    double *x = nullptr, *y = nullptr, *m = nullptr;
-    double *psi = nullptr, *ome = nullptr, *omeold = nullptr, *omeoldold = nullptr;
-    double *rhs = nullptr, *delta = nullptr, *red = nullptr, *partial = nullptr, *hbuf = nullptr;
-    uint8_t *flags = nullptr;
-    int *pre_psi = nullptr, *pre_ome = nullptr;
-    int npsi_out = 0;
-    int64_t nnz = 0;
-    std::vector<uint8_t> hflags;
-    void *val = nullptr;      // kSlots * N of T: the last assembled Jacobian
-    void *val_pc = nullptr;   // the values the stored block LU was built from
-    DirectPattern pat;
-    DirectWork<double> wd;
-    DirectWork<float> wf;
-    // GMRES (mixed) work, FP32 vectors
-    int gm_cap = 30;   // allocated Krylov vectors
-    int gm_max = 30;
-    double gm_rtol = 1e-6;
-    int refactor_every = 1;
-    float *gV = nullptr, *gZ = nullptr, *gb = nullptr;
-    // the FP32 block-LU apply as a captured graph over fixed buffers
-    // (ga_in -> ga_out), replayed by GMRES; built on first use, dropped when
-    // the pattern or the sweep kernel changes
-    bool use_graph = true;
-    float *ga_in = nullptr, *ga_out = nullptr;
-    hipGraph_t ag = nullptr;
-    hipGraphExec_t age = nullptr;
-    void drop_graph()
-    {
-        if (age) (void)hipGraphExecDestroy(age);
-        if (ag) (void)hipGraphDestroy(ag);
-        age = nullptr;
-        ag = nullptr;
-    }
-    double *gxd = nullptr;
-    // reverse sweep of the unsteady (BDF2) adjoint: the adjoint vectors of the
-    // two later time levels, the current level's, its right-hand side, the
-    // accumulated metric-plane gradient and (dJ/dRe, dJ/dalpha [rad]) sums;
-    // allocated at the first v2d_ctx_adjoint_bdf2_begin
-    double *adj_l1 = nullptr, *adj_l2 = nullptr, *adj_lam = nullptr, *adj_rhs = nullptr, *adj_gm = nullptr;
-    double adj_gs[2] = {0.0, 0.0};
-    int64_t adj_levels = -1;   // levels swept so far; -1: no sweep begun
-    bool factored = false;
-    int since_factor = 0;
-    // refactorisation decision: V2D_REFACTOR_FIXED allows gm_max iterations on a
-    // stored factorisation (a function of the iterates only, so trajectories do
-    // not depend on GPU load); V2D_REFACTOR_MEASURED weighs the measured costs
-    // below (milliseconds) instead
-    int refactor_policy = V2D_REFACTOR_FIXED;
-    double fac_ms_last = 0.0, it_ms_avg = 0.0;
-    int fresh_its = 0;
-    // inexact Newton forcing (Eisenstat-Walker choice 2); eta_max <= 0: off
-    double eta_max = 0.0, eta_prev = 0.0, rn_prev = 0.0;
-    double r_target = 0.0;     // residual norm the solve aims at (forcing safeguard; 0: off)
-    double rtol_step = 1e-6;   // GMRES tolerance of the current Newton step
-    double last_re = 0, last_invdtau = 0, last_inv2dt = 0, last_urot = 0;
-    // timers: ev[0..3] bracket a Newton step (start, assembled, preconditioner
-    // ready, solved); ev[4..5] bracket the factorisation made inside it, if any
-    hipEvent_t ev[6];
-    bool fac_in_step = false;
-    double t_asm = 0, t_fac = 0, t_sol = 0, t_tot = 0;
-    int64_t n_steps = 0, n_fac = 0, n_lin = 0;
-    std::vector<void *> allocs;
-
-    template <typename T> T *alloc(int64_t count, int &rc)
-    {
-        void *p = nullptr;
-        if (hipMallocAsync(&p, sizeof(T) * std::max<int64_t>(count, 1), s) != hipSuccess) {
-            set_error("hipMallocAsync of " + std::to_string(sizeof(T) * count) + " bytes failed");
-            rc = -1;
-            return nullptr;
-        }
-        allocs.push_back(p);
-        bytes += sizeof(T) * std::max<int64_t>(count, 1);
-        return static_cast<T *>(p);
-    }
-    int *err_host = nullptr;   // pinned: the solve error words of wd, wf
-    ~Ctx()
-    {
-        drop_graph();
-        for (void *p : allocs) (void)hipFreeAsync(p, s);
-        direct_free_pattern(pat, s);
-        if (s) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
-        if (err_host) (void)hipHostFree(err_host);
-        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
-    }
-    AsmArgs args(double re, double invdtau, double inv2dt, double urot) const
-    {
-        AsmArgs a;
-        a.imax = imax; a.jmax = jmax; a.ff_bc = ff_bc;
-        a.re = re; a.invdtau = invdtau; a.inv2dt = inv2dt; a.urot = urot;
-        a.dksi = dksi; a.deta = deta; a.ca = ca; a.sa = sa;
-        for (int q = 0; q < 10; ++q) a.m[q] = m + q * N;
-        a.psi = psi; a.ome = ome; a.omeold = omeold; a.omeoldold = omeoldold;
-        return a;
-    }
-};
-
 static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
 {
     c->ff_bc = ff_bc; c->ca = ca; c->sa = sa;
@@ -222,7 +95,7 @@ static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
 // A persistent solve sweep that waited past its spin limit leaves 2 in the
 // error word; direct_solve copies the word to pinned memory after every solve,
 // so it is current after the caller's next synchronisation.
-static int sweep_failed(const int *err_host)
+int sweep_failed(const int *err_host)
 {
     if (err_host && *err_host) {
         set_error("block-LU solve: a sweep step waited past its spin limit (results invalid)");
@@ -236,7 +109,7 @@ static int sweep_failed(const int *err_host)
 // (captured on first use; its kernels read the factorisation through fixed
 // pointers, so it stays valid across refactorisations) between two
 // device-to-device copies; otherwise the kernels are launched directly.
-static int apply_pc(Ctx *c, const float *v, float *z)
+int apply_pc(Ctx *c, const float *v, float *z)
 {
     const float *vpc = static_cast<const float *>(c->val_pc);
     const int64_t n = 2 * c->N;
@@ -262,26 +135,116 @@ static int apply_pc(Ctx *c, const float *v, float *z)
 }
 
 // ------------------------------------------------------------- GMRES ------
+void GmresHost::begin(int cap, int budget_)
+{
+    m = cap;
+    budget = budget_;
+    j = 0;
+    H.assign((size_t)(m + 1) * m, 0.0);
+    cs.assign(m, 0.0);
+    sn.assign(m, 0.0);
+    g.assign(m + 1, 0.0);
+    g[0] = 1.0;
+    h.assign(m + 1, 0.0);
+    h2.assign(m + 1, 0.0);
+    hist.assign(m, 1.0);
+    est = 1.0;
+    hv = 1.0;
+}
+
+int GmresHost::advance(double wn2, double rtol, int *iters, double *relres)
+{
+    const double hn = std::sqrt(std::max(wn2, 0.0));
+    if (!std::isfinite(hn)) {   // overflow in the FP32 vectors: not converged
+        *iters = j + 1;
+        *relres = INFINITY;
+        return 1;
+    }
+    for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
+    H[(j + 1) * m + j] = hn;
+    hv = hn > 0.0 ? hn : 1.0;   // v_{j+1} = w / hv
+    for (int i = 0; i < j; ++i) {
+        const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+        H[(i + 1) * m + j] = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+        H[i * m + j] = t;
+    }
+    const double d = std::hypot(H[j * m + j], H[(j + 1) * m + j]);
+    cs[j] = d > 0.0 ? H[j * m + j] / d : 1.0;
+    sn[j] = d > 0.0 ? H[(j + 1) * m + j] / d : 0.0;
+    H[j * m + j] = d;
+    H[(j + 1) * m + j] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    est = std::fabs(g[j + 1]);
+    hist[j] = est;
+    if (!std::isfinite(est) || !(d > 0.0)) {
+        *iters = j + 1;
+        *relres = INFINITY;
+        return 1;
+    }
+    if (est <= rtol || hn == 0.0) {
+        ++j;
+        *iters = j;
+        *relres = est;
+        return 0;
+    }
+    if (budget > 0 && j >= 2) {
+        // geometric projection of the iterations this preconditioner needs,
+        // from the average rate so far, and of the iterations still needed
+        // from the rate of the last two (a stale factorisation often gives
+        // one large first reduction and then stalls, which only the recent
+        // rate shows)
+        const double rate = std::pow(est, 1.0 / (j + 1));
+        const double need = rate < 1.0 ? std::log(rtol) / std::log(rate) : INFINITY;
+        double rest = 0.0;
+        if (j >= 3) {
+            const double rr = std::sqrt(est / hist[j - 2]);
+            rest = rr < 1.0 ? std::log(rtol / est) / std::log(rr) : INFINITY;
+        }
+        if (need > budget || rest > budget) {
+            *iters = j + 1;
+            *relres = est;
+            return 2;
+        }
+    }
+    ++j;
+    if (j >= m) {   // basis full
+        *iters = m;
+        *relres = est;
+        return 0;
+    }
+    return 3;
+}
+
+std::vector<double> GmresHost::solution() const
+{
+    const int k = std::min(j, m);
+    std::vector<double> yv(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+        double acc = g[i];
+        for (int q = i + 1; q < k; ++q) acc -= H[i * m + q] * yv[q];
+        yv[i] = acc / H[i * m + i];
+    }
+    return yv;
+}
+
 // Right-preconditioned GMRES with the block LU M, classical Gram-Schmidt with
 // one re-orthogonalisation (fused multi-dot / multi-axpy), Givens rotations on
-// the host.  Solves A x = b for unit-norm b (FP32 vectors, FP64 dot products);
-// Z_j = M V_j is kept so the solution is x = Z y (flexible form).
+// the host (GmresHost).  Solves A x = b for unit-norm b (FP32 vectors, FP64 dot
+// products); Z_j = M V_j is kept so the solution is x = Z y (flexible form).
 // budget > 0 (a stored, older factorisation): give up (return 2) once the
 // convergence rate seen so far projects more than `budget` iterations.
 static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
 {
     const int64_t n = 2 * c->N;
-    const int m = cap;
     const float *val = static_cast<const float *>(c->val);        // operator (the preconditioner: apply_pc)
     const unsigned nb = std::min<unsigned>(1024, grid_for(n));
-    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1), hist(m, 1.0);
+    GmresHost G;
+    G.begin(cap, budget);
     // v0 = b (unit norm already) -> gV[0]
     V2D_HIP(hipMemcpyAsync(c->gV, c->gb, sizeof(float) * n, hipMemcpyDeviceToDevice, c->s));
-    g[0] = 1.0;
-    int j = 0;
-    double est = 1.0;
-    double hv = 1.0;   // host source of the async normalisation-scalar copies
-    for (; j < m; ++j) {
+    for (;;) {
+        const int j = G.j;
         float *vj = c->gV + j * n, *zj = c->gZ + j * n, *w = c->gV + (j + 1) * n;
         if (apply_pc(c, vj, zj)) return -1;
         hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
@@ -295,7 +258,7 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
             hipLaunchKernelGGL(k_multiaxpy<float>, dim3(grid_for(n)), dim3(256), 0, c->s, n, j + 1,
                                c->gV, n, c->hbuf, w);
             V2D_LAUNCH();
-            V2D_HIP(hipMemcpyAsync(pass ? h2.data() : h.data(), c->hbuf, sizeof(double) * (j + 1),
+            V2D_HIP(hipMemcpyAsync(pass ? G.h2.data() : G.h.data(), c->hbuf, sizeof(double) * (j + 1),
                                    hipMemcpyDeviceToHost, c->s));
         }
         hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, 1, w, n, w, c->partial);
@@ -306,75 +269,25 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
         V2D_HIP(hipMemcpyAsync(&wn2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
         V2D_HIP(hipStreamSynchronize(c->s));
         if (sweep_failed(c->wf.err_host)) return -1;
-        const double hn = std::sqrt(std::max(wn2, 0.0));
-        if (!std::isfinite(hn)) {   // overflow in the FP32 vectors: not converged
-            *iters = j + 1;
-            *relres = INFINITY;
-            return 1;
-        }
-        for (int i = 0; i <= j; ++i) H[i * m + j] = h[i] + h2[i];
-        H[(j + 1) * m + j] = hn;
-        {   // v_{j+1} = w / hn  (hv outlives the copy: the next iteration syncs before reuse)
-            hv = hn > 0.0 ? hn : 1.0;
-            V2D_HIP(hipMemcpyAsync(c->red, &hv, sizeof(double), hipMemcpyHostToDevice, c->s));
+        const int st = G.advance(wn2, c->rtol_step, iters, relres);
+        if (st == 1 || st == 2) return st;
+        {   // v_{j+1} = w / hv  (G.hv outlives the copy: the next iteration syncs before reuse)
+            V2D_HIP(hipMemcpyAsync(c->red, &G.hv, sizeof(double), hipMemcpyHostToDevice, c->s));
             hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, w,
                                c->red, 1, w);
             V2D_LAUNCH();
         }
-        for (int i = 0; i < j; ++i) {
-            const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
-            H[(i + 1) * m + j] = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
-            H[i * m + j] = t;
-        }
-        const double d = std::hypot(H[j * m + j], H[(j + 1) * m + j]);
-        cs[j] = d > 0.0 ? H[j * m + j] / d : 1.0;
-        sn[j] = d > 0.0 ? H[(j + 1) * m + j] / d : 0.0;
-        H[j * m + j] = d;
-        H[(j + 1) * m + j] = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
-        est = std::fabs(g[j + 1]);
-        hist[j] = est;
-        if (!std::isfinite(est) || !(d > 0.0)) {
-            *iters = j + 1;
-            *relres = INFINITY;
-            return 1;
-        }
-        if (est <= c->rtol_step || hn == 0.0) { ++j; break; }
-        if (budget > 0 && j >= 2) {
-            // geometric projection of the iterations this preconditioner needs,
-            // from the average rate so far, and of the iterations still needed
-            // from the rate of the last two (a stale factorisation often gives
-            // one large first reduction and then stalls, which only the recent
-            // rate shows)
-            const double rate = std::pow(est, 1.0 / (j + 1));
-            const double need = rate < 1.0 ? std::log(c->rtol_step) / std::log(rate) : INFINITY;
-            double rest = 0.0;
-            if (j >= 3) {
-                const double rr = std::sqrt(est / hist[j - 2]);
-                rest = rr < 1.0 ? std::log(c->rtol_step / est) / std::log(rr) : INFINITY;
-            }
-            if (need > budget || rest > budget) {
-                *iters = j + 1;
-                *relres = est;
-                return 2;
-            }
-        }
+        if (st == 0) break;
     }
-    const int k = std::min(j, m);
-    std::vector<double> yv(k, 0.0);
-    for (int i = k - 1; i >= 0; --i) {
-        double acc = g[i];
-        for (int q = i + 1; q < k; ++q) acc -= H[i * m + q] * yv[q];
-        yv[i] = acc / H[i * m + i];
-    }
+    const std::vector<double> yv = G.solution();
+    const int k = (int)yv.size();
     V2D_HIP(hipMemcpyAsync(c->hbuf, yv.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->s));
     hipLaunchKernelGGL((k_fill_zero_d), dim3(grid_for(n)), dim3(256), 0, c->s, n, c->gxd);
     hipLaunchKernelGGL((k_update_sol<float, double>), dim3(grid_for(n)), dim3(256), 0, c->s, n, k, c->gZ,
                        n, c->hbuf, 1.0, c->gxd);
     V2D_LAUNCH();
+    V2D_HIP(hipStreamSynchronize(c->s));   // yv is a local
     *iters = k;
-    *relres = est;
     return 0;
 }
 
@@ -382,9 +295,6 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
 
 using namespace v2d;
 
-struct v2d_ctx {
-    std::unique_ptr<Ctx> c;
-};
 
 extern "C" {
 
@@ -865,7 +775,10 @@ int v2d_ctx_set_forcing(v2d_ctx *ctx, double eta_max, double r_target)
     return 0;
 }
 
-static int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double urot)
+}  // extern "C"
+
+namespace v2d {
+int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double urot)
 {
     const AsmArgs a = c->args(re, invdtau, inv2dt, urot);
     c->last_re = re; c->last_invdtau = invdtau; c->last_inv2dt = inv2dt; c->last_urot = urot;
@@ -879,6 +792,9 @@ static int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double
     V2D_LAUNCH();
     return 0;
 }
+}  // namespace v2d
+
+extern "C" {
 
 int v2d_ctx_set_sweep_kernel(v2d_ctx *ctx, int kind)
 {
@@ -940,7 +856,6 @@ int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind)
     return 0;
 }
 
-static int ctx_factor(Ctx *c);
 
 int v2d_ctx_profile_factor(v2d_ctx *ctx, double *out)
 {
@@ -1073,7 +988,10 @@ int v2d_ctx_spmv(v2d_ctx *ctx, const double *x, double *y)
     return 0;
 }
 
-static int ctx_factor(Ctx *c)
+}  // extern "C"
+
+namespace v2d {
+int ctx_factor(Ctx *c)
 {
     int rc;
     V2D_HIP(hipEventRecord(c->ev[4], c->s));
@@ -1094,6 +1012,9 @@ static int ctx_factor(Ctx *c)
     ++c->n_fac;
     return 0;
 }
+}  // namespace v2d
+
+extern "C" {
 
 int v2d_ctx_factor(v2d_ctx *ctx)
 {
@@ -1143,6 +1064,37 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
     return 0;
 }
 
+}  // extern "C"
+
+namespace v2d {
+// GMRES tolerance: gmres_rtol, or the Eisenstat-Walker forcing term
+// eta_k = 0.9 (||R_k|| / ||R_k-1||)^2 (safeguarded, capped at eta_max)
+double forcing_rtol(Ctx *c, double bn)
+{
+    double rtol = c->gm_rtol;
+    if (c->eta_max > 0.0) {
+        double eta = c->eta_max;
+        if (c->rn_prev > 0.0) {
+            const double r = bn / c->rn_prev;
+            eta = 0.9 * r * r;
+            const double sg = 0.9 * c->eta_prev * c->eta_prev;
+            if (sg > 0.1) eta = std::max(eta, sg);
+        }
+        eta = std::min(eta, c->eta_max);
+        // no more accuracy than the residual target needs: eta_k >=
+        // 0.5 r_target / ||R_k|| (Kelley, Iterative Methods for Linear
+        // and Nonlinear Equations, 1995, section 6.3.2)
+        if (c->r_target > 0.0) eta = std::min(c->eta_max, std::max(eta, 0.5 * c->r_target / bn));
+        rtol = std::max(eta, c->gm_rtol);
+        c->eta_prev = eta;
+    }
+    c->rn_prev = bn;
+    return rtol;
+}
+}  // namespace v2d
+
+extern "C" {
+
 int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, double urot,
                         double *res, double *rnorm, int *lin_iters)
 {
@@ -1180,26 +1132,7 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
                                c->rhs, c->red, 1, c->gb);
             V2D_LAUNCH();
             double rr = 1.0;
-            // GMRES tolerance: gmres_rtol, or the Eisenstat-Walker forcing term
-            // eta_k = 0.9 (||R_k|| / ||R_k-1||)^2 (safeguarded, capped at eta_max)
-            c->rtol_step = c->gm_rtol;
-            if (c->eta_max > 0.0) {
-                double eta = c->eta_max;
-                if (c->rn_prev > 0.0) {
-                    const double r = bn / c->rn_prev;
-                    eta = 0.9 * r * r;
-                    const double sg = 0.9 * c->eta_prev * c->eta_prev;
-                    if (sg > 0.1) eta = std::max(eta, sg);
-                }
-                eta = std::min(eta, c->eta_max);
-                // no more accuracy than the residual target needs: eta_k >=
-                // 0.5 r_target / ||R_k|| (Kelley, Iterative Methods for Linear
-                // and Nonlinear Equations, 1995, section 6.3.2)
-                if (c->r_target > 0.0) eta = std::min(c->eta_max, std::max(eta, 0.5 * c->r_target / bn));
-                c->rtol_step = std::max(eta, c->gm_rtol);
-                c->eta_prev = eta;
-            }
-            c->rn_prev = bn;
+            c->rtol_step = forcing_rtol(c, bn);
             // Iterations on the stored factorisation are worth spending while
             // they cost less than a refactorisation plus the iterations a
             // fresh one needs (both measured on this context).

@@ -88,6 +88,64 @@ __global__ void k_apply_update(int64_t N, const double *d, double *psi, double *
 __global__ void k_forces(int imax, double re, double dksi, double deta, const double *m, int64_t N,
                          const double *ome, double ref_length, double x0, double y0, double *out);
 
+// ------------------------------------------------------ batched launches --
+// One launch over up to kBatchMax cases of uneven sizes: case c owns the
+// blocks [off[c], off[c + 1]) of the grid and runs exactly what its own
+// single-case launch would run on them (the same per-node or per-block device
+// code, the same block count for the reductions), so a batched result equals
+// the single-case result bit for bit.
+constexpr int kBatchMax = 16;
+struct BMap {
+    int nc;
+    int off[kBatchMax + 1];
+};
+__device__ __forceinline__ int bmap_case(const BMap &bm, int b)
+{
+    int c = 0;
+    while (c + 1 < bm.nc && b >= bm.off[c + 1]) ++c;
+    return c;
+}
+// One case's assembly.
+struct BAsm {
+    AsmArgs a;
+    AsmConsts k;
+    const uint8_t *flags;
+    void *val;
+    double *rhs;
+};
+// One case's operands of a batched vector operation (the fields an
+// operation does not use are ignored).
+struct BVec {
+    int imax, jmax;       // mesh (SpMV)
+    int m;                // basis vectors taking part (multi-dot, multi-axpy)
+    int nb;               // blocks of the case's single-context reduction
+    int inv;              // scale: divide by s (1) or multiply (0)
+    int64_t n, ld;        // vector length, basis leading dimension
+    const uint8_t *flags;
+    const void *val;      // Jacobian slots (SpMV)
+    const void *x;        // SpMV / scale / Newton-update input
+    void *y;              // SpMV / scale output; multi-axpy in-out vector
+    const void *Vb;       // basis (multi-dot, multi-axpy)
+    const void *w;        // multi-dot right operand
+    double *partial;      // per-block partial sums
+    double *out;          // reduced sums
+    const double *h;      // multi-axpy coefficients
+    const double *s;      // scale factor
+    double *psi, *ome;    // Newton update targets
+};
+template <typename T>
+__global__ void k_assemble_bt(BMap bm, const BAsm *cs);
+template <typename T, typename V>
+__global__ void k_spmv_bt(BMap bm, const BVec *v);
+template <typename V>
+__global__ void k_multidot_bt(BMap bm, const BVec *v);
+__global__ void k_reduce_bt(const BVec *v);
+template <typename V>
+__global__ void k_multiaxpy_bt(BMap bm, const BVec *v);
+template <typename A, typename B>
+__global__ void k_scale_copy_bt(BMap bm, const BVec *v);
+__global__ void k_apply_update_bt(BMap bm, const BVec *v);
+
 // ------------------------------------------------------- error state -------
 void set_error(const std::string &msg);
 #define V2D_HIP(call)                                                                       \

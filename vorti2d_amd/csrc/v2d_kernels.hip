@@ -270,13 +270,12 @@ __device__ __forceinline__ NodeIx global_ix(int imax, int i, int j)
 // ulp from the reference's rounding, and no FP64 divide sequences in the
 // interior.  The parity COO kernels keep the reference's exact operations.
 template <typename T>
-__global__ void __launch_bounds__(256) k_assemble(AsmArgs a, AsmConsts c,
-                                                  const uint8_t *__restrict__ flags,
-                                                  T *__restrict__ val, double *__restrict__ rhs)
+__device__ __forceinline__ void assemble_node(const AsmArgs &a, const AsmConsts &c,
+                                              const uint8_t *__restrict__ flags, T *__restrict__ val,
+                                              double *__restrict__ rhs, int64_t k)
 {
     const int imax = a.imax, jmax = a.jmax;
     const int64_t N = (int64_t)imax * jmax;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
     const int j = (int)(k / imax), i = (int)(k - (int64_t)j * imax);
     const NodeIx s = global_ix(imax, i, j);
@@ -368,6 +367,13 @@ __global__ void __launch_bounds__(256) k_assemble(AsmArgs a, AsmConsts c,
         vo[k] = (T)1.0;
         rhs[N + k] = -O[k];
     }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_assemble(AsmArgs a, AsmConsts c,
+                                                  const uint8_t *__restrict__ flags,
+                                                  T *__restrict__ val, double *__restrict__ rhs)
+{
+    assemble_node<T>(a, c, flags, val, rhs, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 template __global__ void k_assemble<double>(AsmArgs, AsmConsts, const uint8_t *, double *, double *);
 template __global__ void k_assemble<float>(AsmArgs, AsmConsts, const uint8_t *, float *, double *);
@@ -555,12 +561,11 @@ template __global__ void k_slots_to_csr<float>(int, int, const uint8_t *, const 
 // One thread per node computes both rows.  Accumulation type A (double for
 // FP64 vectors, float for FP32 vectors).
 template <typename T, typename V>
-__global__ void __launch_bounds__(256) k_spmv(int imax, int jmax, const uint8_t *__restrict__ flags,
-                                              const T *__restrict__ val, const V *__restrict__ x,
-                                              V *__restrict__ y)
+__device__ __forceinline__ void spmv_node(int imax, int jmax, const uint8_t *__restrict__ flags,
+                                          const T *__restrict__ val, const V *__restrict__ x,
+                                          V *__restrict__ y, int64_t k)
 {
     const int64_t N = (int64_t)imax * jmax;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
     const int j = (int)(k / imax), i = (int)(k - (int64_t)j * imax);
     const V *xp = x, *xo = x + N;
@@ -590,6 +595,13 @@ __global__ void __launch_bounds__(256) k_spmv(int imax, int jmax, const uint8_t 
     y[k] = yp;
     y[N + k] = yo;
 }
+template <typename T, typename V>
+__global__ void __launch_bounds__(256) k_spmv(int imax, int jmax, const uint8_t *__restrict__ flags,
+                                              const T *__restrict__ val, const V *__restrict__ x,
+                                              V *__restrict__ y)
+{
+    spmv_node<T, V>(imax, jmax, flags, val, x, y, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
 template __global__ void k_spmv<double, double>(int, int, const uint8_t *, const double *, const double *, double *);
 template __global__ void k_spmv<float, float>(int, int, const uint8_t *, const float *, const float *, float *);
 
@@ -597,19 +609,20 @@ template __global__ void k_spmv<float, float>(int, int, const uint8_t *, const f
 // partial[b*m + c] = sum over the block's share of  V_c . w   (c < m), FP64
 // accumulation.  V is column-major with leading dimension ld.  Fused
 // multi-dot for classical Gram-Schmidt: w is read once per block for all c.
+// Block `blk` of `nblk` (256 threads) of the fused multi-dot.
 template <typename V>
-__global__ void __launch_bounds__(256) k_multidot(int64_t n, int m, const V *__restrict__ Vb, int64_t ld,
-                                                  const V *__restrict__ w, double *__restrict__ partial)
+__device__ __forceinline__ void multidot_block(int64_t n, int m, const V *__restrict__ Vb, int64_t ld,
+                                               const V *__restrict__ w, double *__restrict__ partial,
+                                               int blk, int nblk, double *sh)
 {
-    __shared__ double sh[17];
     constexpr int kMax = 8;
     for (int c0 = 0; c0 < m; c0 += kMax) {
         const int mc = min(kMax, m - c0);
         double acc[kMax];
 #pragma unroll
         for (int c = 0; c < kMax; ++c) acc[c] = 0.0;
-        for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-             t += (int64_t)gridDim.x * blockDim.x) {
+        for (int64_t t = (int64_t)blk * blockDim.x + threadIdx.x; t < n;
+             t += (int64_t)nblk * blockDim.x) {
             const double wt = (double)w[t];
 #pragma unroll
             for (int c = 0; c < kMax; ++c)
@@ -617,23 +630,35 @@ __global__ void __launch_bounds__(256) k_multidot(int64_t n, int m, const V *__r
         }
         for (int c = 0; c < mc; ++c) {
             const double s = block_sum(acc[c], sh);
-            if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * m + c0 + c] = s;
+            if (threadIdx.x == 0) partial[(int64_t)blk * m + c0 + c] = s;
         }
     }
+}
+template <typename V>
+__global__ void __launch_bounds__(256) k_multidot(int64_t n, int m, const V *__restrict__ Vb, int64_t ld,
+                                                  const V *__restrict__ w, double *__restrict__ partial)
+{
+    __shared__ double sh[17];
+    multidot_block<V>(n, m, Vb, ld, w, partial, blockIdx.x, gridDim.x, sh);
 }
 template __global__ void k_multidot<double>(int64_t, int, const double *, int64_t, const double *, double *);
 template __global__ void k_multidot<float>(int64_t, int, const float *, int64_t, const float *, double *);
 
 // out[c] = sum_b partial[b*m + c]
-__global__ void k_reduce_partials(int nb, int m, const double *__restrict__ partial, double *__restrict__ out)
+__device__ __forceinline__ void reduce_partials_block(int nb, int m, const double *__restrict__ partial,
+                                                      double *__restrict__ out, double *sh)
 {
-    __shared__ double sh[17];
     for (int c = 0; c < m; ++c) {
         double s = 0.0;
         for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(int64_t)b * m + c];
         s = block_sum(s, sh);
         if (threadIdx.x == 0) out[c] = s;
     }
+}
+__global__ void k_reduce_partials(int nb, int m, const double *__restrict__ partial, double *__restrict__ out)
+{
+    __shared__ double sh[17];
+    reduce_partials_block(nb, m, partial, out, sh);
 }
 
 // w -= sum_c h[c] V_c  (h FP64 on device)
@@ -745,6 +770,92 @@ __global__ void __launch_bounds__(256) k_apply_update(int64_t N, const double *_
     if (t >= N) return;
     psi[t] += d[t];
     ome[t] += d[N + t];
+}
+
+// ====================================================== batched launches ==
+// Every case of a batched launch runs its single-case code on its own blocks
+// (BMap); pointwise kernels index by the block's offset inside the case, the
+// multi-dot keeps the case's own block count and stride.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_assemble_bt(BMap bm, const BAsm *__restrict__ cs)
+{
+    const int c = bmap_case(bm, blockIdx.x);
+    const BAsm &q = cs[c];
+    assemble_node<T>(q.a, q.k, q.flags, static_cast<T *>(q.val), q.rhs,
+                     (int64_t)(blockIdx.x - bm.off[c]) * blockDim.x + threadIdx.x);
+}
+template __global__ void k_assemble_bt<double>(BMap, const BAsm *);
+template __global__ void k_assemble_bt<float>(BMap, const BAsm *);
+
+template <typename T, typename V>
+__global__ void __launch_bounds__(1024) k_spmv_bt(BMap bm, const BVec *__restrict__ v)
+{
+    const int c = bmap_case(bm, blockIdx.x);
+    const BVec &q = v[c];
+    spmv_node<T, V>(q.imax, q.jmax, q.flags, static_cast<const T *>(q.val), static_cast<const V *>(q.x),
+                    static_cast<V *>(q.y), (int64_t)(blockIdx.x - bm.off[c]) * blockDim.x + threadIdx.x);
+}
+template __global__ void k_spmv_bt<float, float>(BMap, const BVec *);
+template __global__ void k_spmv_bt<double, double>(BMap, const BVec *);
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_multidot_bt(BMap bm, const BVec *__restrict__ v)
+{
+    __shared__ double sh[17];
+    const int c = bmap_case(bm, blockIdx.x);
+    const BVec &q = v[c];
+    multidot_block<V>(q.n, q.m, static_cast<const V *>(q.Vb), q.ld, static_cast<const V *>(q.w), q.partial,
+                      blockIdx.x - bm.off[c], q.nb, sh);
+}
+template __global__ void k_multidot_bt<float>(BMap, const BVec *);
+template __global__ void k_multidot_bt<double>(BMap, const BVec *);
+
+__global__ void __launch_bounds__(256) k_reduce_bt(const BVec *__restrict__ v)
+{
+    __shared__ double sh[17];
+    const BVec &q = v[blockIdx.x];
+    reduce_partials_block(q.nb, q.m, q.partial, q.out, sh);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(1024) k_multiaxpy_bt(BMap bm, const BVec *__restrict__ v)
+{
+    const int c = bmap_case(bm, blockIdx.x);
+    const BVec &q = v[c];
+    const int64_t t = (int64_t)(blockIdx.x - bm.off[c]) * blockDim.x + threadIdx.x;
+    if (t >= q.n) return;
+    const V *Vb = static_cast<const V *>(q.Vb);
+    V *w = static_cast<V *>(q.y);
+    double acc = (double)w[t];
+    for (int k = 0; k < q.m; ++k) acc -= q.h[k] * (double)Vb[k * q.ld + t];
+    w[t] = (V)acc;
+}
+template __global__ void k_multiaxpy_bt<float>(BMap, const BVec *);
+template __global__ void k_multiaxpy_bt<double>(BMap, const BVec *);
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(1024) k_scale_copy_bt(BMap bm, const BVec *__restrict__ v)
+{
+    const int c = bmap_case(bm, blockIdx.x);
+    const BVec &q = v[c];
+    const int64_t t = (int64_t)(blockIdx.x - bm.off[c]) * blockDim.x + threadIdx.x;
+    if (t >= q.n) return;
+    const double f = q.inv ? 1.0 / q.s[0] : q.s[0];
+    static_cast<B *>(q.y)[t] = (B)((double)static_cast<const A *>(q.x)[t] * f);
+}
+template __global__ void k_scale_copy_bt<float, float>(BMap, const BVec *);
+template __global__ void k_scale_copy_bt<double, float>(BMap, const BVec *);
+template __global__ void k_scale_copy_bt<double, double>(BMap, const BVec *);
+
+__global__ void __launch_bounds__(1024) k_apply_update_bt(BMap bm, const BVec *__restrict__ v)
+{
+    const int c = bmap_case(bm, blockIdx.x);
+    const BVec &q = v[c];
+    const int64_t t = (int64_t)(blockIdx.x - bm.off[c]) * blockDim.x + threadIdx.x;
+    if (t >= q.n) return;
+    const double *d = static_cast<const double *>(q.x);
+    q.psi[t] += d[t];
+    q.ome[t] += d[q.n + t];
 }
 
 // ================================================================ forces ====
