@@ -180,6 +180,54 @@ int v2d_ctx_set_sweep_spin_limit(v2d_ctx *ctx, int polls);
  * waits expired) and the number of persistent solves redone with launch pairs
  * so far.  No counterpart in the reference. */
 int v2d_ctx_get_sweep_kernel(v2d_ctx *ctx, int *kind, int64_t *fallbacks);
+/* Linear solver of the Newton systems (the exact FP64 step, the FP32
+ * preconditioner of the mixed path, the adjoint's transposed solves).
+ * V2D_SOLVER_MULTIFRONTAL: nested-dissection multifrontal LU (the structure of
+ * the reference's MUMPS direct solve, linsolve="mumps":
+ * vorti2d/petsc_solver.py:37-62, vorti2d/dist_solver.py:133-137) -- the grid's
+ * node graph is dissected by grid lines into a tree of dense fronts, factored
+ * height by height on the device.  V2D_SOLVER_BLOCK_LU: the block-tridiagonal
+ * LU over radial levels with dense per-level inverses.  Both are exact LUs
+ * without pivoting; they differ in rounding.  Selecting a solver drops a
+ * stored factorisation; each solver's storage is allocated at its first
+ * factorisation. */
+enum { V2D_SOLVER_BLOCK_LU = 0, V2D_SOLVER_MULTIFRONTAL = 1 };
+int v2d_ctx_set_linear_solver(v2d_ctx *ctx, int kind);
+int v2d_ctx_get_linear_solver(v2d_ctx *ctx, int *kind);
+/* Sizes of the multifrontal analysis of a context's grid (built on first
+ * call): out[0..7] = fronts, tree heights, fronts swept by the matrix-core
+ * Gauss-Jordan steps, front storage (elements), front-vector elements,
+ * Gauss-Jordan panel elements, Jacobian entries, distinct front positions they
+ * fill; *flops = floating-point operations of one factorisation.  No
+ * counterpart in the reference (MUMPS reports its own analysis statistics). */
+int v2d_ctx_mf_info(v2d_ctx *ctx, int64_t *out, double *flops);
+
+/* Copy of the factored fronts (front storage elements, FP64; each front's
+ * nf x ld block row-major at its offset fo, holding [G W; X C]), for
+ * inspection.  No counterpart in the reference. */
+int v2d_ctx_mf_fronts(v2d_ctx *ctx, double *F);
+
+/* The multifrontal analysis alone, on the host (no device call), for
+ * inspection: ff_flags[i] are the far-field outflow bits of node (i, jmax-1)
+ * (1: omega row couples j-1, j-2; 2: psi row does; NULL: none).  No
+ * counterpart in the reference. */
+typedef struct v2d_mf_analysis v2d_mf_analysis;
+int v2d_mf_analyse(int imax, int jmax, const uint8_t *ff_flags, v2d_mf_analysis **out);
+/* out[0..15]: fronts, heights, big fronts, front storage elements, front
+ * vector elements, pivot-vector elements, front-unknown entries, child-map
+ * entries, child-list entries, distinct assembly positions, Jacobian entries,
+ * rows-table entries, 0, 0, 0, 0; *flops as in v2d_ctx_mf_info. */
+int v2d_mf_sizes(const v2d_mf_analysis *a, int64_t *out, double *flops);
+/* fronts: 14 int64 per front (fo, vo, zo, uo, mo, ns, nsp, nb, nf, ld, nch,
+ * co, big, height); fu: the field-blocked unknown of every front row (-1 on
+ * pivot padding); ch: child lists; map: per front and child, the child's
+ * coupling row of every parent front row (-1: none); aoff: the front-storage
+ * offset of every assembly position, aseg (positions + 1) its range in avid,
+ * avid: slot-layout value indices (the value plane times N plus the node).
+ * Any pointer may be NULL. */
+int v2d_mf_export(const v2d_mf_analysis *a, int64_t *fronts, int *fu, int *ch, int *map, int64_t *aoff,
+                  int *aseg, int *avid);
+int v2d_mf_destroy(v2d_mf_analysis *a);
 /* Mixed precision, refactor_every = 0: when GMRES on a stored FP32
  * factorisation gives up.  V2D_REFACTOR_FIXED (the default): once the
  * geometric projection of its convergence exceeds gmres_max_iter iterations,

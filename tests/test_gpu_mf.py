@@ -1,0 +1,145 @@
+"""The device multifrontal LU (V2D_SOLVER_MULTIFRONTAL) against scipy's sparse
+LU and against the block-tridiagonal LU, on the Newton Jacobian the device
+assembles (the steady cylinder, Re 40, after one Newton step from the
+impulsive start).
+
+* FP64 solves A x = b and A^T x = b within 1e-9 (relative 2-norm) of SuperLU
+  on the exported matrix, for the three far-field conditions and grids from
+  33^2 to 181^2 (small fronts in LDS only at 33^2; dataflow Gauss-Jordan
+  fronts from 97^2 on), and within 1e-9 of the block LU's solution;
+* FP32 (the mixed path's preconditioner) within 1e-3 of SuperLU;
+* at 513^2 (multi-step Gauss-Jordan fronts, up to 17 pivot blocks) the FP64
+  residual of the Newton right-hand side is below 1e-8 and within 4x of the
+  block LU's, and the two solutions agree within 1e-8;
+* a steady FP64 Newton solve with it takes the block LU's steps and ends on
+  its state within 1e-10; the mixed solve converges to the same state within
+  1e-8.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _ctx(v, n, ff, prec, solver):
+    xg, yg = v.generate_cylinder(n, n, 0.5, 50.0)
+    ctx = v.DeviceContext(xg, yg, ff, 1.0, 0.0, prec)
+    ctx.set_linear_solver(solver)
+    ctx.init_state()
+    return ctx
+
+
+def _system_after_one_step(ctx):
+    ctx.newton_step(40.0)
+    ctx.assemble(40.0)
+    ip, ix, dv = ctx.export_csr()
+    n2 = 2 * ctx.N
+    return sp.csr_matrix((dv, ix, ip), shape=(n2, n2))
+
+
+@pytest.mark.parametrize("n,ff", [(33, 0), (41, 1), (97, 2), (181, 0)])
+def test_fp64_solves_match_superlu(n, ff):
+    import vorti2d_amd as v
+    say(f"multifrontal FP64 {n}x{n} ff {ff}")
+    ctx = _ctx(v, n, ff, "fp64", "multifrontal")
+    A = _system_after_one_step(ctx)
+    ctx.factor()
+    rng = np.random.default_rng(n + ff)
+    b = rng.standard_normal(A.shape[0])
+    x = ctx.solve(b)
+    xt = ctx.solve_transpose(b)
+    lu = spla.splu(A.tocsc())
+    xr, xtr = lu.solve(b), lu.solve(b, trans="T")
+    e, et = (np.linalg.norm(p - q) / np.linalg.norm(q) for p, q in ((x, xr), (xt, xtr)))
+    say(f"  solve {e:.2e}, transposed {et:.2e} (relative to SuperLU)")
+    assert e <= 1e-9 and et <= 1e-9
+    blu = ctx.like("fp64")
+    blu.set_linear_solver("block_lu")
+    blu.assemble(40.0)
+    blu.factor()
+    xb = blu.solve(b)
+    assert np.linalg.norm(x - xb) <= 1e-9 * np.linalg.norm(xb)
+    blu.close()
+    ctx.close()
+
+
+def test_fp32_solve_close_to_superlu():
+    import vorti2d_amd as v
+    ctx = _ctx(v, 97, 0, "mixed", "multifrontal")
+    ctx.set_solver(refactor_every=1)
+    A = _system_after_one_step(ctx)
+    ctx.factor()
+    b = np.random.default_rng(3).standard_normal(A.shape[0])
+    x = ctx.solve(b)
+    xr = spla.spsolve(A.tocsc(), b)
+    e = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    say(f"multifrontal FP32 97x97: {e:.2e} relative to SuperLU")
+    assert e <= 1e-3
+    ctx.close()
+
+
+def test_fp64_513_residual():
+    """Residual of the Newton system's own right-hand side (a random one is
+    far rougher: scipy's pivoted SuperLU leaves ||A x - b|| / ||b|| ~ 2e-8 on
+    it at 257^2) against the block LU's on the same system."""
+    import vorti2d_amd as v
+    n = 513
+    say(f"multifrontal FP64 {n}x{n}")
+    ctx = _ctx(v, n, 0, "fp64", "multifrontal")
+    ctx.newton_step(40.0)
+    ctx.assemble(40.0)
+    info = ctx.mf_info()
+    say(f"  {info['fronts']} fronts, {info['heights']} heights, {info['big_fronts']} big, "
+        f"{info['flops']:.3e} flop")
+    b = ctx.export_coo()[3]
+    ctx.factor()
+    x = ctx.solve(b)
+    r = np.linalg.norm(ctx.spmv(x) - b) / np.linalg.norm(b)
+    blu = ctx.like("fp64")
+    blu.set_linear_solver("block_lu")
+    blu.assemble(40.0)
+    blu.factor()
+    xb = blu.solve(b)
+    rb = np.linalg.norm(blu.spmv(xb) - b) / np.linalg.norm(b)
+    say(f"  ||A x - b|| / ||b|| = {r:.2e} (block LU {rb:.2e}); |x - x_blu| / |x_blu| "
+        f"{np.linalg.norm(x - xb) / np.linalg.norm(xb):.2e}")
+    assert r <= 1e-8 and r <= 4.0 * rb + 1e-12
+    assert np.linalg.norm(x - xb) <= 1e-8 * np.linalg.norm(xb)
+    blu.close()
+    ctx.close()
+
+
+def test_newton_matches_block_lu():
+    import vorti2d_amd as v
+    n = 181
+    out = {}
+    for solver in ("block_lu", "multifrontal"):
+        ctx = _ctx(v, n, 0, "fp64", solver)
+        r = ctx.steady_newton(40.0, tol=1e-10, rms_tol=1e-10, max_iter=30, record=True)
+        out[solver] = (r, *ctx.get_state(("psi", "ome")))
+        say(f"  {solver}: {r.iters} steps, res {r.res:.2e}, ||R|| {r.rnorm:.2e}")
+        ctx.close()
+    (rb, pb, ob), (rm, pm, om) = out["block_lu"], out["multifrontal"]
+    assert rb.converged and rm.converged and rb.iters == rm.iters
+    assert np.abs(pm - pb).max() <= 1e-10 * max(1.0, np.abs(pb).max())
+    assert np.abs(om - ob).max() <= 1e-10 * max(1.0, np.abs(ob).max())
+    ctx = _ctx(v, n, 0, "mixed", "multifrontal")
+    ctx.set_solver(refactor_every=0, gmres_rtol=1e-6, gmres_max_iter=30)
+    ctx.set_forcing(0.25, 1e-10 * np.sqrt(2 * ctx.N))
+    r = ctx.steady_newton(40.0, tol=1e-10, rms_tol=1e-10, max_iter=30, record=True)
+    p, o = ctx.get_state(("psi", "ome"))
+    say(f"  mixed multifrontal: {r.iters} steps, GMRES {sum(r.lin_iters)}, res {r.res:.2e}")
+    assert r.converged
+    assert np.abs(p - pb).max() <= 1e-8 and np.abs(o - ob).max() <= 1e-8
+    ctx.close()

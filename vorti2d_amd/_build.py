@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvorti2d.so")
-SOURCES = ("v2d_kernels.hip", "v2d_direct.hip", "v2d_adjoint.hip", "v2d_capi.hip", "v2d_batch.hip", "v2d_noalloc.hip")
+SOURCES = ("v2d_kernels.hip", "v2d_direct.hip", "v2d_adjoint.hip", "v2d_capi.hip", "v2d_batch.hip", "v2d_noalloc.hip", "v2d_mf.hip")
 
 # -ffp-contract=off keeps the reference operation order in the metric and
 # assembly arithmetic (bit-identical FP64); the dense kernels use explicit fma.

@@ -84,6 +84,14 @@ _SIGS = {
     "v2d_ctx_adjoint_bdf2_begin": ([vp], i32),
     "v2d_ctx_adjoint_bdf2_step": ([vp, f64, f64, f64, f64, f64, f64, vp], i32),
     "v2d_ctx_adjoint_bdf2_end": ([vp, vp, vp, vp], i32),
+    "v2d_ctx_set_linear_solver": ([vp, i32], i32),
+    "v2d_ctx_get_linear_solver": ([vp, P(i32)], i32),
+    "v2d_ctx_mf_info": ([vp, vp, P(f64)], i32),
+    "v2d_ctx_mf_fronts": ([vp, vp], i32),
+    "v2d_mf_analyse": ([i32, i32, vp, P(vp)], i32),
+    "v2d_mf_sizes": ([vp, vp, P(f64)], i32),
+    "v2d_mf_export": ([vp, vp, vp, vp, vp, vp, vp, vp], i32),
+    "v2d_mf_destroy": ([vp], i32),
 }
 
 for _name, (_args, _res) in _SIGS.items():

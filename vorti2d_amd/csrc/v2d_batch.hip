@@ -588,8 +588,7 @@ int v2d_batch_newton_step(v2d_batch *bh, const int *active, const double *re, co
         if (c->prec == V2D_FP64) {
             tasks.emplace_back([c] {
                 if (ctx_factor(c)) return -1;
-                return direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->rhs,
-                                            c->delta, c->s);
+                return lin_solve<double>(c, c->rhs, c->delta, false);
             });
             continue;
         }

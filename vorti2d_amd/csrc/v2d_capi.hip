@@ -86,11 +86,104 @@ static int ctx_bc(Ctx *c, int ff_bc, double ca, double sa)
     V2D_HIP(hipStreamSynchronize(c->s));
     c->nnz = count_rows(c->imax, c->jmax, 0, 2 * N, c->hflags);
     c->drop_graph();
+    c->drop_mf();
     direct_free_pattern(c->pat, c->s);
     if (direct_build_pattern(c->pat, c->imax, c->jmax, c->hflags.data(), c->s)) return -1;
     c->factored = false;
     return 0;
 }
+
+// ------------------------------------------------------- linear solver -----
+// Block-LU storage of the context's precision, allocated at its first
+// factorisation.
+static int ensure_block_lu(Ctx *c)
+{
+    if (c->blu_ready) return 0;
+    int rc = 0;
+    const int64_t N = c->N, nm = c->pat.nmax;
+    if (c->prec == V2D_FP64) {
+        c->wd.sinv = c->alloc<double>(c->pat.sinv_elems, rc);
+        c->wd.tmat = c->alloc<double>(2 * nm * c->pat.ldmax, rc);
+        c->wd.colk = c->alloc<double>(2 * nm * 64, rc);
+        c->wd.rowk = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
+        c->wd.colk2 = c->alloc<double>(2 * nm * 64, rc);
+        c->wd.rowk2 = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
+        c->wd.dinv = c->alloc<double>(2 * 64 * 64, rc);
+        c->wd.z = c->alloc<double>(2 * N, rc);
+        c->wd.wv = c->alloc<double>(2 * N, rc);
+        c->wd.cfL = c->alloc<double>(kCouple * 2 * N, rc);
+        c->wd.cfU = c->alloc<double>(kCouple * 2 * N, rc);
+        c->wd.cfC = c->alloc<double>(std::max<int64_t>(1, c->pat.ncx), rc);
+        c->wd.yv = c->alloc<double>(2 * N, rc);
+        c->wd.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
+    } else {
+        c->wf.sinv = c->alloc<float>(c->pat.sinv_elems, rc);
+        c->wf.tmat = c->alloc<float>(2 * nm * c->pat.ldmax, rc);
+        c->wf.colk = c->alloc<float>(2 * nm * 64, rc);
+        c->wf.rowk = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
+        c->wf.colk2 = c->alloc<float>(2 * nm * 64, rc);
+        c->wf.rowk2 = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
+        c->wf.dinv = c->alloc<float>(2 * 64 * 64, rc);
+        c->wf.z = c->alloc<float>(2 * N, rc);
+        c->wf.wv = c->alloc<float>(2 * N, rc);
+        c->wf.cfL = c->alloc<float>(kCouple * 2 * N, rc);
+        c->wf.cfU = c->alloc<float>(kCouple * 2 * N, rc);
+        c->wf.cfC = c->alloc<float>(std::max<int64_t>(1, c->pat.ncx), rc);
+        c->wf.yv = c->alloc<float>(2 * N, rc);
+        c->wf.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
+    }
+    if (rc) return -1;
+    c->blu_ready = true;
+    return 0;
+}
+
+// The multifrontal analysis of the grid (host, then its tables uploaded).
+static int ensure_mf_pattern(Ctx *c)
+{
+    if (c->mfp) return 0;
+    auto p = std::make_unique<MfPattern>();
+    if (mf_analyse(*p, c->imax, c->jmax, c->hflags.data())) return -1;
+    if (mf_upload(*p, c->s)) return -1;
+    c->mfp = std::move(p);
+    return 0;
+}
+
+// ... and the multifrontal storage of the context's precision.
+static int ensure_mf(Ctx *c)
+{
+    if (ensure_mf_pattern(c)) return -1;
+    const MfPattern &P = *c->mfp;
+    const int64_t elems = P.F_elems + P.V_elems + P.Z_elems + P.pool_elems;
+    if (c->prec == V2D_FP64 && !c->md.F) {
+        if (mf_alloc(P, c->md, c->s)) return -1;
+        c->bytes += (int64_t)sizeof(double) * elems;
+    }
+    if (c->prec != V2D_FP64 && !c->mw.F) {
+        if (mf_alloc(P, c->mw, c->s)) return -1;
+        c->bytes += (int64_t)sizeof(float) * elems;
+    }
+    return 0;
+}
+
+template <typename T> static DirectWork<T> &blu_work(Ctx *c);
+template <> DirectWork<double> &blu_work<double>(Ctx *c) { return c->wd; }
+template <> DirectWork<float> &blu_work<float>(Ctx *c) { return c->wf; }
+template <typename T> static MfWork<T> &mf_work(Ctx *c);
+template <> MfWork<double> &mf_work<double>(Ctx *c) { return c->md; }
+template <> MfWork<float> &mf_work<float>(Ctx *c) { return c->mw; }
+
+// x = A^-1 b (trans: A^-T b) with the stored factorisation of the context's
+// solver, queued on the context's stream.
+template <typename T>
+int lin_solve(Ctx *c, const T *b, T *x, bool trans)
+{
+    if (c->solver == V2D_SOLVER_MULTIFRONTAL) return mf_solve<T>(*c->mfp, mf_work<T>(c), b, x, trans, c->s);
+    const T *v = static_cast<const T *>(c->val_pc);
+    return trans ? direct_solve_t<T>(c->pat, blu_work<T>(c), v, b, x, c->s)
+                 : direct_solve<T>(c->pat, blu_work<T>(c), v, b, x, c->s);
+}
+template int lin_solve<double>(Ctx *, const double *, double *, bool);
+template int lin_solve<float>(Ctx *, const float *, float *, bool);
 
 // A persistent solve sweep that waited past its spin limit leaves 2 in the
 // error word; direct_solve copies the word to pinned memory after every solve,
@@ -111,13 +204,13 @@ int sweep_failed(const int *err_host)
 // device-to-device copies; otherwise the kernels are launched directly.
 int apply_pc(Ctx *c, const float *v, float *z)
 {
-    const float *vpc = static_cast<const float *>(c->val_pc);
     const int64_t n = 2 * c->N;
-    if (!c->use_graph || c->wf.sweep_kind == V2D_SWEEP_PERSISTENT || !c->ga_in || !c->ga_out)
-        return direct_solve<float>(c->pat, c->wf, vpc, v, z, c->s);
+    const bool mf = c->solver == V2D_SOLVER_MULTIFRONTAL;
+    if (!c->use_graph || (!mf && c->wf.sweep_kind == V2D_SWEEP_PERSISTENT) || !c->ga_in || !c->ga_out)
+        return lin_solve<float>(c, v, z, false);
     if (!c->age) {
         V2D_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-        const int rc = direct_solve<float>(c->pat, c->wf, vpc, c->ga_in, c->ga_out, c->s);
+        const int rc = lin_solve<float>(c, c->ga_in, c->ga_out, false);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(c->s, &g);
         if (rc || ec != hipSuccess) {
@@ -295,6 +388,10 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
 
 using namespace v2d;
 
+
+struct v2d_mf_analysis {
+    v2d::MfPattern p;
+};
 
 extern "C" {
 
@@ -532,6 +629,8 @@ static int ctx_bc_copy(Ctx *c, const Ctx *src)
     V2D_HIP(hipMemcpyAsync(c->pre_psi, src->pre_psi, sizeof(int) * c->imax, hipMemcpyDeviceToDevice, c->s));
     V2D_HIP(hipMemcpyAsync(c->pre_ome, src->pre_ome, sizeof(int) * c->imax, hipMemcpyDeviceToDevice, c->s));
     c->drop_graph();
+    c->drop_mf();
+    c->solver = src->solver;
     if (direct_copy_pattern(c->pat, src->pat, c->s)) return -1;
     c->factored = false;
     return 0;
@@ -591,37 +690,7 @@ static int ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const
     const size_t tsz = precision == V2D_FP64 ? sizeof(double) : sizeof(float);
     c->val = c->alloc<unsigned char>((int64_t)kSlots * N * tsz, rc);
     c->val_pc = c->alloc<unsigned char>((int64_t)kSlots * N * tsz, rc);
-    const int64_t nm = c->pat.nmax;
-    if (precision == V2D_FP64) {
-        c->wd.sinv = c->alloc<double>(c->pat.sinv_elems, rc);
-        c->wd.tmat = c->alloc<double>(2 * nm * c->pat.ldmax, rc);
-        c->wd.colk = c->alloc<double>(2 * nm * 64, rc);
-        c->wd.rowk = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
-        c->wd.colk2 = c->alloc<double>(2 * nm * 64, rc);
-        c->wd.rowk2 = c->alloc<double>(2 * (int64_t)c->pat.ldmax * 64, rc);
-        c->wd.dinv = c->alloc<double>(2 * 64 * 64, rc);
-        c->wd.z = c->alloc<double>(2 * N, rc);
-        c->wd.wv = c->alloc<double>(2 * N, rc);
-        c->wd.cfL = c->alloc<double>(kCouple * 2 * N, rc);
-        c->wd.cfU = c->alloc<double>(kCouple * 2 * N, rc);
-        c->wd.cfC = c->alloc<double>(std::max<int64_t>(1, c->pat.ncx), rc);
-        c->wd.yv = c->alloc<double>(2 * N, rc);
-        c->wd.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
-    } else {
-        c->wf.sinv = c->alloc<float>(c->pat.sinv_elems, rc);
-        c->wf.tmat = c->alloc<float>(2 * nm * c->pat.ldmax, rc);
-        c->wf.colk = c->alloc<float>(2 * nm * 64, rc);
-        c->wf.rowk = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
-        c->wf.colk2 = c->alloc<float>(2 * nm * 64, rc);
-        c->wf.rowk2 = c->alloc<float>(2 * (int64_t)c->pat.ldmax * 64, rc);
-        c->wf.dinv = c->alloc<float>(2 * 64 * 64, rc);
-        c->wf.z = c->alloc<float>(2 * N, rc);
-        c->wf.wv = c->alloc<float>(2 * N, rc);
-        c->wf.cfL = c->alloc<float>(kCouple * 2 * N, rc);
-        c->wf.cfU = c->alloc<float>(kCouple * 2 * N, rc);
-        c->wf.cfC = c->alloc<float>(std::max<int64_t>(1, c->pat.ncx), rc);
-        c->wf.yv = c->alloc<float>(2 * N, rc);
-        c->wf.ctr = c->alloc<int>(std::max(1, c->pat.sweep_grid), rc);
+    if (precision != V2D_FP64) {
         c->gV = c->alloc<float>((int64_t)(c->gm_cap + 1) * 2 * N, rc);
         c->gZ = c->alloc<float>((int64_t)c->gm_cap * 2 * N, rc);
         c->gb = c->alloc<float>(2 * N, rc);
@@ -844,6 +913,110 @@ int v2d_ctx_set_graph(v2d_ctx *ctx, int enable)
     return 0;
 }
 
+int v2d_ctx_set_linear_solver(v2d_ctx *ctx, int kind)
+{
+    if (kind != V2D_SOLVER_BLOCK_LU && kind != V2D_SOLVER_MULTIFRONTAL) {
+        set_error("linear solver must be V2D_SOLVER_BLOCK_LU (0) or V2D_SOLVER_MULTIFRONTAL (1)");
+        return -1;
+    }
+    Ctx *c = ctx->c.get();
+    if (c->solver != kind) {
+        V2D_HIP(hipStreamSynchronize(c->s));
+        c->drop_graph();   // a captured apply belongs to one solver
+        c->solver = kind;
+        c->factored = false;
+    }
+    return 0;
+}
+
+int v2d_ctx_get_linear_solver(v2d_ctx *ctx, int *kind)
+{
+    *kind = ctx->c->solver;
+    return 0;
+}
+
+int v2d_ctx_mf_info(v2d_ctx *ctx, int64_t *out, double *flops)
+{
+    Ctx *c = ctx->c.get();
+    if (ensure_mf_pattern(c)) return -1;
+    const MfPattern &P = *c->mfp;
+    const int64_t v[8] = {P.nfront, P.nheight, P.nbig, P.F_elems, P.V_elems, P.pool_elems, P.nvals, P.npos};
+    for (int q = 0; q < 8; ++q) out[q] = v[q];
+    if (flops) *flops = P.flops;
+    return 0;
+}
+
+int v2d_ctx_mf_fronts(v2d_ctx *ctx, double *F)
+{
+    Ctx *c = ctx->c.get();
+    if (c->solver != V2D_SOLVER_MULTIFRONTAL || !c->factored || !c->mfp) {
+        set_error("mf_fronts: no multifrontal factorisation");
+        return -1;
+    }
+    const int64_t n = c->mfp->F_elems;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    if (c->prec == V2D_FP64) {
+        V2D_HIP(hipMemcpyAsync(F, c->md.F, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+        V2D_HIP(hipStreamSynchronize(c->s));
+        return 0;
+    }
+    std::vector<float> t(n);
+    V2D_HIP(hipMemcpyAsync(t.data(), c->mw.F, sizeof(float) * n, hipMemcpyDeviceToHost, c->s));
+    V2D_HIP(hipStreamSynchronize(c->s));
+    for (int64_t q = 0; q < n; ++q) F[q] = t[q];
+    return 0;
+}
+
+int v2d_mf_analyse(int imax, int jmax, const uint8_t *ff_flags, v2d_mf_analysis **out)
+{
+    *out = nullptr;
+    if (check_grid(imax, jmax)) return -1;
+    auto a = std::make_unique<v2d_mf_analysis>();
+    std::vector<uint8_t> fl(imax, 0);
+    if (ff_flags) std::copy(ff_flags, ff_flags + imax, fl.begin());
+    if (mf_analyse(a->p, imax, jmax, fl.data())) return -1;
+    *out = a.release();
+    return 0;
+}
+
+int v2d_mf_sizes(const v2d_mf_analysis *a, int64_t *out, double *flops)
+{
+    const MfPattern &P = a->p;
+    const int64_t v[16] = {P.nfront, P.nheight, P.nbig, P.F_elems, P.V_elems, P.Z_elems, P.nfu, P.nmap,
+                           P.nch, P.npos, P.nvals, P.nrows, 0, 0, 0, 0};
+    for (int q = 0; q < 16; ++q) out[q] = v[q];
+    if (flops) *flops = P.flops;
+    return 0;
+}
+
+int v2d_mf_export(const v2d_mf_analysis *a, int64_t *fronts, int *fu, int *ch, int *map, int64_t *aoff,
+                  int *aseg, int *avid)
+{
+    const MfPattern &P = a->p;
+    if (fronts) {
+        for (int h = 0; h < P.nheight; ++h)
+            for (int f = P.lv[h].f0; f < P.lv[h].f1; ++f) {
+                const MfFront &F = P.fr[f];
+                const int64_t r[14] = {F.fo, F.vo, F.zo, F.uo, F.mo, F.ns, F.nsp, F.nb, F.nf, F.ld, F.nch,
+                                       F.co, F.big, h};
+                std::copy(r, r + 14, fronts + 14 * (int64_t)f);
+            }
+    }
+    if (fu) std::copy(P.fu.begin(), P.fu.end(), fu);
+    if (ch) std::copy(P.ch.begin(), P.ch.end(), ch);
+    if (map) std::copy(P.map.begin(), P.map.end(), map);
+    if (aoff) std::copy(P.aoff.begin(), P.aoff.end(), aoff);
+    if (aseg) std::copy(P.aseg.begin(), P.aseg.end(), aseg);
+    if (avid) std::copy(P.avid.begin(), P.avid.end(), avid);
+    return 0;
+}
+
+int v2d_mf_destroy(v2d_mf_analysis *a)
+{
+    delete a;
+    return 0;
+}
+
 int v2d_ctx_set_update_kernel(v2d_ctx *ctx, int kind)
 {
     if (kind != V2D_UPDATE_VALU64 && kind != V2D_UPDATE_VALU128 && kind != V2D_UPDATE_MFMA64) {
@@ -1000,10 +1173,17 @@ int ctx_factor(Ctx *c)
     // so a stored factorisation stays consistent after later assemblies.
     const size_t tsz = c->prec == V2D_FP64 ? sizeof(double) : sizeof(float);
     V2D_HIP(hipMemcpyAsync(c->val_pc, c->val, (size_t)kSlots * c->N * tsz, hipMemcpyDeviceToDevice, c->s));
-    if (c->prec == V2D_FP64)
-        rc = direct_factor<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->s);
-    else
-        rc = direct_factor<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), c->s);
+    if (c->solver == V2D_SOLVER_MULTIFRONTAL) {
+        if (ensure_mf(c)) return -1;
+        rc = c->prec == V2D_FP64
+                 ? mf_factor<double>(*c->mfp, c->md, static_cast<const double *>(c->val_pc), c->s)
+                 : mf_factor<float>(*c->mfp, c->mw, static_cast<const float *>(c->val_pc), c->s);
+    } else {
+        if (ensure_block_lu(c)) return -1;
+        rc = c->prec == V2D_FP64
+                 ? direct_factor<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->s)
+                 : direct_factor<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), c->s);
+    }
     if (rc) return rc;
     V2D_HIP(hipEventRecord(c->ev[5], c->s));
     c->fac_in_step = true;
@@ -1038,9 +1218,7 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
         V2D_ALLOC_S(db, sizeof(double) * n, c->s);
         V2D_ALLOC_S(dx, sizeof(double) * n, c->s);
         V2D_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
-        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), db.as<double>(),
-                                 dx.as<double>(), c->s))
-            return -1;
+        if (lin_solve<double>(c, db.as<double>(), dx.as<double>(), false)) return -1;
         V2D_HIP(hipStreamSynchronize(c->s));
         if (sweep_failed(c->wd.err_host)) return -1;
         V2D_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
@@ -1053,9 +1231,7 @@ int v2d_ctx_solve(v2d_ctx *ctx, const double *b, double *x)
     V2D_ALLOC_S(db, sizeof(float) * n, c->s);
     V2D_ALLOC_S(dx, sizeof(float) * n, c->s);
     V2D_HIP(hipMemcpyAsync(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->s));
-    if (direct_solve<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), db.as<float>(),
-                            dx.as<float>(), c->s))
-        return -1;
+    if (lin_solve<float>(c, db.as<float>(), dx.as<float>(), false)) return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
     if (sweep_failed(c->wf.err_host)) return -1;
     V2D_HIP(hipMemcpyAsync(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->s));
@@ -1116,8 +1292,7 @@ int v2d_ctx_newton_step(v2d_ctx *ctx, double re, double invdtau, double inv2dt, 
     if (c->prec == V2D_FP64) {
         if (ctx_factor(c)) return -1;
         V2D_HIP(hipEventRecord(c->ev[2], c->s));
-        if (direct_solve<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->rhs, c->delta, c->s))
-            return -1;
+        if (lin_solve<double>(c, c->rhs, c->delta, false)) return -1;
     } else {
         // Factorisation policy: refactor_every N > 0 refactors every N steps;
         // N = 0 reuses the last FP32 block LU while GMRES reaches its tolerance
@@ -1296,9 +1471,7 @@ int v2d_ctx_solve_transpose(v2d_ctx *ctx, const double *b, double *x)
         V2D_ALLOC_S(db, sizeof(double) * n, c->s);
         V2D_ALLOC_S(dx, sizeof(double) * n, c->s);
         V2D_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
-        if (direct_solve_t<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), db.as<double>(),
-                                   dx.as<double>(), c->s))
-            return -1;
+        if (lin_solve<double>(c, db.as<double>(), dx.as<double>(), true)) return -1;
         V2D_HIP(hipStreamSynchronize(c->s));
         V2D_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
         V2D_HIP(hipStreamSynchronize(c->s));
@@ -1310,9 +1483,7 @@ int v2d_ctx_solve_transpose(v2d_ctx *ctx, const double *b, double *x)
     V2D_ALLOC_S(db, sizeof(float) * n, c->s);
     V2D_ALLOC_S(dx, sizeof(float) * n, c->s);
     V2D_HIP(hipMemcpyAsync(db.p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->s));
-    if (direct_solve_t<float>(c->pat, c->wf, static_cast<const float *>(c->val_pc), db.as<float>(),
-                              dx.as<float>(), c->s))
-        return -1;
+    if (lin_solve<float>(c, db.as<float>(), dx.as<float>(), true)) return -1;
     V2D_HIP(hipStreamSynchronize(c->s));
     V2D_HIP(hipMemcpyAsync(xf.data(), dx.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->s));
     V2D_HIP(hipStreamSynchronize(c->s));
@@ -1480,9 +1651,7 @@ int v2d_ctx_adjoint_bdf2_step(v2d_ctx *ctx, double re, double inv2dt, double uro
                            c->adj_l1, c->adj_l2, c->adj_rhs);
         V2D_LAUNCH();
     }
-    if (direct_solve_t<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), c->adj_rhs, c->adj_lam,
-                               c->s))
-        return -1;
+    if (lin_solve<double>(c, c->adj_rhs, c->adj_lam, true)) return -1;
     double rs[2];
     if (residual_vjp_dev(c, re, inv2dt, urot, c->adj_lam, c->adj_gm, rs)) return -1;
     c->adj_gs[0] += rs[0];
@@ -1544,9 +1713,7 @@ int v2d_ctx_adjoint_forces(v2d_ctx *ctx, double re, double urot, double ref_leng
     double f3[3];
     // dJ/dU lives in the omega half of the field-blocked vector
     if (forces_vjp_dev(c, re, ref_length, wd, wl, gu.as<double>() + N, gm.as<double>(), f3)) return -1;
-    if (direct_solve_t<double>(c->pat, c->wd, static_cast<const double *>(c->val_pc), gu.as<double>(),
-                               lam.as<double>(), c->s))
-        return -1;
+    if (lin_solve<double>(c, gu.as<double>(), lam.as<double>(), true)) return -1;
     double rs[2];
     if (residual_vjp_dev(c, re, 0.0, urot, lam.as<double>(), gm.as<double>(), rs)) return -1;
     if (metrics_vjp_dev(c->imax, c->jmax, c->dksi, c->deta, c->x, c->y, gm.as<double>(), gx.as<double>(),

@@ -56,6 +56,20 @@ struct Ctx {
     DirectPattern pat;
     DirectWork<double> wd;
     DirectWork<float> wf;
+    // linear solver of the Newton systems (V2D_SOLVER_*); the storage of each
+    // solver is allocated at its first factorisation
+    int solver = V2D_SOLVER_BLOCK_LU;
+    bool blu_ready = false;
+    std::unique_ptr<MfPattern> mfp;   // multifrontal analysis, device tables uploaded
+    MfWork<double> md;
+    MfWork<float> mw;
+    void drop_mf()
+    {
+        mf_free(md, s);
+        mf_free(mw, s);
+        if (mfp) mf_free_pattern(*mfp, s);
+        mfp.reset();
+    }
     // GMRES (mixed) work, FP32 vectors
     int gm_cap = 30;   // allocated Krylov vectors
     int gm_max = 30;
@@ -124,6 +138,7 @@ struct Ctx {
         drop_graph();
         for (void *p : allocs) (void)hipFreeAsync(p, s);
         direct_free_pattern(pat, s);
+        drop_mf();
         if (s) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
@@ -147,6 +162,10 @@ struct Ctx {
 int ctx_factor(Ctx *c);
 int ctx_assemble(Ctx *c, double re, double invdtau, double inv2dt, double urot);
 int apply_pc(Ctx *c, const float *v, float *z);
+// x = A^-1 b (trans: A^-T b) with the stored factorisation of the context's
+// linear solver, on the context's stream; b and x distinct.
+template <typename T>
+int lin_solve(Ctx *c, const T *b, T *x, bool trans);
 int sweep_failed(const int *err_host);
 // GMRES tolerance of the Newton step whose residual norm is bn (Eisenstat-Walker
 // forcing with the residual-target safeguard); updates the forcing history.

@@ -315,20 +315,31 @@ void direct_free_pattern(DirectPattern &P, hipStream_t s)
 // Both elimination chains advance together: every launch below carries up to
 // two level jobs and selects its job with blockIdx.z.
 template <typename T>
-struct Job {
-    T *S;                  // dense Schur complement -> its inverse (n x ld, row-major)
-    int n, ld, l;          // size, row stride, level
-    int64_t voff;          // offset of the level in level-ordered vectors / CSR rows
-    T *colk, *dinv, *tmat; // per-job scratch
-    T *rowk;               // transposed pivot-row panel (MFMA update only, else null)
-    T *colk_w, *rowk_w;    // panels of the next step (MFMA dataflow step only)
-};
-
-template <typename T>
 struct Batch {
     Job<T> j[2];
     int nj;
     int kind;   // Gauss-Jordan update kernel (UpdateKind)
+    __device__ __forceinline__ const Job<T> &job(int z) const { return j[z]; }
+};
+
+// Many jobs from a device table (the fronts of one elimination-tree level);
+// par = 1 swaps the read and write panel buffers of every job (the dataflow
+// steps alternate them).
+template <typename T>
+struct TableBatch {
+    const Job<T> *jt;
+    int nj;
+    int kind;
+    int par;
+    __device__ __forceinline__ Job<T> job(int z) const
+    {
+        Job<T> J = jt[z];
+        if (par) {
+            T *t = J.colk; J.colk = J.colk_w; J.colk_w = t;
+            t = J.rowk; J.rowk = J.rowk_w; J.rowk_w = t;
+        }
+        return J;
+    }
 };
 
 // ------------------------------------------------------- level build ------
@@ -680,15 +691,15 @@ __device__ void gj_tile(T *sa, int lda, int kb, T (&a)[16], T *out)
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_gj_diag_b(Batch<T> B, int k0)
+template <typename T, typename BT = Batch<T>>
+__global__ void __launch_bounds__(256) k_gj_diag_b(BT B, int k0)
 {
     constexpr int LT = NB + 4;   // padded LDS row stride
     __shared__ T sa[NB * LT];
-    const Job<T> &J = B.j[blockIdx.z];
-    const int n = J.n, ld = J.ld;
-    if (k0 >= n) return;
-    const int kb = min(NB, n - k0);
+    const Job<T> &J = B.job(blockIdx.z);
+    const int np = J.np, ld = J.ld;
+    if (k0 >= np) return;
+    const int kb = min(NB, np - k0);
     const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
     T a[16];
 #pragma unroll
@@ -709,16 +720,16 @@ template <> struct Vec8<double> { using type = double2; static constexpr int w =
 //                  critical path of every Gauss-Jordan step)
 // blockIdx.y == 1: colk[r][t] <- A[r][k0+t] for a 16-row tile of rows outside K
 constexpr int RP = 16;
-template <typename T>
-__global__ void __launch_bounds__(256) k_gj_rowpanel_b(Batch<T> B, int k0)
+template <typename T, typename BT = Batch<T>>
+__global__ void __launch_bounds__(256) k_gj_rowpanel_b(BT B, int k0)
 {
     __shared__ T sdT[NB * NB];   // sdT[s][t] = Dinv[t][s]
     __shared__ T sp[NB * RP];    // sp[s][c]  = A[k0+s][base+c]
-    const Job<T> &J = B.j[blockIdx.z];
+    const Job<T> &J = B.job(blockIdx.z);
     T *__restrict__ A = J.S;
     const int n = J.n, ld = J.ld;
-    if (k0 >= n) return;
-    const int kb = min(NB, n - k0);
+    if (k0 >= J.np) return;
+    const int kb = min(NB, J.np - k0);
     const int base = blockIdx.x * RP;
     if (base >= n) return;
     if (blockIdx.y == 1) {
@@ -961,8 +972,8 @@ template <> struct Mfma<double> {
 // The pivot-row work thus overlaps the bulk update of the step.
 constexpr int kSpinLimit = 1 << 21;
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int epoch,
+template <typename T, typename BT = Batch<T>>
+__global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
                                                         int *__restrict__ flags, int *__restrict__ err)
 {
     using M = Mfma<T>;
@@ -972,16 +983,16 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
     constexpr int LT = NB + 4;    // padded LDS row stride
     __shared__ T sT[NB * LT];     // next pivot block, or a tile stored transposed
     const int z = blockIdx.x % B.nj;
-    const Job<T> &J = B.j[z];
+    const Job<T> &J = B.job(z);
     T *__restrict__ A = J.S;
-    const int n = J.n, ld = J.ld;
-    if (k0 >= n) return;
-    const int kb = min(NB, n - k0);
+    const int n = J.n, ld = J.ld, np = J.np;
+    if (k0 >= np) return;
+    const int kb = min(NB, np - k0);
     const int nt = (n + NB - 1) / NB;
     int lin = blockIdx.x / B.nj;
     if (lin >= nt * nt) return;
     const int k1 = k0 + NB;
-    if (k1 < n) {
+    if (k1 < np) {
         const int piv = (k1 / NB) * nt + k1 / NB;
         if (lin == 0) lin = piv;
         else if (lin <= piv) lin -= 1;
@@ -990,17 +1001,17 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
     if (i0 == k0) {
         // the current pivot rows are not updated; their part of column tile
         // k1 is still the next column panel's rows k0 .. k0 + 63
-        if (k1 < n && c0 == k1) {
+        if (k1 < np && c0 == k1) {
             for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
                 const int r = idx / NB, t = idx % NB;
-                J.colk_w[(int64_t)(k0 + r) * NB + t] = (k1 + t < n) ? A[(int64_t)(k0 + r) * ld + k1 + t] : T(0);
+                J.colk_w[(int64_t)(k0 + r) * NB + t] = (k1 + t < np) ? A[(int64_t)(k0 + r) * ld + k1 + t] : T(0);
             }
         }
         return;
     }
-    const bool is_piv = k1 < n && i0 == k1 && c0 == k1;
-    const bool is_prow = k1 < n && i0 == k1 && c0 != k1;
-    const bool is_pcol = k1 < n && c0 == k1 && i0 != k1;
+    const bool is_piv = k1 < np && i0 == k1 && c0 == k1;
+    const bool is_prow = k1 < np && i0 == k1 && c0 != k1;
+    const bool is_pcol = k1 < np && c0 == k1 && i0 != k1;
     // the pivot chain (next pivot inverse, then the next pivot rows) bounds the
     // step: its waves take issue priority over the bulk update sharing the SIMDs
     if (is_piv || is_prow) __builtin_amdgcn_s_setprio(3);
@@ -1108,7 +1119,7 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(Batch<T> B, int k0, int 
             }
         return;
     }
-    const int kb2 = min(NB, n - k1);
+    const int kb2 = min(NB, np - k1);
     if (is_piv) {
         // ------------------------------------- next pivot block inverse ----
 #pragma unroll
@@ -1306,6 +1317,34 @@ static int gj_invert_b(const Batch<T> &B, DirectWork<T> &W, LaunchProbe *probe, 
     return 0;
 }
 
+// Gauss-Jordan sweeps of a device table of jobs (MFMA dataflow steps): every
+// job's pivots [0, np) are eliminated from its n x n matrix in place, np a
+// multiple of 64 at most npmax, n at most nmax.  flags holds one word per job.
+template <typename T>
+int gj_sweep_table(const Job<T> *jobs, int nj, int npmax, int nmax, int *flags, int *err, int &epoch,
+                   hipStream_t s)
+{
+    if (nj <= 0 || npmax <= 0) return 0;
+    if (nj > 65535) {
+        set_error("Gauss-Jordan table: more than 65535 jobs in one launch");
+        return -1;
+    }
+    const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
+    const unsigned npr = (unsigned)((nmax + RP - 1) / RP);
+    TableBatch<T> B{jobs, nj, kUpdateMfma64, 0};
+    hipLaunchKernelGGL((k_gj_diag_b<T, TableBatch<T>>), dim3(1, 1, (unsigned)nj), dim3(256), 0, s, B, 0);
+    hipLaunchKernelGGL((k_gj_rowpanel_b<T, TableBatch<T>>), dim3(npr, 2, (unsigned)nj), dim3(256), 0, s, B, 0);
+    for (int k0 = 0; k0 < npmax; k0 += NB) {
+        B.par = (k0 / NB) & 1;
+        hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>>), dim3(nt * nt * (unsigned)nj), dim3(256), 0, s,
+                           B, k0, ++epoch, flags, err);
+    }
+    V2D_LAUNCH();
+    return 0;
+}
+template int gj_sweep_table<float>(const Job<float> *, int, int, int, int *, int *, int &, hipStream_t);
+template int gj_sweep_table<double>(const Job<double> *, int, int, int, int *, int *, int &, hipStream_t);
+
 // ------------------------------------------------------------ factor ------
 // Twisted block factorisation: the bottom chain eliminates levels 0..M-1 from
 // the wall outward (S_l = D_l - L_l S_{l-1}^-1 U_{l-1}), the top chain levels
@@ -1317,6 +1356,7 @@ static Job<T> make_job(const DirectPattern &P, DirectWork<T> &W, int l, int slot
     Job<T> J;
     J.S = W.sinv + P.soff[l];
     J.n = P.n[l];
+    J.np = J.n;
     J.ld = P.ld[l];
     J.l = l;
     J.voff = P.voff[l];

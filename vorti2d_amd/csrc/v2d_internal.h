@@ -290,6 +290,27 @@ struct DirectWork {
     int64_t sweep_fallbacks = 0;        // persistent solves redone with launch pairs
 };
 
+// One dense Gauss-Jordan job: the matrix S (n x n, row stride ld) has its
+// pivots [0, np) eliminated in place (np = n: S becomes its inverse; np < n, a
+// multiple of 64: a partial sweep, S = [A B; C D] becoming
+// [A^-1, A^-1 B; -C A^-1, D - C A^-1 B]).
+template <typename T>
+struct Job {
+    T *S;                  // dense matrix (n x ld, row-major), swept in place
+    int n, ld, l;          // size, row stride, level (block LU only)
+    int np;                // pivots
+    int64_t voff;          // offset of the level in level-ordered vectors (block LU only)
+    T *colk, *dinv, *tmat; // per-job scratch (tmat: block LU only)
+    T *rowk;               // transposed pivot-row panel (MFMA update only, else null)
+    T *colk_w, *rowk_w;    // panels of the next step (MFMA dataflow step only)
+};
+
+// Sweeps of a device table of jobs with the MFMA dataflow steps (colk, colk_w:
+// n x 64; rowk, rowk_w: 64-rounded n x 64; dinv 64 x 64 per job).
+template <typename T>
+int gj_sweep_table(const Job<T> *jobs, int nj, int npmax, int nmax, int *flags, int *err, int &epoch,
+                   hipStream_t s);
+
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
 template <typename T>
@@ -312,5 +333,89 @@ __global__ void k_metrics_vjp_1(int imax, int jmax, double dksi, double deta, co
                                 double *g_eta);
 __global__ void k_metrics_vjp_2(int imax, int jmax, double deta, const double *g_eta, double *g_x,
                                 double *g_y);
+
+// ------------------------------------------- multifrontal direct solver
+// Nested-dissection multifrontal LU of the Newton matrix (the structure of the
+// reference's MUMPS factorisation, vorti2d/petsc_solver.py:37-62).  The node
+// graph is dissected by grid lines; every separator is one front, a dense
+// (nf x nf) matrix over its own unknowns (ns pivots, padded with identity to
+// nsp) and the unknowns of its ancestors it couples to (nb).  A partial
+// Gauss-Jordan sweep of the nsp pivots leaves [G W; X C] in the front:
+// G = F11^-1, W = G F12, X = -F21 G, C = F22 - F21 G F12, the contribution
+// block the parent gathers.  Fronts are processed by height in the
+// elimination tree, one batch of launches per height.
+struct MfFront {
+    int64_t fo;    // front matrix (elements), nf rows of stride ld
+    int64_t vo;    // front vector, nf entries
+    int64_t zo;    // pivot-part vector, nsp entries
+    int64_t uo;    // unknowns of the front rows (field-blocked global index, -1: padding)
+    int64_t mo;    // child maps: nch arrays of nf entries (child's struct row or -1)
+    int ns, nsp, nb, nf, ld;
+    int nch, co;   // children: ch[co .. co + nch)
+    int big;       // 1: swept by the dataflow Gauss-Jordan steps (nsp a multiple of 64)
+};
+
+struct MfLevel {
+    int f0, f1, fbig;        // fronts [f0, f1); [fbig, f1) are big
+    int64_t r0, r1;          // rows table range (every front row)
+    int64_t g0, g1;          // forward row blocks (kMfRows rows of all front rows)
+    int64_t b0, b1;          // backward row blocks (pivot rows)
+    int64_t c0, c1;          // transposed column blocks (64 columns of all front columns)
+    int64_t p0, p1;          // transposed column blocks of the pivot columns
+    int64_t a0, a1;          // assembly positions
+    int nfmax_small, nfmax_big, npmax_big;
+    int64_t pool;            // Gauss-Jordan panel elements of the big fronts
+    int64_t job0;            // first job of the level in the job table
+};
+
+constexpr int kMfRows = 16;   // rows of one forward / backward row block
+
+struct MfPattern {
+    int64_t N = 0;
+    int imax = 0, jmax = 0;
+    int nfront = 0, nheight = 0, nbig = 0;
+    std::vector<MfFront> fr;
+    std::vector<MfLevel> lv;
+    int64_t F_elems = 0, V_elems = 0, Z_elems = 0, pool_elems = 0;
+    int64_t nrows = 0, ngemv = 0, nbwd = 0, ncolt = 0, npcolt = 0, npos = 0, nvals = 0;
+    int64_t nfu = 0, nmap = 0, nch = 0;
+    double flops = 0.0;      // floating-point operations of one factorisation
+    // host copies (kept for export)
+    std::vector<int> fu, ch, map;
+    std::vector<int64_t> aoff;
+    std::vector<int> aseg, avid;
+    std::vector<int2> rows, gemv, bwd, colt, pcolt;
+    // device copies
+    MfFront *d_fr = nullptr;
+    int *d_fu = nullptr, *d_ch = nullptr, *d_map = nullptr;
+    int2 *d_rows = nullptr, *d_gemv = nullptr, *d_bwd = nullptr, *d_colt = nullptr, *d_pcolt = nullptr;
+    int64_t *d_aoff = nullptr;
+    int *d_aseg = nullptr, *d_avid = nullptr;
+    bool on_device = false;
+};
+
+// Host analysis only (no device call).
+int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags);
+int mf_upload(MfPattern &P, hipStream_t s);
+void mf_free_pattern(MfPattern &P, hipStream_t s);
+
+template <typename T>
+struct MfWork {
+    T *F = nullptr, *Vf = nullptr, *Zf = nullptr, *pool = nullptr;
+    Job<T> *jobs = nullptr;   // Gauss-Jordan jobs of the big fronts, level by level
+    int *flags = nullptr, *err = nullptr;
+    int epoch = 0;
+    bool factored = false;
+};
+template <typename T>
+int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s);
+template <typename T>
+void mf_free(MfWork<T> &W, hipStream_t s);
+// val: the Jacobian in the slot layout (kSlots planes of N).
+template <typename T>
+int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s);
+// x = A^-1 b (trans: A^-T b); b and x distinct 2N vectors.
+template <typename T>
+int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hipStream_t s);
 
 }  // namespace v2d

@@ -1,0 +1,920 @@
+// v2d_mf.hip -- nested-dissection multifrontal LU of the Newton matrix.
+//
+// The reference factors the assembled Newton matrix with MUMPS, a multifrontal
+// LU (vorti2d/petsc_solver.py:37-62: KSP PREONLY, PC LU, factor solver
+// "mumps"; vorti2d/dist_solver.py:133-137; the default linsolve="mumps" of
+// vorti2d/config.py:74-77).  This is the device counterpart, designed for the
+// structured O-grid:
+//
+//   * analysis (host, once per grid): the node graph of the matrix (psi and
+//     omega of a node share a vertex) is dissected recursively by grid lines:
+//     a node set is split at the median of its longer extent (i periodic, the
+//     two seam copies sharing i = 0), the separator is the smaller of the two
+//     sides' boundary nodes, the rest splits into connected components, down to
+//     leaves of kMfLeaf nodes.  Every separator is one front; its rows are its
+//     own unknowns (pivots) and the ancestor unknowns it couples to, found by
+//     the usual symbolic pass (a front's coupling set is its separator's graph
+//     neighbours above it and its children's coupling sets, less itself).
+//   * factorisation (device, per Jacobian): height by height in the
+//     elimination tree, every front of a height is assembled -- its children's
+//     contribution blocks gathered (fixed child order, so the sums are
+//     deterministic), then the Jacobian entries it owns added -- and swept by
+//     partial Gauss-Jordan on its pivots: small fronts in LDS, one workgroup
+//     each; large ones by the MFMA dataflow steps of the block LU, as one job
+//     table per height.  The front then holds [G W; X C] (G = F11^-1,
+//     W = G F12, X = -F21 G, C = F22 - F21 G F12).
+//   * solve: forward by height (front vector = b on the pivots + the
+//     children's contribution vectors; z = G v_s; v_b += X v_s), backward from
+//     the root (x_s = z - W x_b).  The transposed solve uses the same fronts
+//     (the sweep of A^T is the transpose of A's: z = G^T v_s, v_b -= W^T v_s,
+//     x_s = z + X^T x_b).
+//
+// No pivoting, as in the block-tridiagonal LU: the fronts' pivot blocks are
+// Schur complements of the same diagonally weighted Newton matrix.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "v2d_internal.h"
+
+namespace v2d {
+
+namespace {
+
+constexpr int kMfLeaf = 32;      // most nodes of a leaf supernode
+constexpr int kMfSmall = 136;    // largest front swept in LDS by one workgroup
+constexpr int kMfAlign = 4;      // row stride and pivot padding of small fronts
+constexpr int kNB = 64;          // Gauss-Jordan block of the large fronts
+
+__host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Symmetrised node graph of the Jacobian pattern (CSR, no self loops).
+struct NodeGraph {
+    std::vector<int64_t> ptr;
+    std::vector<int> adj;
+};
+
+void node_graph(int imax, int jmax, const uint8_t *flags, NodeGraph &G)
+{
+    const int64_t N = (int64_t)imax * jmax;
+    std::vector<int64_t> cnt(N + 1, 0);
+    auto visit = [&](auto &&edge) {
+        int64_t cols[13];
+        for (int comp = 0; comp < 2; ++comp)
+            for (int64_t k = 0; k < N; ++k) {
+                const int j = (int)(k / imax), i = (int)(k % imax);
+                const uint8_t fl = j == jmax - 1 ? flags[i] : 0;
+                const int c = row_columns(imax, jmax, i, j, comp, fl, N, cols);
+                for (int q = 0; q < c; ++q) {
+                    const int64_t kc = cols[q] >= N ? cols[q] - N : cols[q];
+                    if (kc != k) edge(k, kc);
+                }
+            }
+    };
+    visit([&](int64_t a, int64_t b) { ++cnt[a + 1]; ++cnt[b + 1]; });
+    for (int64_t k = 0; k < N; ++k) cnt[k + 1] += cnt[k];
+    std::vector<int> raw(cnt[N]);
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    visit([&](int64_t a, int64_t b) { raw[fill[a]++] = (int)b; raw[fill[b]++] = (int)a; });
+    G.ptr.assign(N + 1, 0);
+    G.adj.clear();
+    G.adj.reserve(raw.size() / 2);
+    for (int64_t k = 0; k < N; ++k) {
+        auto b = raw.begin() + cnt[k], e = raw.begin() + cnt[k + 1];
+        std::sort(b, e);
+        e = std::unique(b, e);
+        G.adj.insert(G.adj.end(), b, e);
+        G.ptr[k + 1] = (int64_t)G.adj.size();
+    }
+}
+
+// Recursive dissection; supernodes are created children first (a postorder).
+struct Dissect {
+    const NodeGraph &G;
+    std::vector<int> ic, jc;
+    std::vector<int> stamp, tag;
+    int cur = 0;
+    std::vector<std::vector<int>> sep, kids;
+
+    Dissect(const NodeGraph &g, int imax, int jmax) : G(g)
+    {
+        const int64_t N = (int64_t)imax * jmax;
+        ic.resize(N);
+        jc.resize(N);
+        for (int64_t k = 0; k < N; ++k) {
+            const int i = (int)(k % imax);
+            ic[k] = i == imax - 1 ? 0 : i;
+            jc[k] = (int)(k / imax);
+        }
+        stamp.assign(N, 0);
+        tag.assign(N, 0);
+    }
+
+    int leaf(std::vector<int> &S)
+    {
+        std::sort(S.begin(), S.end());
+        sep.push_back(std::move(S));
+        kids.emplace_back();
+        return (int)sep.size() - 1;
+    }
+
+    std::vector<int> rec(std::vector<int> S)
+    {
+        if ((int)S.size() <= kMfLeaf) return {leaf(S)};
+        int ext[2];
+        for (int ax = 0; ax < 2; ++ax) {
+            const std::vector<int> &co = ax ? jc : ic;
+            int lo = 1 << 30, hi = -1;
+            for (int v : S) { lo = std::min(lo, co[v]); hi = std::max(hi, co[v]); }
+            ext[ax] = hi - lo + 1;
+        }
+        const int axes[2] = {ext[0] > ext[1] ? 0 : 1, ext[0] > ext[1] ? 1 : 0};
+        std::vector<int> best;
+        bool found = false;
+        for (int ax : axes) {
+            const std::vector<int> &co = ax ? jc : ic;
+            std::vector<int> c(S.size());
+            for (size_t q = 0; q < S.size(); ++q) c[q] = co[S[q]];
+            std::nth_element(c.begin(), c.begin() + c.size() / 2, c.end());
+            const int med = c[c.size() / 2];
+            ++cur;
+            size_t na = 0;
+            for (int v : S) {
+                stamp[v] = cur;
+                tag[v] = co[v] < med ? 1 : 2;
+                na += co[v] < med;
+            }
+            if (na == 0 || na == S.size()) continue;
+            std::vector<int> sa, sb;
+            for (int v : S) {
+                const int mine = tag[v], other = 3 - mine;
+                for (int64_t e = G.ptr[v]; e < G.ptr[v + 1]; ++e) {
+                    const int u = G.adj[e];
+                    if (stamp[u] == cur && tag[u] == other) {
+                        (mine == 1 ? sa : sb).push_back(v);
+                        break;
+                    }
+                }
+            }
+            best = sa.size() <= sb.size() ? std::move(sa) : std::move(sb);
+            found = true;
+            break;
+        }
+        if (!found) return {leaf(S)};
+        // the rest, split into connected components (discovery order)
+        ++cur;
+        for (int v : S) { stamp[v] = cur; tag[v] = 0; }
+        for (int v : best) tag[v] = 3;
+        std::vector<std::vector<int>> comps;
+        std::vector<int> queue;
+        for (int v0 : S) {
+            if (tag[v0] != 0) continue;
+            tag[v0] = 4;
+            queue.assign(1, v0);
+            for (size_t h = 0; h < queue.size(); ++h) {
+                const int v = queue[h];
+                for (int64_t e = G.ptr[v]; e < G.ptr[v + 1]; ++e) {
+                    const int u = G.adj[e];
+                    if (stamp[u] == cur && tag[u] == 0) {
+                        tag[u] = 4;
+                        queue.push_back(u);
+                    }
+                }
+            }
+            comps.push_back(queue);
+        }
+        S.clear();
+        S.shrink_to_fit();
+        std::vector<int> ch;
+        for (auto &cpt : comps) {
+            std::vector<int> r = rec(std::move(cpt));
+            ch.insert(ch.end(), r.begin(), r.end());
+        }
+        if (best.empty()) return ch;
+        std::sort(best.begin(), best.end());
+        sep.push_back(std::move(best));
+        kids.push_back(std::move(ch));
+        return {(int)sep.size() - 1};
+    }
+};
+
+}  // namespace
+
+int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
+{
+    if (imax < 4 || jmax < 4) {
+        set_error("multifrontal: grid smaller than 4 x 4");
+        return -1;
+    }
+    const int64_t N = (int64_t)imax * jmax;
+    if (2 * N >= (int64_t)1 << 31) {
+        set_error("multifrontal: more than 2^31 unknowns");
+        return -1;
+    }
+    P = MfPattern{};
+    P.N = N;
+    P.imax = imax;
+    P.jmax = jmax;
+    NodeGraph G;
+    node_graph(imax, jmax, flags, G);
+    Dissect D(G, imax, jmax);
+    {
+        std::vector<int> all(N);
+        std::iota(all.begin(), all.end(), 0);
+        (void)D.rec(std::move(all));
+    }
+    const int ns_ = (int)D.sep.size();
+    std::vector<int> owner(N, -1), pos(N, -1);
+    for (int s = 0; s < ns_; ++s)
+        for (size_t q = 0; q < D.sep[s].size(); ++q) {
+            const int v = D.sep[s][q];
+            if (owner[v] >= 0) {
+                set_error("multifrontal: a node in two separators");
+                return -1;
+            }
+            owner[v] = s;
+            pos[v] = (int)q;
+        }
+    for (int64_t v = 0; v < N; ++v)
+        if (owner[v] < 0) {
+            set_error("multifrontal: a node in no separator");
+            return -1;
+        }
+    // heights and coupling sets (children come before parents)
+    std::vector<int> height(ns_, 0);
+    std::vector<std::vector<int>> st(ns_);
+    {
+        std::vector<int> mark(N, -1);
+        for (int s = 0; s < ns_; ++s) {
+            for (int c : D.kids[s]) height[s] = std::max(height[s], height[c] + 1);
+            std::vector<int> &L = st[s];
+            for (int v : D.sep[s])
+                for (int64_t e = G.ptr[v]; e < G.ptr[v + 1]; ++e) {
+                    const int u = G.adj[e];
+                    if (owner[u] > s && mark[u] != s) { mark[u] = s; L.push_back(u); }
+                }
+            for (int c : D.kids[s])
+                for (int u : st[c])
+                    if (owner[u] != s && mark[u] != s) { mark[u] = s; L.push_back(u); }
+            std::sort(L.begin(), L.end(), [&](int a, int b) {
+                return owner[a] != owner[b] ? owner[a] < owner[b] : pos[a] < pos[b];
+            });
+        }
+    }
+    // fronts ordered by height (stable in creation order); small before big
+    // within a height
+    std::vector<int> order(ns_);
+    std::iota(order.begin(), order.end(), 0);
+    auto is_big = [&](int s) {
+        return round_up(2 * (int)D.sep[s].size(), kMfAlign) + 2 * (int)st[s].size() > kMfSmall;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (height[a] != height[b]) return height[a] < height[b];
+        return (int)is_big(a) < (int)is_big(b);
+    });
+    std::vector<int> fid(ns_);
+    for (int f = 0; f < ns_; ++f) fid[order[f]] = f;
+    P.nfront = ns_;
+    P.nheight = ns_ ? height[order[ns_ - 1]] + 1 : 0;
+    P.fr.resize(ns_);
+    int64_t fo = 0, vo = 0, zo = 0, uo = 0, mo = 0;
+    for (int f = 0; f < ns_; ++f) {
+        const int s = order[f];
+        MfFront &F = P.fr[f];
+        F.ns = 2 * (int)D.sep[s].size();
+        F.nb = 2 * (int)st[s].size();
+        F.big = is_big(s) ? 1 : 0;
+        F.nsp = round_up(F.ns, F.big ? kNB : kMfAlign);
+        F.nf = F.nsp + F.nb;
+        F.ld = round_up(F.nf, kMfAlign);
+        F.nch = (int)D.kids[s].size();
+        F.co = (int)P.ch.size();
+        for (int c : D.kids[s]) P.ch.push_back(fid[c]);
+        F.fo = fo; fo += (int64_t)F.nf * F.ld;
+        F.vo = vo; vo += F.nf;
+        F.zo = zo; zo += F.nsp;
+        F.uo = uo; uo += F.nf;
+        F.mo = mo; mo += (int64_t)F.nch * F.nf;
+        P.nbig += F.big;
+        P.flops += 2.0 * (F.big ? F.nsp : F.ns) * (double)F.nf * F.nf;
+    }
+    P.F_elems = fo;
+    P.V_elems = vo;
+    P.Z_elems = zo;
+    P.nfu = uo;
+    P.nmap = mo;
+    P.nch = (int64_t)P.ch.size();
+    // unknowns of the front rows
+    P.fu.assign(uo, -1);
+    for (int f = 0; f < ns_; ++f) {
+        const int s = order[f];
+        const MfFront &F = P.fr[f];
+        int *u = P.fu.data() + F.uo;
+        for (size_t q = 0; q < D.sep[s].size(); ++q) {
+            u[2 * q] = D.sep[s][q];
+            u[2 * q + 1] = (int)(N + D.sep[s][q]);
+        }
+        for (size_t q = 0; q < st[s].size(); ++q) {
+            u[F.nsp + 2 * q] = st[s][q];
+            u[F.nsp + 2 * q + 1] = (int)(N + st[s][q]);
+        }
+    }
+    // child maps: parent front row -> child's coupling row
+    P.map.assign(mo, -1);
+    {
+        std::vector<int> idx(N, -1);
+        for (int f = 0; f < ns_; ++f) {
+            const int s = order[f];
+            const MfFront &F = P.fr[f];
+            for (size_t q = 0; q < st[s].size(); ++q) idx[st[s][q]] = (int)q;
+            for (int k = 0; k < F.nch; ++k) {
+                const int c = D.kids[s][k];
+                int *m = P.map.data() + F.mo + (int64_t)k * F.nf;
+                for (size_t q = 0; q < st[c].size(); ++q) {
+                    const int u = st[c][q];
+                    const int a = owner[u] == s ? 2 * pos[u] : F.nsp + 2 * idx[u];
+                    if (owner[u] != s && (owner[u] < s || idx[u] < 0)) {
+                        set_error("multifrontal: a child coupling outside its parent front");
+                        return -1;
+                    }
+                    m[a] = 2 * (int)q;
+                    m[a + 1] = 2 * (int)q + 1;
+                }
+            }
+            for (size_t q = 0; q < st[s].size(); ++q) idx[st[s][q]] = -1;
+        }
+    }
+    // Jacobian entries: each belongs to the front of the earlier of its row's
+    // and column's supernodes, where both are front rows
+    {
+        std::vector<int64_t> cnt(ns_ + 1, 0);
+        int64_t cols[13];
+        auto each = [&](auto &&fn) {
+            for (int comp = 0; comp < 2; ++comp)
+                for (int64_t k = 0; k < N; ++k) {
+                    const int j = (int)(k / imax), i = (int)(k % imax);
+                    const uint8_t fl = j == jmax - 1 ? flags[i] : 0;
+                    const int c = row_columns(imax, jmax, i, j, comp, fl, N, cols);
+                    for (int q = 0; q < c; ++q) {
+                        const int64_t kc = cols[q] >= N ? cols[q] - N : cols[q];
+                        const int s = std::min(owner[k], owner[kc]);
+                        const int vidx = (int)(((comp ? kPsiSlots : 0) + q) * N + k);
+                        fn(fid[s], s, comp * N + k, cols[q], vidx);
+                    }
+                }
+        };
+        each([&](int f, int, int64_t, int64_t, int) { ++cnt[f + 1]; });
+        for (int f = 0; f < ns_; ++f) cnt[f + 1] += cnt[f];
+        P.nvals = cnt[ns_];
+        std::vector<int> eg(P.nvals), ec(P.nvals), ev(P.nvals);
+        {
+            std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+            each([&](int f, int, int64_t g, int64_t gc, int vidx) {
+                const int64_t e = fill[f]++;
+                eg[e] = (int)g;
+                ec[e] = (int)gc;
+                ev[e] = vidx;
+            });
+        }
+        std::vector<int> idx(N, -1);
+        std::vector<std::pair<int64_t, int>> pv;
+        std::vector<int64_t> pend(ns_, 0);   // end of every front's positions
+        P.aseg.push_back(0);
+        for (int f = 0; f < ns_; ++f) {
+            const int s = order[f];
+            const MfFront &F = P.fr[f];
+            for (size_t q = 0; q < st[s].size(); ++q) idx[st[s][q]] = (int)q;
+            auto loc = [&](int64_t g) -> int {
+                const int64_t u = g >= N ? g - N : g;
+                const int fld = g >= N;
+                if (owner[u] == s) return 2 * pos[u] + fld;
+                return idx[u] < 0 ? -1 : F.nsp + 2 * idx[u] + fld;
+            };
+            pv.clear();
+            for (int64_t e = cnt[f]; e < cnt[f + 1]; ++e) {
+                const int a = loc(eg[e]), b = loc(ec[e]);
+                if (a < 0 || b < 0) {
+                    set_error("multifrontal: a Jacobian entry outside its front");
+                    return -1;
+                }
+                pv.emplace_back(F.fo + (int64_t)a * F.ld + b, ev[e]);
+            }
+            std::sort(pv.begin(), pv.end());
+            for (size_t q = 0; q < pv.size(); ++q) {
+                if (q == 0 || pv[q].first != pv[q - 1].first) {
+                    if (q) P.aseg.push_back((int)P.avid.size());
+                    P.aoff.push_back(pv[q].first);
+                }
+                P.avid.push_back(pv[q].second);
+            }
+            if (!pv.empty()) P.aseg.push_back((int)P.avid.size());
+            for (size_t q = 0; q < st[s].size(); ++q) idx[st[s][q]] = -1;
+            pend[f] = (int64_t)P.aoff.size();
+        }
+        P.npos = (int64_t)P.aoff.size();
+        // level ranges of the positions
+        P.lv.assign(P.nheight, MfLevel{});
+        int f = 0;
+        int64_t a = 0;
+        for (int h = 0; h < P.nheight; ++h) {
+            MfLevel &L = P.lv[h];
+            L.f0 = f;
+            L.a0 = a;
+            while (f < ns_ && height[order[f]] == h) {
+                a = pend[f];
+                ++f;
+            }
+            L.f1 = f;
+            L.a1 = a;
+        }
+    }
+    // per-height tables
+    int64_t pool_max = 0;
+    for (int h = 0; h < P.nheight; ++h) {
+        MfLevel &L = P.lv[h];
+        L.fbig = L.f1;
+        for (int f = L.f0; f < L.f1; ++f)
+            if (P.fr[f].big) { L.fbig = f; break; }
+        L.r0 = (int64_t)P.rows.size();
+        L.g0 = (int64_t)P.gemv.size();
+        L.b0 = (int64_t)P.bwd.size();
+        L.c0 = (int64_t)P.colt.size();
+        L.p0 = (int64_t)P.pcolt.size();
+        L.nfmax_small = L.nfmax_big = L.npmax_big = 0;
+        L.pool = 0;
+        for (int f = L.f0; f < L.f1; ++f) {
+            const MfFront &F = P.fr[f];
+            for (int r = 0; r < F.nf; ++r) P.rows.push_back(make_int2(f, r));
+            for (int r = 0; r < F.nf; r += kMfRows) P.gemv.push_back(make_int2(f, r));
+            for (int r = 0; r < F.ns; r += kMfRows) P.bwd.push_back(make_int2(f, r));
+            for (int c = 0; c < F.nf; c += 64) P.colt.push_back(make_int2(f, c));
+            for (int c = 0; c < F.ns; c += 64) P.pcolt.push_back(make_int2(f, c));
+            if (F.big) {
+                L.nfmax_big = std::max(L.nfmax_big, F.nf);
+                L.npmax_big = std::max(L.npmax_big, F.nsp);
+                L.pool += 2 * (int64_t)F.nf * kNB + 2 * (int64_t)round_up(F.nf, kNB) * kNB + kNB * kNB;
+            } else {
+                L.nfmax_small = std::max(L.nfmax_small, F.nf);
+            }
+        }
+        L.r1 = (int64_t)P.rows.size();
+        L.g1 = (int64_t)P.gemv.size();
+        L.b1 = (int64_t)P.bwd.size();
+        L.c1 = (int64_t)P.colt.size();
+        L.p1 = (int64_t)P.pcolt.size();
+        pool_max = std::max(pool_max, L.pool);
+    }
+    P.pool_elems = pool_max;
+    int64_t job = 0;
+    for (int h = 0; h < P.nheight; ++h) {
+        P.lv[h].job0 = job;
+        job += P.lv[h].f1 - P.lv[h].fbig;
+    }
+    P.nrows = (int64_t)P.rows.size();
+    P.ngemv = (int64_t)P.gemv.size();
+    P.nbwd = (int64_t)P.bwd.size();
+    P.ncolt = (int64_t)P.colt.size();
+    P.npcolt = (int64_t)P.pcolt.size();
+    return 0;
+}
+
+namespace {
+template <typename X>
+int upload(X *&d, const std::vector<X> &h, hipStream_t s)
+{
+    if (h.empty()) return 0;
+    V2D_HIP(hipMallocAsync((void **)&d, sizeof(X) * h.size(), s));
+    V2D_HIP(hipMemcpyAsync(d, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice, s));
+    return 0;
+}
+}  // namespace
+
+int mf_upload(MfPattern &P, hipStream_t s)
+{
+    if (P.on_device) return 0;
+    if (upload(P.d_fr, P.fr, s) || upload(P.d_fu, P.fu, s) || upload(P.d_ch, P.ch, s) ||
+        upload(P.d_map, P.map, s) || upload(P.d_rows, P.rows, s) || upload(P.d_gemv, P.gemv, s) ||
+        upload(P.d_bwd, P.bwd, s) || upload(P.d_colt, P.colt, s) || upload(P.d_pcolt, P.pcolt, s) ||
+        upload(P.d_aoff, P.aoff, s) || upload(P.d_aseg, P.aseg, s) || upload(P.d_avid, P.avid, s))
+        return -1;
+    V2D_HIP(hipStreamSynchronize(s));   // the host vectors may change after this returns
+    P.on_device = true;
+    return 0;
+}
+
+void mf_free_pattern(MfPattern &P, hipStream_t s)
+{
+    void *ptrs[] = {P.d_fr, P.d_fu, P.d_ch, P.d_map, P.d_rows, P.d_gemv, P.d_bwd,
+                    P.d_colt, P.d_pcolt, P.d_aoff, P.d_aseg, P.d_avid};
+    for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
+    P.d_fr = nullptr; P.d_fu = P.d_ch = P.d_map = nullptr;
+    P.d_rows = P.d_gemv = P.d_bwd = P.d_colt = P.d_pcolt = nullptr;
+    P.d_aoff = nullptr; P.d_aseg = P.d_avid = nullptr;
+    P.on_device = false;
+}
+
+template <typename T>
+int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s)
+{
+    if (!P.on_device) {
+        set_error("multifrontal: pattern not on the device");
+        return -1;
+    }
+    V2D_HIP(hipMallocAsync((void **)&W.F, sizeof(T) * std::max<int64_t>(P.F_elems, 1), s));
+    V2D_HIP(hipMallocAsync((void **)&W.Vf, sizeof(T) * std::max<int64_t>(P.V_elems, 1), s));
+    V2D_HIP(hipMallocAsync((void **)&W.Zf, sizeof(T) * std::max<int64_t>(P.Z_elems, 1), s));
+    V2D_HIP(hipMallocAsync((void **)&W.pool, sizeof(T) * std::max<int64_t>(P.pool_elems, 1), s));
+    int nbmax = 1;
+    std::vector<Job<T>> jobs;
+    for (const MfLevel &L : P.lv) {
+        nbmax = std::max(nbmax, L.f1 - L.fbig);
+        int64_t off = 0;
+        for (int f = L.fbig; f < L.f1; ++f) {
+            const MfFront &F = P.fr[f];
+            Job<T> J{};
+            J.S = W.F + F.fo;
+            J.n = F.nf;
+            J.ld = F.ld;
+            J.l = 0;
+            J.np = F.nsp;
+            J.voff = 0;
+            const int64_t r64 = round_up(F.nf, kNB);
+            J.colk = W.pool + off; off += (int64_t)F.nf * kNB;
+            J.colk_w = W.pool + off; off += (int64_t)F.nf * kNB;
+            J.rowk = W.pool + off; off += r64 * kNB;
+            J.rowk_w = W.pool + off; off += r64 * kNB;
+            J.dinv = W.pool + off; off += kNB * kNB;
+            J.tmat = nullptr;
+            jobs.push_back(J);
+        }
+    }
+    if (!jobs.empty()) {
+        V2D_HIP(hipMallocAsync((void **)&W.jobs, sizeof(Job<T>) * jobs.size(), s));
+        V2D_HIP(hipMemcpyAsync(W.jobs, jobs.data(), sizeof(Job<T>) * jobs.size(), hipMemcpyHostToDevice, s));
+    }
+    V2D_HIP(hipMallocAsync((void **)&W.flags, sizeof(int) * (nbmax + 1), s));
+    V2D_HIP(hipMemsetAsync(W.flags, 0, sizeof(int) * (nbmax + 1), s));
+    W.err = W.flags + nbmax;
+    W.epoch = 0;
+    W.factored = false;
+    V2D_HIP(hipStreamSynchronize(s));   // jobs is a host temporary
+    return 0;
+}
+
+template <typename T>
+void mf_free(MfWork<T> &W, hipStream_t s)
+{
+    void *ptrs[] = {W.F, W.Vf, W.Zf, W.pool, W.jobs, W.flags};
+    for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
+    W = MfWork<T>{};
+}
+
+// ------------------------------------------------------------ kernels ----
+namespace {
+
+// F rows of the fronts of one height: the children's contribution blocks
+// gathered (children in list order), identity on the padding pivots; one
+// wavefront per row, four rows per workgroup.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__ fr, const int2 *__restrict__ rows,
+                                                     int64_t nrows, const int *__restrict__ ch,
+                                                     const int *__restrict__ map, T *__restrict__ F)
+{
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= nrows) return;
+    const int lane = threadIdx.x & 63;
+    const int2 fa = rows[t];
+    const MfFront f = fr[fa.x];
+    const int a = fa.y;
+    T *dst = F + f.fo + (int64_t)a * f.ld;
+    const bool pad = a >= f.ns && a < f.nsp;
+    for (int b = lane; b < f.ld; b += 64) {
+        T acc = (pad && b == a) ? T(1) : T(0);
+        if (b < f.nf) {
+            for (int k = 0; k < f.nch; ++k) {
+                const int *m = map + f.mo + (int64_t)k * f.nf;
+                const int ia = m[a];
+                if (ia < 0) continue;
+                const int ib = m[b];
+                if (ib < 0) continue;
+                const MfFront c = fr[ch[f.co + k]];
+                acc += F[c.fo + (int64_t)(c.nsp + ia) * c.ld + c.nsp + ib];
+            }
+        }
+        dst[b] = acc;
+    }
+}
+
+// The Jacobian entries the fronts own: F[aoff[u]] += sum of their values.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_scatter(int64_t u0, int64_t u1, const int64_t *__restrict__ aoff,
+                                                    const int *__restrict__ aseg, const int *__restrict__ avid,
+                                                    const T *__restrict__ val, T *__restrict__ F)
+{
+    const int64_t u = u0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= u1) return;
+    T acc = T(0);
+    for (int e = aseg[u]; e < aseg[u + 1]; ++e) acc += val[avid[e]];
+    F[aoff[u]] += acc;
+}
+
+// Partial Gauss-Jordan of one small front in LDS (pivots 0 .. ns-1), one
+// workgroup per front: per pivot, the scaled pivot row and the old pivot
+// column are staged, then every entry is updated.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_small(const MfFront *__restrict__ fr, int f0, T *__restrict__ F)
+{
+    extern __shared__ unsigned char smem[];
+    const MfFront f = fr[f0 + blockIdx.x];
+    const int nf = f.nf, ld = nf;
+    T *sF = reinterpret_cast<T *>(smem);
+    T *prow = sF + (int64_t)nf * nf;
+    T *pcol = prow + nf;
+    T *g = F + f.fo;
+    for (int e = threadIdx.x; e < nf * nf; e += blockDim.x) sF[e] = g[(int64_t)(e / nf) * f.ld + e % nf];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int p = 0; p < f.ns; ++p) {
+        const T ip = T(1) / sF[p * ld + p];
+        for (int c = threadIdx.x; c < nf; c += blockDim.x) {
+            prow[c] = sF[p * ld + c] * ip;
+            pcol[c] = sF[c * ld + p];
+        }
+        __syncthreads();
+        for (int r = wave; r < nf; r += 4) {
+            const T pc = pcol[r];
+            for (int c = lane; c < nf; c += 64) {
+                T v;
+                if (r == p) v = c == p ? ip : prow[c];
+                else if (c == p) v = -pc * ip;
+                else v = sF[r * ld + c] - pc * prow[c];
+                sF[r * ld + c] = v;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < nf * nf; e += blockDim.x) g[(int64_t)(e / nf) * f.ld + e % nf] = sF[e];
+}
+
+// Forward gather: v = b on the pivots, 0 on padding and couplings, plus the
+// children's contribution vectors (list order).  One thread per front row.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_fgather(const MfFront *__restrict__ fr, const int2 *__restrict__ rows,
+                                                    int64_t r0, int64_t r1, const int *__restrict__ ch,
+                                                    const int *__restrict__ map, const int *__restrict__ fu,
+                                                    const T *__restrict__ b, T *__restrict__ Vf)
+{
+    const int64_t t = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= r1) return;
+    const int2 fa = rows[t];
+    const MfFront f = fr[fa.x];
+    const int a = fa.y;
+    T v = a < f.ns ? b[fu[f.uo + a]] : T(0);
+    for (int k = 0; k < f.nch; ++k) {
+        const int m = map[f.mo + (int64_t)k * f.nf + a];
+        if (m < 0) continue;
+        const MfFront c = fr[ch[f.co + k]];
+        v += Vf[c.vo + c.nsp + m];
+    }
+    Vf[f.vo + a] = v;
+}
+
+template <typename T> struct V16;
+template <> struct V16<float> { using type = float4; static constexpr int w = 4; };
+template <> struct V16<double> { using type = double2; static constexpr int w = 2; };
+
+template <typename T>
+__device__ __forceinline__ double dot16(const typename V16<T>::type &a, const T *x)
+{
+    const T *e = reinterpret_cast<const T *>(&a);
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < V16<T>::w; ++q) s += (double)e[q] * (double)x[q];
+    return s;
+}
+
+__device__ __forceinline__ double wave_sum64(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Row products of one front against an LDS vector xs of length ncol starting
+// at column c0 of F: out(r) = sum_c F[r][c0 + c] xs[c], in FP64, for the
+// kMfRows rows row0 .. of the block, 4 per wavefront.  ncol and c0 are
+// multiples of 4 (the 16-byte vectors stay aligned).
+template <typename T, typename Out>
+__device__ __forceinline__ void mf_rows(const MfFront &f, const T *F, int row0, int rend, int c0, int ncol,
+                                        const T *xs, Out &&out)
+{
+    using V = typename V16<T>::type;
+    constexpr int W = V16<T>::w;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nv = ncol / W;
+    for (int q = 0; q < kMfRows / 4; ++q) {
+        const int r = row0 + wave * (kMfRows / 4) + q;
+        if (r >= rend) break;
+        const V *R = reinterpret_cast<const V *>(F + f.fo + (int64_t)r * f.ld + c0);
+        double acc = 0.0;
+        for (int c = lane; c < nv; c += 64) acc += dot16<T>(R[c], xs + c * W);
+        acc = wave_sum64(acc);
+        if (lane == 0) out(r, acc);
+    }
+}
+
+// Forward row products: u = F[:, 0:nsp] v_s; z = u on the pivots (Zf), v_b += u.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_fgemv(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
+                                                  int64_t g0, const T *__restrict__ F, T *__restrict__ Vf,
+                                                  T *__restrict__ Zf)
+{
+    extern __shared__ unsigned char smem[];
+    T *xs = reinterpret_cast<T *>(smem);
+    const int2 fb = blk[g0 + blockIdx.x];
+    const MfFront f = fr[fb.x];
+    for (int c = threadIdx.x; c < f.nsp; c += blockDim.x) xs[c] = Vf[f.vo + c];
+    __syncthreads();
+    mf_rows<T>(f, F, fb.y, f.nf, 0, f.nsp, xs, [&](int r, double acc) {
+        if (r < f.nsp) Zf[f.zo + r] = (T)acc;
+        else Vf[f.vo + r] = (T)((double)Vf[f.vo + r] + acc);
+    });
+}
+
+// Backward row products: x_s = z - W x_b, x_b gathered from the solution.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_bwd(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
+                                                int64_t b0, const T *__restrict__ F, const T *__restrict__ Zf,
+                                                const int *__restrict__ fu, T *__restrict__ x)
+{
+    extern __shared__ unsigned char smem[];
+    T *xs = reinterpret_cast<T *>(smem);
+    const int2 fb = blk[b0 + blockIdx.x];
+    const MfFront f = fr[fb.x];
+    const int nbp = round_up(f.nb, kMfAlign);
+    for (int q = threadIdx.x; q < nbp; q += blockDim.x) xs[q] = q < f.nb ? x[fu[f.uo + f.nsp + q]] : T(0);
+    __syncthreads();
+    // the W block is read up to the 4-rounded coupling count: those columns
+    // lie inside the row stride and meet zeros in xs
+    mf_rows<T>(f, F, fb.y, f.ns, f.nsp, nbp, xs, [&](int r, double acc) {
+        x[fu[f.uo + r]] = (T)((double)Zf[f.zo + r] - acc);
+    });
+}
+
+// Column products for the transposed solve: out(c) = sum_r F[r0 + r][c] ws[r]
+// over rows [r0, r1) for the 64 columns c0 .. of the block; the four
+// wavefronts take every fourth row and their sums are added in wavefront
+// order.
+template <typename T, typename Out>
+__device__ __forceinline__ void mf_cols(const MfFront &f, const T *F, int r0, int r1, int c0, int cend,
+                                        const T *ws, double *red, Out &&out)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = c0 + lane;
+    double acc = 0.0;
+    if (c < cend)
+        for (int r = r0 + wave; r < r1; r += 4) acc += (double)F[f.fo + (int64_t)r * f.ld + c] * (double)ws[r - r0];
+    red[wave * 64 + lane] = acc;
+    __syncthreads();
+    if (wave == 0 && c < cend) out(c, ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane]);
+}
+
+// Transposed forward: u' = F[0:nsp, :]^T v_s; z = u' on the pivots, v_b -= u'.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_fgemv_t(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
+                                                    int64_t c0, const T *__restrict__ F, T *__restrict__ Vf,
+                                                    T *__restrict__ Zf)
+{
+    extern __shared__ unsigned char smem[];
+    double *red = reinterpret_cast<double *>(smem);
+    T *ws = reinterpret_cast<T *>(red + 256);
+    const int2 fb = blk[c0 + blockIdx.x];
+    const MfFront f = fr[fb.x];
+    for (int r = threadIdx.x; r < f.nsp; r += blockDim.x) ws[r] = Vf[f.vo + r];
+    __syncthreads();
+    mf_cols<T>(f, F, 0, f.nsp, fb.y, f.nf, ws, red, [&](int c, double acc) {
+        if (c < f.nsp) Zf[f.zo + c] = (T)acc;
+        else Vf[f.vo + c] = (T)((double)Vf[f.vo + c] - acc);
+    });
+}
+
+// Transposed backward: x_s = z + X^T x_b.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_bwd_t(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
+                                                  int64_t p0, const T *__restrict__ F, const T *__restrict__ Zf,
+                                                  const int *__restrict__ fu, T *__restrict__ x)
+{
+    extern __shared__ unsigned char smem[];
+    double *red = reinterpret_cast<double *>(smem);
+    T *ws = reinterpret_cast<T *>(red + 256);
+    const int2 fb = blk[p0 + blockIdx.x];
+    const MfFront f = fr[fb.x];
+    for (int q = threadIdx.x; q < f.nb; q += blockDim.x) ws[q] = x[fu[f.uo + f.nsp + q]];
+    __syncthreads();
+    mf_cols<T>(f, F, f.nsp, f.nf, fb.y, f.ns, ws, red, [&](int c, double acc) {
+        x[fu[f.uo + c]] = (T)((double)Zf[f.zo + c] + acc);
+    });
+}
+
+__global__ void k_mf_zero_int(int *p) { *p = 0; }
+
+}  // namespace
+
+template <typename T>
+int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s)
+{
+    if (!W.F) {
+        set_error("multifrontal: work not allocated");
+        return -1;
+    }
+    W.factored = false;
+    hipLaunchKernelGGL(k_mf_zero_int, dim3(1), dim3(1), 0, s, W.err);
+    for (int h = 0; h < P.nheight; ++h) {
+        const MfLevel &L = P.lv[h];
+        const int64_t nr = L.r1 - L.r0;
+        if (nr > 0)
+            hipLaunchKernelGGL(k_mf_assemble<T>, dim3((unsigned)((nr + 3) / 4)), dim3(256), 0, s, P.d_fr,
+                               P.d_rows + L.r0, nr, P.d_ch, P.d_map, W.F);
+        if (L.a1 > L.a0)
+            hipLaunchKernelGGL(k_mf_scatter<T>, dim3(grid_for(L.a1 - L.a0)), dim3(256), 0, s, L.a0, L.a1,
+                               P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
+        if (L.fbig > L.f0) {
+            const size_t lds = sizeof(T) * ((size_t)L.nfmax_small * L.nfmax_small + 2 * L.nfmax_small);
+            hipLaunchKernelGGL(k_mf_small<T>, dim3((unsigned)(L.fbig - L.f0)), dim3(256), lds, s, P.d_fr, L.f0,
+                               W.F);
+        }
+        V2D_LAUNCH();
+        if (L.f1 > L.fbig &&
+            gj_sweep_table<T>(W.jobs + L.job0, L.f1 - L.fbig, L.npmax_big, L.nfmax_big, W.flags, W.err, W.epoch, s))
+            return -1;
+    }
+    int e = 0;
+    V2D_HIP(hipMemcpyAsync(&e, W.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    V2D_HIP(hipStreamSynchronize(s));
+    if (e) {
+        set_error("multifrontal: a Gauss-Jordan step waited past its spin limit for the pivot inverse "
+                  "(results invalid)");
+        return -1;
+    }
+    W.factored = true;
+    return 0;
+}
+
+template <typename T>
+int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hipStream_t s)
+{
+    if (!W.factored) {
+        set_error("multifrontal: solve before a factorisation");
+        return -1;
+    }
+    if (b == x) {
+        set_error("multifrontal: right-hand side and solution must be distinct");
+        return -1;
+    }
+    for (int h = 0; h < P.nheight; ++h) {
+        const MfLevel &L = P.lv[h];
+        int nspmax = 0;
+        for (int f = L.f0; f < L.f1; ++f) nspmax = std::max(nspmax, P.fr[f].nsp);
+        if (L.r1 > L.r0)
+            hipLaunchKernelGGL(k_mf_fgather<T>, dim3(grid_for(L.r1 - L.r0)), dim3(256), 0, s, P.d_fr, P.d_rows,
+                               L.r0, L.r1, P.d_ch, P.d_map, P.d_fu, b, W.Vf);
+        if (!trans && L.g1 > L.g0)
+            hipLaunchKernelGGL(k_mf_fgemv<T>, dim3((unsigned)(L.g1 - L.g0)), dim3(256), sizeof(T) * nspmax, s,
+                               P.d_fr, P.d_gemv, L.g0, W.F, W.Vf, W.Zf);
+        if (trans && L.c1 > L.c0)
+            hipLaunchKernelGGL(k_mf_fgemv_t<T>, dim3((unsigned)(L.c1 - L.c0)), dim3(256),
+                               sizeof(double) * 256 + sizeof(T) * nspmax, s, P.d_fr, P.d_colt, L.c0, W.F, W.Vf,
+                               W.Zf);
+        V2D_LAUNCH();
+    }
+    for (int h = P.nheight - 1; h >= 0; --h) {
+        const MfLevel &L = P.lv[h];
+        int nbmax = 0;
+        for (int f = L.f0; f < L.f1; ++f) nbmax = std::max(nbmax, round_up(P.fr[f].nb, kMfAlign));
+        if (!trans && L.b1 > L.b0)
+            hipLaunchKernelGGL(k_mf_bwd<T>, dim3((unsigned)(L.b1 - L.b0)), dim3(256), sizeof(T) * std::max(nbmax, 1),
+                               s, P.d_fr, P.d_bwd, L.b0, W.F, W.Zf, P.d_fu, x);
+        if (trans && L.p1 > L.p0)
+            hipLaunchKernelGGL(k_mf_bwd_t<T>, dim3((unsigned)(L.p1 - L.p0)), dim3(256),
+                               sizeof(double) * 256 + sizeof(T) * std::max(nbmax, 1), s, P.d_fr, P.d_pcolt, L.p0,
+                               W.F, W.Zf, P.d_fu, x);
+        V2D_LAUNCH();
+    }
+    return 0;
+}
+
+template int mf_alloc<float>(const MfPattern &, MfWork<float> &, hipStream_t);
+template int mf_alloc<double>(const MfPattern &, MfWork<double> &, hipStream_t);
+template void mf_free<float>(MfWork<float> &, hipStream_t);
+template void mf_free<double>(MfWork<double> &, hipStream_t);
+template int mf_factor<float>(const MfPattern &, MfWork<float> &, const float *, hipStream_t);
+template int mf_factor<double>(const MfPattern &, MfWork<double> &, const double *, hipStream_t);
+template int mf_solve<float>(const MfPattern &, MfWork<float> &, const float *, float *, bool, hipStream_t);
+template int mf_solve<double>(const MfPattern &, MfWork<double> &, const double *, double *, bool, hipStream_t);
+
+}  // namespace v2d

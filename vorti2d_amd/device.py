@@ -15,6 +15,7 @@ from ._lib import check, f64c, lib, ptr
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
 REFACTOR_POLICIES = {"fixed": 0, "measured": 1}
+LINEAR_SOLVERS = {"block_lu": 0, "multifrontal": 1}
 UPDATE_KERNELS = {"valu64": 0, "valu128": 1, "mfma64": 2}
 SWEEP_KERNELS = {"levels": 0, "persistent": 1, "fused": 2}
 
@@ -147,6 +148,33 @@ class DeviceContext:
         factorisation and iteration times measured on the context instead
         (load-dependent trajectories)."""
         check(lib.v2d_ctx_set_refactor_policy(self._h, REFACTOR_POLICIES.get(policy, policy)))
+
+    def set_linear_solver(self, kind: str | int):
+        """Linear solver of the Newton systems: "multifrontal" (nested-dissection
+        multifrontal LU, the structure of the reference's MUMPS solve) or
+        "block_lu" (block-tridiagonal LU over radial levels).  Drops a stored
+        factorisation."""
+        check(lib.v2d_ctx_set_linear_solver(self._h, LINEAR_SOLVERS.get(kind, kind)))
+
+    def linear_solver(self) -> str:
+        k = C.c_int()
+        check(lib.v2d_ctx_get_linear_solver(self._h, C.byref(k)))
+        return {v: n for n, v in LINEAR_SOLVERS.items()}[k.value]
+
+    def mf_info(self) -> dict:
+        """Sizes of the multifrontal analysis of this grid (see v2d_ctx_mf_info)."""
+        out = np.zeros(8, np.int64)
+        fl = C.c_double()
+        check(lib.v2d_ctx_mf_info(self._h, ptr(out), C.byref(fl)))
+        keys = ("fronts", "heights", "big_fronts", "front_elems", "vector_elems", "panel_elems",
+                "entries", "positions")
+        return dict(zip(keys, (int(v) for v in out)), flops=fl.value)
+
+    def mf_fronts(self) -> np.ndarray:
+        """The factored multifrontal front storage (FP64 copy), for inspection."""
+        out = np.empty(self.mf_info()["front_elems"])
+        check(lib.v2d_ctx_mf_fronts(self._h, ptr(out)))
+        return out
 
     def set_graph(self, enable: bool):
         """Mixed precision: replay the block-LU apply of each GMRES iteration
