@@ -1,11 +1,15 @@
 """Fixed workloads for hardware-counter and kernel-trace runs under rocprofv3.
 
-    python profile_kernels.py factor   [--size 1025] [--precision mixed]
+    python profile_kernels.py newton   [--size 1025] [--precision mixed] [--solver multifrontal]
+    python profile_kernels.py factor   [--size 1025] [--precision mixed] [--solver block_lu]
     python profile_kernels.py assembly [--size 1025] [--launches 20]
     python profile_kernels.py copy     [--mib 256] [--launches 5]
 
-factor:   one block-LU factorisation of the steady cylinder Jacobian (initial
-          state, Re 40) after one warm-up factorisation.
+newton:   the headline solve: pure Newton from the impulsive start to res <= 1e-10
+          and ||R||_2 / sqrt(2N) < 1e-10 (mixed: the benchmark's GMRES and
+          forcing settings), after one warm-up solve.
+factor:   one factorisation of the steady cylinder Jacobian (initial state,
+          Re 40) after one warm-up factorisation.
 assembly: repeated FP32 device assemblies of the same case.
 copy:     a plain device-to-device copy of a known size (calibrates FETCH_SIZE /
           WRITE_SIZE against bytes that are certainly moved once).
@@ -21,7 +25,11 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))   # the repositor
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["factor", "assembly", "copy"])
+    ap.add_argument("what", choices=["newton", "factor", "assembly", "copy"])
+    ap.add_argument("--solver", default="multifrontal", choices=["multifrontal", "block_lu"])
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="newton, mixed: launch the preconditioner-apply kernels one by one "
+                         "(kernel tracing does not follow captured-graph replays)")
     ap.add_argument("--size", type=int, default=1025)
     ap.add_argument("--jmax", type=int, default=0,
                     help="radial size (default: --size); the factor launches of a "
@@ -45,8 +53,34 @@ def main(argv=None):
     jmax = a.jmax or a.size
     xg, yg = generate_annulus(a.size, jmax, 0.5, 50.0)
     ctx = DeviceContext(xg, yg, 0, 1.0, 0.0, a.precision)
+    ctx.set_linear_solver(a.solver)
+    if a.what == "newton":
+        import numpy as np
+        tol = 1e-10
+        if a.precision == "mixed":
+            ctx.set_solver(refactor_every=0, gmres_rtol=1e-6, gmres_max_iter=30)
+            ctx.set_forcing(0.25, tol * np.sqrt(2 * ctx.N))
+            ctx.set_graph(a.graph)
+        for k in range(2):
+            ctx.init_state()
+            ctx.timers(reset=True)
+            r = ctx.steady_newton(a.re, tol=tol, rms_tol=tol, max_iter=30, record=True)
+            t = ctx.timers()
+            print(f"newton {a.size}x{jmax} {a.precision} {a.solver} solve {k}: {r.iters} steps, "
+                  f"{t['factorizations']} factorisations, {sum(r.lin_iters)} GMRES iterations, "
+                  f"{t['step_ms']:.1f} ms (factor {t['factor_ms']:.1f}, solve {t['solve_ms']:.1f})",
+                  flush=True)
+        return 0
     ctx.init_state()
     ctx.assemble(a.re)
+    if a.what == "factor" and a.solver == "multifrontal":
+        import time
+        ctx.factor()
+        t0 = time.perf_counter()
+        ctx.factor()
+        print(f"factor {a.size}x{jmax} {a.precision} multifrontal: "
+              f"{(time.perf_counter() - t0) * 1e3:.1f} ms, {ctx.mf_info()['flops']:.3e} flop", flush=True)
+        return 0
     if a.what == "factor":
         ctx.factor()
         p = ctx.profile_factor()

@@ -5,8 +5,9 @@ impulsive start).
 
 * FP64 solves A x = b and A^T x = b within 1e-9 (relative 2-norm) of SuperLU
   on the exported matrix, for the three far-field conditions and grids from
-  33^2 to 181^2 (small fronts in LDS only at 33^2; dataflow Gauss-Jordan
-  fronts from 97^2 on), and within 1e-9 of the block LU's solution;
+  7 x 6 and 20 x 4 to 181^2 (small fronts in LDS only on the small grids;
+  dataflow Gauss-Jordan fronts from 97^2 on), and within 1e-9 of the block
+  LU's solution;
 * FP32 (the mixed path's preconditioner) within 1e-3 of SuperLU;
 * at 513^2 (multi-step Gauss-Jordan fronts, up to 17 pivot blocks) the FP64
   residual of the Newton right-hand side is below 1e-8 and within 4x of the
@@ -33,7 +34,8 @@ def say(msg):
 
 
 def _ctx(v, n, ff, prec, solver):
-    xg, yg = v.generate_cylinder(n, n, 0.5, 50.0)
+    imax, jmax = n if isinstance(n, tuple) else (n, n)
+    xg, yg = v.generate_cylinder(imax, jmax, 0.5, 50.0)
     ctx = v.DeviceContext(xg, yg, ff, 1.0, 0.0, prec)
     ctx.set_linear_solver(solver)
     ctx.init_state()
@@ -48,14 +50,15 @@ def _system_after_one_step(ctx):
     return sp.csr_matrix((dv, ix, ip), shape=(n2, n2))
 
 
-@pytest.mark.parametrize("n,ff", [(33, 0), (41, 1), (97, 2), (181, 0)])
+@pytest.mark.parametrize("n,ff", [(33, 0), (41, 1), (97, 2), (181, 0), ((7, 6), 0), ((20, 4), 1),
+                                  ((57, 33), 2), ((130, 9), 2)])
 def test_fp64_solves_match_superlu(n, ff):
     import vorti2d_amd as v
-    say(f"multifrontal FP64 {n}x{n} ff {ff}")
+    say(f"multifrontal FP64 {n} ff {ff}")
     ctx = _ctx(v, n, ff, "fp64", "multifrontal")
     A = _system_after_one_step(ctx)
     ctx.factor()
-    rng = np.random.default_rng(n + ff)
+    rng = np.random.default_rng(ff + 7)
     b = rng.standard_normal(A.shape[0])
     x = ctx.solve(b)
     xt = ctx.solve_transpose(b)

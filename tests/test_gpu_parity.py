@@ -122,10 +122,12 @@ def test_local_variants_golden(v):
 
 
 # ------------------------------------------------- device context / pattern --
-def _ctx_case(v, oracle_mod, C, imax, jmax, kind, prec, ff=0, ca=1.0, sa=0.0, seed=3):
+def _ctx_case(v, oracle_mod, C, imax, jmax, kind, prec, ff=0, ca=1.0, sa=0.0, seed=3, solver=None):
     gen = oracle_mod.generate_annulus if kind == "ann" else oracle_mod.generate_cylinder
     xg, yg = gen(imax, jmax)
     ctx = v.DeviceContext(xg, yg, ff, ca, sa, prec)
+    if solver:
+        ctx.set_linear_solver(solver)
     m = C.compute_metrics(1.0 / (imax - 1), 1.0 / (jmax - 1), xg, yg)
     N = imax * jmax
     psi, ome, o1, o2 = oracle_mod.smooth_state(m[8], m[9])
@@ -183,7 +185,7 @@ def test_ctx_export_matches_oracle(v, C, oracle_mod, prec, ff, ca, sa):
 @pytest.mark.parametrize("upd", ["mfma64", "valu64", "valu128"])
 def test_direct_solve_fp64(v, C, oracle_mod, imax, jmax, kind, ff, upd):
     say(f"direct fp64 {kind} {imax}x{jmax} ff={ff} update={upd}")
-    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "fp64", ff)
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "fp64", ff, solver="block_lu")
     ctx.set_update_kernel(upd)
     ctx.assemble(40.0, 0.0, 2.5, 0.1)
     ci, cj, cv, b = ctx.export_coo()
@@ -202,7 +204,7 @@ def test_direct_solve_fp32_update_kernels(v, C, oracle_mod, imax, jmax, kind, ff
     against the exact solve of the same (FP32-valued) matrix is set by FP32
     rounding, so the matrix-core and VALU kernels must agree in size."""
     say(f"direct fp32 update kernels {kind} {imax}x{jmax} ff={ff}")
-    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "mixed", ff)
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "mixed", ff, solver="block_lu")
     ctx.assemble(40.0, 0.0, 2.5, 0.1)
     ci, cj, cv, b = ctx.export_coo()
     N = imax * jmax
@@ -231,7 +233,7 @@ def test_persistent_sweep_equals_level_sweeps(v, C, oracle_mod, prec, imax, jmax
     level counts, and 4100-unknown levels (rows past the register prefetch,
     GEMV rows past the prefetched width)."""
     say(f"persistent sweep {prec} {imax}x{jmax} {kind} ff={ff}")
-    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, prec, ff)
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, prec, ff, solver="block_lu")
     ctx.assemble(40.0, 0.0, 2.5, 0.1)
     ci, cj, cv, b = ctx.export_coo()
     ctx.factor()
@@ -259,7 +261,7 @@ def test_persistent_sweep_expired_waits_fall_back(v, C, oracle_mod, prec):
     launch-pair solve bit for bit, and a later persistent solve with the
     default bound does too, without a fall-back."""
     say(f"persistent sweep fall-back {prec}")
-    ctx, m, st = _ctx_case(v, oracle_mod, C, 73, 40, "ann", prec, 2)
+    ctx, m, st = _ctx_case(v, oracle_mod, C, 73, 40, "ann", prec, 2, solver="block_lu")
     ctx.assemble(40.0, 0.0, 2.5, 0.1)
     ci, cj, cv, b = ctx.export_coo()
     ctx.factor()
@@ -290,7 +292,7 @@ def test_fused_sweep_matches_level_sweeps(v, C, oracle_mod, imax, jmax, kind, ff
     differ in order only, so their FP32 roundings mostly coincide).  The
     4100-unknown end levels of imax 1025 run as launch pairs."""
     say(f"fused sweep {imax}x{jmax} {kind} ff={ff}")
-    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "mixed", ff)
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, kind, "mixed", ff, solver="block_lu")
     ctx.assemble(40.0, 0.0, 2.5, 0.1)
     ci, cj, cv, b = ctx.export_coo()
     ctx.factor()
@@ -327,6 +329,7 @@ def test_fused_sweep_newton_equals_level_sweeps(v, oracle_mod):
     for kind in ("levels", "fused"):
         say(f"fused sweep newton: {kind}")
         ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, "mixed")
+        ctx.set_linear_solver("block_lu")
         ctx.set_solver(refactor_every=1, gmres_rtol=1e-6, gmres_max_iter=30)
         ctx.set_sweep_kernel(kind)
         ctx.init_state()
@@ -374,17 +377,20 @@ def test_forcing_residual_target(v, C, oracle_mod):
     assert tot[1e-10 * n2] <= tot[0.0]
 
 
-def test_graph_replayed_gmres_equals_eager(v, C, oracle_mod):
-    """The mixed Newton step with the block-LU apply replayed from a captured
-    HIP graph equals the step with the apply's kernels launched one by one, bit
-    for bit, over several steps (each replay followed by eager work of the
-    step: SpMV, orthogonalisation, host synchronisation), across
-    refactorisations, and after switching graphs off and on again."""
+@pytest.mark.parametrize("solver", ["multifrontal", "block_lu"])
+def test_graph_replayed_gmres_equals_eager(v, C, oracle_mod, solver):
+    """The mixed Newton step with the preconditioner apply (either linear
+    solver) replayed from a captured HIP graph equals the step with the
+    apply's kernels launched one by one, bit for bit, over several steps (each
+    replay followed by eager work of the step: SpMV, orthogonalisation, host
+    synchronisation), across refactorisations, and after switching graphs off
+    and on again."""
     imax, jmax = 97, 49
     xg, yg = oracle_mod.generate_annulus(imax, jmax)
     hist = {}
     for mode in ("eager", "graph", "toggled"):
         ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, "mixed")
+        ctx.set_linear_solver(solver)
         # refactor every step: the iteration counts then do not depend on
         # measured timings, and the graph is replayed across refactorisations
         ctx.set_solver(refactor_every=1, gmres_rtol=1e-8, gmres_max_iter=30)
@@ -408,13 +414,14 @@ def test_graph_replayed_gmres_equals_eager(v, C, oracle_mod):
         assert np.array_equal(hist["eager"][5][1], hist[mode][5][1])
 
 
+@pytest.mark.parametrize("solver", ["multifrontal", "block_lu"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_stored_factorisation_is_self_contained(v, C, oracle_mod, prec):
-    """A stored block LU keeps solving the system it was built from after later
-    assemblies overwrite the Jacobian (its solve sweeps must not read the new
-    inter-level couplings)."""
+def test_stored_factorisation_is_self_contained(v, C, oracle_mod, prec, solver):
+    """A stored factorisation keeps solving the system it was built from after
+    later assemblies overwrite the Jacobian (the block LU's solve sweeps must
+    not read the new inter-level couplings)."""
     imax, jmax = 73, 41
-    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, "cyl", prec, 1)
+    ctx, m, st = _ctx_case(v, oracle_mod, C, imax, jmax, "cyl", prec, 1, solver=solver)
     ctx.assemble(40.0, 0.0, 2.5, 0.1)
     ci, cj, cv, b = ctx.export_coo()
     ctx.factor()

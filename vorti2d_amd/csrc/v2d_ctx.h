@@ -58,7 +58,7 @@ struct Ctx {
     DirectWork<float> wf;
     // linear solver of the Newton systems (V2D_SOLVER_*); the storage of each
     // solver is allocated at its first factorisation
-    int solver = V2D_SOLVER_BLOCK_LU;
+    int solver = V2D_SOLVER_MULTIFRONTAL;
     bool blu_ready = false;
     std::unique_ptr<MfPattern> mfp;   // multifrontal analysis, device tables uploaded
     MfWork<double> md;

@@ -358,7 +358,7 @@ struct MfFront {
 struct MfLevel {
     int f0, f1, fbig;        // fronts [f0, f1); [fbig, f1) are big
     int64_t r0, r1;          // rows table range (every front row)
-    int64_t g0, g1;          // forward row blocks (kMfRows rows of all front rows)
+    int64_t g0, g1;          // forward row blocks (all front rows)
     int64_t b0, b1;          // backward row blocks (pivot rows)
     int64_t c0, c1;          // transposed column blocks (64 columns of all front columns)
     int64_t p0, p1;          // transposed column blocks of the pivot columns
@@ -367,8 +367,6 @@ struct MfLevel {
     int64_t pool;            // Gauss-Jordan panel elements of the big fronts
     int64_t job0;            // first job of the level in the job table
 };
-
-constexpr int kMfRows = 16;   // rows of one forward / backward row block
 
 struct MfPattern {
     int64_t N = 0;

@@ -47,11 +47,22 @@ namespace v2d {
 namespace {
 
 constexpr int kMfLeaf = 32;      // most nodes of a leaf supernode
-constexpr int kMfSmall = 136;    // largest front swept in LDS by one workgroup
+constexpr int kMfSmall = 144;    // largest front swept in registers by one workgroup (16 x 9)
 constexpr int kMfAlign = 4;      // row stride and pivot padding of small fronts
 constexpr int kNB = 64;          // Gauss-Jordan block of the large fronts
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Lanes that share one row product of ncol columns (a power of two, 4 .. 64:
+// about one 16-byte FP32 vector per lane and pass), and the rows of one
+// workgroup's block at that width (four passes of 256 / lanes rows).
+__host__ __device__ inline int row_lanes(int ncol)
+{
+    int l = 4;
+    while (l < 64 && 4 * l < ncol) l <<= 1;
+    return l;
+}
+__host__ __device__ inline int block_rows(int ncol) { return 4 * (256 / row_lanes(ncol)); }
 
 // Symmetrised node graph of the Jacobian pattern (CSR, no self loops).
 struct NodeGraph {
@@ -450,8 +461,8 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
         for (int f = L.f0; f < L.f1; ++f) {
             const MfFront &F = P.fr[f];
             for (int r = 0; r < F.nf; ++r) P.rows.push_back(make_int2(f, r));
-            for (int r = 0; r < F.nf; r += kMfRows) P.gemv.push_back(make_int2(f, r));
-            for (int r = 0; r < F.ns; r += kMfRows) P.bwd.push_back(make_int2(f, r));
+            for (int r = 0; r < F.nf; r += block_rows(F.nsp)) P.gemv.push_back(make_int2(f, r));
+            for (int r = 0; r < F.ns; r += block_rows(round_up(F.nb, kMfAlign))) P.bwd.push_back(make_int2(f, r));
             for (int c = 0; c < F.nf; c += 64) P.colt.push_back(make_int2(f, c));
             for (int c = 0; c < F.ns; c += 64) P.pcolt.push_back(make_int2(f, c));
             if (F.big) {
@@ -623,42 +634,69 @@ __global__ void __launch_bounds__(256) k_mf_scatter(int64_t u0, int64_t u1, cons
     F[aoff[u]] += acc;
 }
 
-// Partial Gauss-Jordan of one small front in LDS (pivots 0 .. ns-1), one
-// workgroup per front: per pivot, the scaled pivot row and the old pivot
-// column are staged, then every entry is updated.
-template <typename T>
+// Partial Gauss-Jordan of one small front (pivots 0 .. ns-1, nf <= 16 NI), one
+// workgroup per front, the front in registers: thread (ty, tx) of a 16 x 16
+// grid holds rows ty + 16 i and columns tx + 16 j.  Per pivot p the owners of
+// row p and of column p stage them in LDS (two buffers, alternating, so one
+// barrier per pivot), then every entry takes the rank-1 update and the pivot
+// row and column are set: A[p][c] /= A[p][p], A[r][p] = -A[r][p] / A[p][p],
+// A[p][p] = 1 / A[p][p].
+template <typename T, int NI>
 __global__ void __launch_bounds__(256) k_mf_small(const MfFront *__restrict__ fr, int f0, T *__restrict__ F)
 {
-    extern __shared__ unsigned char smem[];
+    __shared__ T prow[2][16 * NI], pcol[2][16 * NI];
     const MfFront f = fr[f0 + blockIdx.x];
-    const int nf = f.nf, ld = nf;
-    T *sF = reinterpret_cast<T *>(smem);
-    T *prow = sF + (int64_t)nf * nf;
-    T *pcol = prow + nf;
+    const int nf = f.nf, ns = f.ns;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
     T *g = F + f.fo;
-    for (int e = threadIdx.x; e < nf * nf; e += blockDim.x) sF[e] = g[(int64_t)(e / nf) * f.ld + e % nf];
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int p = 0; p < f.ns; ++p) {
-        const T ip = T(1) / sF[p * ld + p];
-        for (int c = threadIdx.x; c < nf; c += blockDim.x) {
-            prow[c] = sF[p * ld + c] * ip;
-            pcol[c] = sF[c * ld + p];
+    T a[NI][NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int r = ty + 16 * i, c = tx + 16 * j;
+            a[i][j] = (r < nf && c < nf) ? g[(int64_t)r * f.ld + c] : T(0);
         }
-        __syncthreads();
-        for (int r = wave; r < nf; r += 4) {
-            const T pc = pcol[r];
-            for (int c = lane; c < nf; c += 64) {
-                T v;
-                if (r == p) v = c == p ? ip : prow[c];
-                else if (c == p) v = -pc * ip;
-                else v = sF[r * ld + c] - pc * prow[c];
-                sF[r * ld + c] = v;
-            }
+    int par = 0;
+#pragma unroll
+    for (int I = 0; I < NI; ++I) {
+        if (16 * I >= ns) break;
+        for (int q = 0; q < 16 && 16 * I + q < ns; ++q) {
+            const int p = 16 * I + q;
+            if (ty == q)
+#pragma unroll
+                for (int j = 0; j < NI; ++j) prow[par][tx + 16 * j] = a[I][j];
+            if (tx == q)
+#pragma unroll
+                for (int i = 0; i < NI; ++i) pcol[par][ty + 16 * i] = a[i][I];
+            __syncthreads();
+            const T ip = T(1) / pcol[par][p];
+            T pr[NI], pc[NI];
+#pragma unroll
+            for (int j = 0; j < NI; ++j) pr[j] = prow[par][tx + 16 * j] * ip;
+#pragma unroll
+            for (int i = 0; i < NI; ++i) pc[i] = pcol[par][ty + 16 * i];
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+#pragma unroll
+                for (int j = 0; j < NI; ++j) a[i][j] = fma(-pc[i], pr[j], a[i][j]);
+            if (ty == q)
+#pragma unroll
+                for (int j = 0; j < NI; ++j) a[I][j] = pr[j];
+            if (tx == q)
+#pragma unroll
+                for (int i = 0; i < NI; ++i) a[i][I] = -pc[i] * ip;
+            if (ty == q && tx == q) a[I][I] = ip;
+            par ^= 1;
         }
-        __syncthreads();
     }
-    for (int e = threadIdx.x; e < nf * nf; e += blockDim.x) g[(int64_t)(e / nf) * f.ld + e % nf] = sF[e];
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int r = ty + 16 * i, c = tx + 16 * j;
+            if (r < nf && c < nf) g[(int64_t)r * f.ld + c] = a[i][j];
+        }
 }
 
 // Forward gather: v = b on the pivots, 0 on padding and couplings, plus the
@@ -706,25 +744,28 @@ __device__ __forceinline__ double wave_sum64(double v)
 }
 
 // Row products of one front against an LDS vector xs of length ncol starting
-// at column c0 of F: out(r) = sum_c F[r][c0 + c] xs[c], in FP64, for the
-// kMfRows rows row0 .. of the block, 4 per wavefront.  ncol and c0 are
-// multiples of 4 (the 16-byte vectors stay aligned).
+// at column c0 of F: out(r) = sum_c F[r][c0 + c] xs[c], in FP64, for the rows
+// row0 .. row0 + block_rows(ncol) - 1 below rend.  row_lanes(ncol) lanes share
+// a row (16-byte vectors, summed across the group by shuffles).  ncol and c0
+// are multiples of 4 (the vectors stay aligned).
 template <typename T, typename Out>
 __device__ __forceinline__ void mf_rows(const MfFront &f, const T *F, int row0, int rend, int c0, int ncol,
                                         const T *xs, Out &&out)
 {
     using V = typename V16<T>::type;
     constexpr int W = V16<T>::w;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int L = row_lanes(ncol), G = 256 / L;
+    const int l = threadIdx.x & (L - 1), g = threadIdx.x / L;
     const int nv = ncol / W;
-    for (int q = 0; q < kMfRows / 4; ++q) {
-        const int r = row0 + wave * (kMfRows / 4) + q;
-        if (r >= rend) break;
-        const V *R = reinterpret_cast<const V *>(F + f.fo + (int64_t)r * f.ld + c0);
+    for (int q = 0; q < 4; ++q) {
+        const int r = row0 + q * G + g;
         double acc = 0.0;
-        for (int c = lane; c < nv; c += 64) acc += dot16<T>(R[c], xs + c * W);
-        acc = wave_sum64(acc);
-        if (lane == 0) out(r, acc);
+        if (r < rend) {
+            const V *R = reinterpret_cast<const V *>(F + f.fo + (int64_t)r * f.ld + c0);
+            for (int c = l; c < nv; c += L) acc += dot16<T>(R[c], xs + c * W);
+        }
+        for (int o = L >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (l == 0 && r < rend) out(r, acc);
     }
 }
 
@@ -844,9 +885,13 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s)
             hipLaunchKernelGGL(k_mf_scatter<T>, dim3(grid_for(L.a1 - L.a0)), dim3(256), 0, s, L.a0, L.a1,
                                P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
         if (L.fbig > L.f0) {
-            const size_t lds = sizeof(T) * ((size_t)L.nfmax_small * L.nfmax_small + 2 * L.nfmax_small);
-            hipLaunchKernelGGL(k_mf_small<T>, dim3((unsigned)(L.fbig - L.f0)), dim3(256), lds, s, P.d_fr, L.f0,
-                               W.F);
+            const dim3 g((unsigned)(L.fbig - L.f0));
+            if (L.nfmax_small <= 64)
+                hipLaunchKernelGGL((k_mf_small<T, 4>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
+            else if (L.nfmax_small <= 96)
+                hipLaunchKernelGGL((k_mf_small<T, 6>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
+            else
+                hipLaunchKernelGGL((k_mf_small<T, 9>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
         }
         V2D_LAUNCH();
         if (L.f1 > L.fbig &&
