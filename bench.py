@@ -16,10 +16,11 @@ of the FP64 solve's (measured along the 1025^2 trajectories), so an RMS below
 1e-10 (||R||_2 < 1.45e-7) keeps them within 1e-9; the bench reports the
 differences.  Each Newton iteration is device assembly + linear solve +
 update; the solve starts with no stored factorisation.  Mesh upload, metrics,
-pattern build and kernel autotuning are one-time setup outside the timed
-region.  The mixed-precision solve (FP32 Jacobian, FP32 block-LU
-preconditioned FP32 GMRES, FP64 residual / state) is the reported value; the
-plain FP64 exact-Newton solve is timed beside it.  A solve that misses the
+pattern build, the multifrontal analysis and the allocation of its fronts are
+one-time setup outside the timed region.  The mixed-precision solve (FP32
+Jacobian, FP32 multifrontal-LU preconditioned FP32 GMRES, FP64 residual /
+state) is the reported value; the plain FP64 exact-Newton solve is timed
+beside it.  --solver block_lu times the block-tridiagonal LU instead.  A solve that misses the
 stopping rules within the iteration cap voids the value (the line then carries
 an "error" and the process exits with status 1).
 
@@ -232,6 +233,7 @@ def sweep_timing(workers=8, n=64):
 def run_precision(v, xg, yg, prec, a):
     t0 = time.perf_counter()
     ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, prec)
+    ctx.set_linear_solver(a.solver)
     if prec == "mixed":
         ctx.set_solver(refactor_every=a.refactor_every, gmres_rtol=a.gmres_rtol,
                        gmres_max_iter=a.gmres_max)
@@ -239,10 +241,18 @@ def run_precision(v, xg, yg, prec, a):
         ctx.set_forcing(a.forcing, TOL * np.sqrt(2 * xg.size) if a.forcing_target else 0.0)
         ctx.set_graph(a.graph)
     ctx.init_state()
-    tune = ctx.autotune_update_kernel(a.re)
+    if a.solver == "block_lu":
+        tune = ctx.autotune_update_kernel(a.re)
+    else:
+        tune = {}
+        ctx.assemble(a.re)
+        ctx.factor()   # analysis, front allocation and the code objects: one-time setup
+        info = ctx.mf_info()
     setup = time.perf_counter() - t0
-    log(f"[{prec}] setup {setup:.2f} s (update-kernel autotune "
-        + ", ".join(f"{k}: {t:.3f} s" for k, t in tune.items())
+    log(f"[{prec}] setup {setup:.2f} s ("
+        + (("update-kernel autotune " + ", ".join(f"{k}: {t:.3f} s" for k, t in tune.items()))
+           if tune else f"multifrontal analysis: {info['fronts']} fronts, {info['heights']} heights, "
+                        f"{info['flops']:.3e} flop per factorisation")
         + f"), device memory {ctx.device_bytes() / 2**30:.1f} GiB")
     nsteps = a.steps if prec == "mixed" else a.fp64_steps
     for _ in range(a.warmup):
@@ -269,8 +279,40 @@ def run_precision(v, xg, yg, prec, a):
         out["spmv_ms"] = ctx.time_spmv(launches=50)
         out["nnz"] = ctx.nnz()
     ctx.assemble(a.re)
-    out["probe"] = ctx.profile_factor()
+    out["probe"] = ctx.profile_factor() if a.solver == "block_lu" else ctx.profile_mf()
+    if a.solver != "block_lu":
+        out["mf_info"] = ctx.mf_info()
     ctx.close()
+    return out
+
+
+def mf_kernels(run, prec, nsolves):
+    """Per kernel group of the multifrontal path: the event-timed launches of
+    one factorisation and one apply (profile_mf), their algorithmic rate, and
+    the group's time per Newton solve (factorisation groups times the
+    factorisations per solve, apply groups times the applies per solve: GMRES
+    iterations for mixed, Newton steps for FP64)."""
+    pr = run["probe"]
+    tm = run["timers"]
+    fac = tm["factorizations"] / max(nsolves, 1)
+    app = (tm["gmres_iters"] if prec == "mixed" else tm["steps"]) / max(nsolves, 1)
+    peak_tf = MFMA_PEAK_TFLOPS["fp32" if prec == "mixed" else "fp64"]
+    out = {}
+    for k, g in pr["groups"].items():
+        if g["launches"] == 0 or g["ms"] <= 0:
+            continue
+        per_solve = g["ms"] * (app if k in ("gather", "fwd", "bwd") else fac)
+        e = {"kernel": g["kernel"], "launches": g["launches"], "ms": g["ms"],
+             "avg_launch_us": g["ms"] / g["launches"] * 1e3, "ms_per_solve": per_solve}
+        if k == "gj_step":
+            tf = g["flops"] / (g["ms"] * 1e-3) / 1e12
+            e.update(bound="mfma", achieved=tf, peak=peak_tf, unit="TFLOP/s", frac=tf / peak_tf,
+                     flops=g["flops"], bytes=g["bytes"])
+        elif g["bytes"] > 0:
+            gbs = g["bytes"] / (g["ms"] * 1e-3) / 1e9
+            e.update(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=gbs / HBM_PEAK_GBS,
+                     bytes=g["bytes"], flops=g["flops"])
+        out[k] = e
     return out
 
 
@@ -296,8 +338,11 @@ def main():
     ap.add_argument("--no-forcing-target", dest="forcing_target", action="store_false",
                     help="mixed: no residual-target safeguard on the forcing term")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
-                    help="mixed: launch the block-LU apply kernels one by one instead of replaying a "
+                    help="mixed: launch the preconditioner-apply kernels one by one instead of replaying a "
                          "captured graph (same kernels; for kernel-trace profilers)")
+    ap.add_argument("--solver", default="multifrontal", choices=["multifrontal", "block_lu"],
+                    help="linear solver: nested-dissection multifrontal LU (default) or the "
+                         "block-tridiagonal LU")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "vorti2d_amd runs on one GPU only; the sharded design "
@@ -318,14 +363,38 @@ def main():
     spmv_b = spmv_bytes(imax, jmax, nnz, 4, 4)
     spmv_gbs = spmv_b / (mx["spmv_ms"] * 1e-3) / 1e9
     pr = mx["probe"]
-    upd_avg_ms = pr["update_ms"] / max(pr["launches"], 1)
-    upd_tf = pr["flops"] / (pr["update_ms"] * 1e-3) / 1e12
     traffic = traffic_table()
-    kname = {"mfma64": "k_gj_step_mfma_b<float>", "valu64": "k_gj_update_b<float, 64>",
-             "valu128": "k_gj_update_b<float, 128>"}[min(mx["tune"], key=mx["tune"].get)]
-    alg_upd = pr["bytes"] / max(pr["launches"], 1)
-    ent = traffic.get(kname) or {}
-    tr_upd = ent["measured_over_algorithmic"] * alg_upd if "measured_over_algorithmic" in ent else None
+    nsol = max(len(mx["times"]), 1)
+    if a.solver == "block_lu":
+        upd_avg_ms = pr["update_ms"] / max(pr["launches"], 1)
+        upd_tf = pr["flops"] / (pr["update_ms"] * 1e-3) / 1e12
+        kname = {"mfma64": "k_gj_step_mfma_b<float>", "valu64": "k_gj_update_b<float, 64>",
+                 "valu128": "k_gj_update_b<float, 128>"}[min(mx["tune"], key=mx["tune"].get)]
+        alg_upd = pr["bytes"] / max(pr["launches"], 1)
+        ent = traffic.get(kname) or {}
+        tr_upd = ent["measured_over_algorithmic"] * alg_upd if "measured_over_algorithmic" in ent else None
+        roof = {"bound": "mfma", "achieved": upd_tf, "peak": MFMA_PEAK_TFLOPS["fp32"],
+                "unit": "TFLOP/s", "frac": upd_tf / MFMA_PEAK_TFLOPS["fp32"],
+                "traffic": tr_upd,
+                "kernel": kname,
+                "avg_launch_us": upd_avg_ms * 1e3, "launches_per_factorization": pr["launches"],
+                "flops_per_factorization": pr["flops"],
+                "algorithmic_bytes_per_launch": alg_upd,
+                "share_of_factorization": pr["update_ms"] / pr["factor_ms"]}
+        kernels = None
+    else:
+        kernels = mf_kernels(mx, "mixed", nsol)
+        dom = max((k for k in kernels if "bound" in kernels[k]), key=lambda k: kernels[k]["ms_per_solve"])
+        d = kernels[dom]
+        kname = d["kernel"] + "<float>"
+        ent = traffic.get(kname) or {}
+        alg = d["bytes"] / d["launches"]
+        tr = ent["measured_over_algorithmic"] * alg if "measured_over_algorithmic" in ent else None
+        roof = {"bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
+                "frac": d["frac"], "traffic": tr, "kernel": kname, "group": dom,
+                "avg_launch_us": d["avg_launch_us"], "launches": d["launches"],
+                "algorithmic_bytes_per_launch": alg, "algorithmic_flops_per_launch": d["flops"] / d["launches"],
+                "ms_per_solve": d["ms_per_solve"], "share_of_solve": d["ms_per_solve"] / (mx["seconds"] * 1e3)}
     ent = traffic.get("k_assemble<float>") or {}
     tr_asm = ent.get("bytes_per_launch")
     tm = mx["timers"]
@@ -341,7 +410,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32/f64 (f32 Jacobian, block LU, GMRES; f64 residual, state, refinement)",
+        "dtype": ("f32/f64 (f32 Jacobian, " + ("block LU" if a.solver == "block_lu" else "multifrontal LU")
+                  + ", GMRES; f64 residual, state, refinement)"),
         "data": "synthetic (closed-form O-grid, reference initial state)",
         "config": {"workload": f"steady cylinder {imax}x{jmax} O-grid (r 0.5..50, geometric "
                                f"radial ratio {v.mesh.stretch_ratio(jmax):.6f}), Re {a.re:g}, "
@@ -349,7 +419,9 @@ def main():
                    "imax": imax, "jmax": jmax, "dof": 2 * N, "nnz": nnz,
                    "criterion": "reference res = ||delta||_2/(2N) <= 1e-10 and FP64 "
                                 "||R||_2/sqrt(2N) < 1e-10 at the final state",
-                   "linear_solver": (f"FP32 block-LU preconditioned FP32 GMRES (rtol "
+                   "linear_solver": (("FP32 block-LU" if a.solver == "block_lu" else
+                                      "FP32 nested-dissection multifrontal-LU")
+                                     + f" preconditioned FP32 GMRES (rtol "
                                      f"{a.gmres_rtol:g}"
                                      + (f", forcing <= {a.forcing:g}" if a.forcing > 0 else "")
                                      + (", forcing >= 0.5 (1e-10 sqrt(2N)) / ||R_k|| (residual "
@@ -367,14 +439,8 @@ def main():
         "rnorm_final_scaled": last["rnorm"][-1] / (2 * N),
         "res_final": last["res"][-1],
         "cd": mx["cd"],
-        "roofline": {"bound": "mfma", "achieved": upd_tf, "peak": MFMA_PEAK_TFLOPS["fp32"],
-                     "unit": "TFLOP/s", "frac": upd_tf / MFMA_PEAK_TFLOPS["fp32"],
-                     "traffic": tr_upd,
-                     "kernel": kname,
-                     "avg_launch_us": upd_avg_ms * 1e3, "launches_per_factorization": pr["launches"],
-                     "flops_per_factorization": pr["flops"],
-                     "algorithmic_bytes_per_launch": alg_upd,
-                     "share_of_factorization": pr["update_ms"] / pr["factor_ms"]},
+        "roofline": roof,
+        "kernels_mixed": kernels,
         "assembly": {"bound": "hbm", "achieved": asm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": asm_gbs / HBM_PEAK_GBS, "traffic": tr_asm,
                      "kernel": "k_assemble<float>",
@@ -386,15 +452,19 @@ def main():
             k: tm[k + "_ms"] / max(len(mx["times"]), 1) for k in ("assemble", "factor", "solve")},
         "times_s": mx["times"],
         "setup_s": mx["setup_s"],
-        "update_kernel_autotune_s": mx["tune"],
+        "update_kernel_autotune_s": mx["tune"] or None,
+        "multifrontal": mx.get("mf_info"),
     }
     if "fp64" in out:
         f = out["fp64"]
         res["fp64"] = {"seconds": f["seconds"], "times_s": f["times"],
                        "newton_iters": f["last"]["iters"], "rnorm_final": f["last"]["rnorm"][-1],
-                       "cd": f["cd"], "factorization_ms": f["probe"]["factor_ms"],
-                       "update_kernel_tflops": f["probe"]["flops"] / (f["probe"]["update_ms"] * 1e-3)
-                       / 1e12}
+                       "cd": f["cd"], "factorization_ms": f["probe"]["factor_ms"]}
+        if a.solver == "block_lu":
+            res["fp64"]["update_kernel_tflops"] = (f["probe"]["flops"] / (f["probe"]["update_ms"] * 1e-3)
+                                                   / 1e12)
+        else:
+            res["fp64"]["kernels"] = mf_kernels(f, "fp64", max(len(f["times"]), 1))
         res["fp64"]["converged"] = f["converged"]
         res["fp64"]["rnorm_final_rms"] = f["last"]["rnorm"][-1] / np.sqrt(2 * N)
         res["mixed_speedup_vs_fp64"] = f["seconds"] / mx["seconds"]

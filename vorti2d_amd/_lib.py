@@ -88,6 +88,7 @@ _SIGS = {
     "v2d_ctx_get_linear_solver": ([vp, P(i32)], i32),
     "v2d_ctx_mf_info": ([vp, vp, P(f64)], i32),
     "v2d_ctx_mf_fronts": ([vp, vp], i32),
+    "v2d_ctx_profile_mf": ([vp, vp], i32),
     "v2d_mf_analyse": ([i32, i32, vp, P(vp)], i32),
     "v2d_mf_sizes": ([vp, vp, P(f64)], i32),
     "v2d_mf_export": ([vp, vp, vp, vp, vp, vp, vp, vp], i32),

@@ -946,6 +946,72 @@ int v2d_ctx_mf_info(v2d_ctx *ctx, int64_t *out, double *flops)
     return 0;
 }
 
+int v2d_ctx_profile_mf(v2d_ctx *ctx, double *out)
+{
+    Ctx *c = ctx->c.get();
+    if (c->solver != V2D_SOLVER_MULTIFRONTAL) {
+        set_error("profile_mf: the context's linear solver is not the multifrontal LU");
+        return -1;
+    }
+    if (ensure_mf(c)) return -1;
+    V2D_HIP(hipStreamSynchronize(c->s));
+    const size_t tsz = c->prec == V2D_FP64 ? sizeof(double) : sizeof(float);
+    V2D_HIP(hipMemcpyAsync(c->val_pc, c->val, (size_t)kSlots * c->N * tsz, hipMemcpyDeviceToDevice, c->s));
+    MfProbe pr;
+    hipEvent_t e0, e1;
+    V2D_HIP(hipEventCreate(&e0));
+    V2D_HIP(hipEventCreate(&e1));
+    int rc;
+    V2D_HIP(hipEventRecord(e0, c->s));
+    if (c->prec == V2D_FP64)
+        rc = mf_factor<double>(*c->mfp, c->md, static_cast<const double *>(c->val_pc), c->s, &pr);
+    else
+        rc = mf_factor<float>(*c->mfp, c->mw, static_cast<const float *>(c->val_pc), c->s, &pr);
+    V2D_HIP(hipEventRecord(e1, c->s));
+    if (!rc) rc = pr.flush();
+    float fac_ms = 0.f;
+    if (!rc) {
+        V2D_HIP(hipEventSynchronize(e1));
+        V2D_HIP(hipEventElapsedTime(&fac_ms, e0, e1));
+    }
+    float sol_ms = 0.f;
+    if (!rc) {
+        MfProbe ps;
+        V2D_HIP(hipEventRecord(e0, c->s));
+        if (c->prec == V2D_FP64) {
+            rc = mf_solve<double>(*c->mfp, c->md, c->rhs, c->delta, false, c->s, &ps);
+        } else {
+            V2D_HIP(hipMemsetAsync(c->ga_in, 0, sizeof(float) * 2 * c->N, c->s));
+            rc = mf_solve<float>(*c->mfp, c->mw, c->ga_in, c->ga_out, false, c->s, &ps);
+        }
+        V2D_HIP(hipEventRecord(e1, c->s));
+        if (!rc) rc = ps.flush();
+        if (!rc) {
+            V2D_HIP(hipEventSynchronize(e1));
+            V2D_HIP(hipEventElapsedTime(&sol_ms, e0, e1));
+            for (int g = MfProbe::kGather; g <= MfProbe::kBwd; ++g) {
+                pr.ms[g] = ps.ms[g];
+                pr.launches[g] = ps.launches[g];
+                pr.flops[g] = ps.flops[g];
+                pr.bytes[g] = ps.bytes[g];
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return -1;
+    c->factored = true;   // the factorisation of the last assembled Jacobian
+    for (int g = 0; g < MfProbe::kGroups; ++g) {
+        out[4 * g] = pr.ms[g];
+        out[4 * g + 1] = (double)pr.launches[g];
+        out[4 * g + 2] = pr.flops[g];
+        out[4 * g + 3] = pr.bytes[g];
+    }
+    out[4 * MfProbe::kGroups] = fac_ms;
+    out[4 * MfProbe::kGroups + 1] = sol_ms;
+    return 0;
+}
+
 int v2d_ctx_mf_fronts(v2d_ctx *ctx, double *F)
 {
     Ctx *c = ctx->c.get();

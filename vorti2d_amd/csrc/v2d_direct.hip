@@ -1320,28 +1320,48 @@ static int gj_invert_b(const Batch<T> &B, DirectWork<T> &W, LaunchProbe *probe, 
 // Gauss-Jordan sweeps of a device table of jobs (MFMA dataflow steps): every
 // job's pivots [0, np) are eliminated from its n x n matrix in place, np a
 // multiple of 64 at most npmax, n at most nmax.  flags holds one word per job.
+// gj_table_begin inverts every first pivot block and forms the step-0 panels;
+// gj_table_step runs the step at pivot k0 (k0 = 0, 64, .. < npmax in order).
+template <typename T>
+int gj_table_begin(const Job<T> *jobs, int nj, int nmax, hipStream_t s)
+{
+    if (nj > 65535) {
+        set_error("Gauss-Jordan table: more than 65535 jobs in one launch");
+        return -1;
+    }
+    const unsigned npr = (unsigned)((nmax + RP - 1) / RP);
+    TableBatch<T> B{jobs, nj, kUpdateMfma64, 0};
+    hipLaunchKernelGGL((k_gj_diag_b<T, TableBatch<T>>), dim3(1, 1, (unsigned)nj), dim3(256), 0, s, B, 0);
+    hipLaunchKernelGGL((k_gj_rowpanel_b<T, TableBatch<T>>), dim3(npr, 2, (unsigned)nj), dim3(256), 0, s, B, 0);
+    V2D_LAUNCH();
+    return 0;
+}
+
+template <typename T>
+int gj_table_step(const Job<T> *jobs, int nj, int nmax, int k0, int *flags, int *err, int &epoch, hipStream_t s)
+{
+    const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
+    TableBatch<T> B{jobs, nj, kUpdateMfma64, (k0 / NB) & 1};
+    hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>>), dim3(nt * nt * (unsigned)nj), dim3(256), 0, s, B, k0,
+                       ++epoch, flags, err);
+    V2D_LAUNCH();
+    return 0;
+}
+
 template <typename T>
 int gj_sweep_table(const Job<T> *jobs, int nj, int npmax, int nmax, int *flags, int *err, int &epoch,
                    hipStream_t s)
 {
     if (nj <= 0 || npmax <= 0) return 0;
-    if (nj > 65535) {
-        set_error("Gauss-Jordan table: more than 65535 jobs in one launch");
-        return -1;
-    }
-    const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
-    const unsigned npr = (unsigned)((nmax + RP - 1) / RP);
-    TableBatch<T> B{jobs, nj, kUpdateMfma64, 0};
-    hipLaunchKernelGGL((k_gj_diag_b<T, TableBatch<T>>), dim3(1, 1, (unsigned)nj), dim3(256), 0, s, B, 0);
-    hipLaunchKernelGGL((k_gj_rowpanel_b<T, TableBatch<T>>), dim3(npr, 2, (unsigned)nj), dim3(256), 0, s, B, 0);
-    for (int k0 = 0; k0 < npmax; k0 += NB) {
-        B.par = (k0 / NB) & 1;
-        hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>>), dim3(nt * nt * (unsigned)nj), dim3(256), 0, s,
-                           B, k0, ++epoch, flags, err);
-    }
-    V2D_LAUNCH();
+    if (gj_table_begin(jobs, nj, nmax, s)) return -1;
+    for (int k0 = 0; k0 < npmax; k0 += NB)
+        if (gj_table_step(jobs, nj, nmax, k0, flags, err, epoch, s)) return -1;
     return 0;
 }
+template int gj_table_begin<float>(const Job<float> *, int, int, hipStream_t);
+template int gj_table_begin<double>(const Job<double> *, int, int, hipStream_t);
+template int gj_table_step<float>(const Job<float> *, int, int, int, int *, int *, int &, hipStream_t);
+template int gj_table_step<double>(const Job<double> *, int, int, int, int *, int *, int &, hipStream_t);
 template int gj_sweep_table<float>(const Job<float> *, int, int, int, int *, int *, int &, hipStream_t);
 template int gj_sweep_table<double>(const Job<double> *, int, int, int, int *, int *, int &, hipStream_t);
 

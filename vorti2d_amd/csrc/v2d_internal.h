@@ -310,6 +310,10 @@ struct Job {
 template <typename T>
 int gj_sweep_table(const Job<T> *jobs, int nj, int npmax, int nmax, int *flags, int *err, int &epoch,
                    hipStream_t s);
+template <typename T>
+int gj_table_begin(const Job<T> *jobs, int nj, int nmax, hipStream_t s);
+template <typename T>
+int gj_table_step(const Job<T> *jobs, int nj, int nmax, int k0, int *flags, int *err, int &epoch, hipStream_t s);
 
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
@@ -383,13 +387,31 @@ struct MfPattern {
     std::vector<int64_t> aoff;
     std::vector<int> aseg, avid;
     std::vector<int2> rows, gemv, bwd, colt, pcolt;
+    std::vector<int64_t> cvo;   // per child-list entry: the child's coupling part in Vf
     // device copies
     MfFront *d_fr = nullptr;
     int *d_fu = nullptr, *d_ch = nullptr, *d_map = nullptr;
     int2 *d_rows = nullptr, *d_gemv = nullptr, *d_bwd = nullptr, *d_colt = nullptr, *d_pcolt = nullptr;
+    int64_t *d_cvo = nullptr;
     int64_t *d_aoff = nullptr;
     int *d_aseg = nullptr, *d_avid = nullptr;
     bool on_device = false;
+};
+
+// Per-kernel timing of a factorisation and a solve (HIP events around every
+// launch on the stream) with the algorithmic counts of what was timed.
+struct MfProbe {
+    enum { kAssemble, kScatter, kSmall, kGjBegin, kGjStep, kGather, kFwd, kBwd, kGroups };
+    double ms[kGroups] = {}, flops[kGroups] = {}, bytes[kGroups] = {};
+    int64_t launches[kGroups] = {};
+    std::vector<hipEvent_t> ev;     // pairs, recorded
+    std::vector<int> grp;           // group of each pair
+    std::vector<hipEvent_t> pool;   // created events for reuse
+    size_t used = 0;
+    ~MfProbe();
+    int begin(hipStream_t s);
+    int end(hipStream_t s, int group, double flops, double bytes);
+    int flush();
 };
 
 // Host analysis only (no device call).
@@ -411,9 +433,10 @@ template <typename T>
 void mf_free(MfWork<T> &W, hipStream_t s);
 // val: the Jacobian in the slot layout (kSlots planes of N).
 template <typename T>
-int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s);
+int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfProbe *probe = nullptr);
 // x = A^-1 b (trans: A^-T b); b and x distinct 2N vectors.
 template <typename T>
-int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hipStream_t s);
+int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hipStream_t s,
+             MfProbe *probe = nullptr);
 
 }  // namespace v2d

@@ -64,6 +64,7 @@ __host__ __device__ inline int row_lanes(int ncol)
 }
 __host__ __device__ inline int block_rows(int ncol) { return 4 * (256 / row_lanes(ncol)); }
 
+
 // Symmetrised node graph of the Jacobian pattern (CSR, no self loops).
 struct NodeGraph {
     std::vector<int64_t> ptr;
@@ -314,6 +315,8 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
         P.nbig += F.big;
         P.flops += 2.0 * (F.big ? F.nsp : F.ns) * (double)F.nf * F.nf;
     }
+    P.cvo.resize(P.ch.size());
+    for (size_t q = 0; q < P.ch.size(); ++q) P.cvo[q] = P.fr[P.ch[q]].vo + P.fr[P.ch[q]].nsp;
     P.F_elems = fo;
     P.V_elems = vo;
     P.Z_elems = zo;
@@ -509,7 +512,7 @@ int mf_upload(MfPattern &P, hipStream_t s)
 {
     if (P.on_device) return 0;
     if (upload(P.d_fr, P.fr, s) || upload(P.d_fu, P.fu, s) || upload(P.d_ch, P.ch, s) ||
-        upload(P.d_map, P.map, s) || upload(P.d_rows, P.rows, s) || upload(P.d_gemv, P.gemv, s) ||
+        upload(P.d_map, P.map, s) || upload(P.d_rows, P.rows, s) || upload(P.d_cvo, P.cvo, s) || upload(P.d_gemv, P.gemv, s) ||
         upload(P.d_bwd, P.bwd, s) || upload(P.d_colt, P.colt, s) || upload(P.d_pcolt, P.pcolt, s) ||
         upload(P.d_aoff, P.aoff, s) || upload(P.d_aseg, P.aseg, s) || upload(P.d_avid, P.avid, s))
         return -1;
@@ -520,11 +523,12 @@ int mf_upload(MfPattern &P, hipStream_t s)
 
 void mf_free_pattern(MfPattern &P, hipStream_t s)
 {
-    void *ptrs[] = {P.d_fr, P.d_fu, P.d_ch, P.d_map, P.d_rows, P.d_gemv, P.d_bwd,
+    void *ptrs[] = {P.d_fr, P.d_fu, P.d_ch, P.d_map, P.d_rows, P.d_cvo, P.d_gemv, P.d_bwd,
                     P.d_colt, P.d_pcolt, P.d_aoff, P.d_aseg, P.d_avid};
     for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
     P.d_fr = nullptr; P.d_fu = P.d_ch = P.d_map = nullptr;
     P.d_rows = P.d_gemv = P.d_bwd = P.d_colt = P.d_pcolt = nullptr;
+    P.d_cvo = nullptr;
     P.d_aoff = nullptr; P.d_aseg = P.d_avid = nullptr;
     P.on_device = false;
 }
@@ -703,7 +707,7 @@ __global__ void __launch_bounds__(256) k_mf_small(const MfFront *__restrict__ fr
 // children's contribution vectors (list order).  One thread per front row.
 template <typename T>
 __global__ void __launch_bounds__(256) k_mf_fgather(const MfFront *__restrict__ fr, const int2 *__restrict__ rows,
-                                                    int64_t r0, int64_t r1, const int *__restrict__ ch,
+                                                    int64_t r0, int64_t r1, const int64_t *__restrict__ cvo,
                                                     const int *__restrict__ map, const int *__restrict__ fu,
                                                     const T *__restrict__ b, T *__restrict__ Vf)
 {
@@ -715,9 +719,7 @@ __global__ void __launch_bounds__(256) k_mf_fgather(const MfFront *__restrict__ 
     T v = a < f.ns ? b[fu[f.uo + a]] : T(0);
     for (int k = 0; k < f.nch; ++k) {
         const int m = map[f.mo + (int64_t)k * f.nf + a];
-        if (m < 0) continue;
-        const MfFront c = fr[ch[f.co + k]];
-        v += Vf[c.vo + c.nsp + m];
+        if (m >= 0) v += Vf[cvo[f.co + k] + m];
     }
     Vf[f.vo + a] = v;
 }
@@ -743,33 +745,53 @@ __device__ __forceinline__ double wave_sum64(double v)
     return v;
 }
 
-// Row products of one front against an LDS vector xs of length ncol starting
-// at column c0 of F: out(r) = sum_c F[r][c0 + c] xs[c], in FP64, for the rows
-// row0 .. row0 + block_rows(ncol) - 1 below rend.  row_lanes(ncol) lanes share
-// a row (16-byte vectors, summed across the group by shuffles).  ncol and c0
-// are multiples of 4 (the vectors stay aligned).
-template <typename T, typename Out>
+// Row products of one front against a vector staged in LDS: out(r) =
+// sum_c F[r][c0 + c] xs[c], in FP64, for the rows row0 + q G + g (q < 4) of a
+// block below rend, G = 256 / L rows per pass, L = row_lanes(ncol) lanes per
+// row (16-byte vectors, summed across the group by shuffles).  The four rows
+// of a lane group are read together, and their first vectors are loaded
+// before stage() fills xs (they do not depend on it).  ncol and c0 are
+// multiples of 4 (the vectors stay aligned); rows past rend read row rend - 1
+// and are dropped.
+template <typename T, typename Stage, typename Out>
 __device__ __forceinline__ void mf_rows(const MfFront &f, const T *F, int row0, int rend, int c0, int ncol,
-                                        const T *xs, Out &&out)
+                                        const T *xs, Stage &&stage, Out &&out)
 {
     using V = typename V16<T>::type;
     constexpr int W = V16<T>::w;
     const int L = row_lanes(ncol), G = 256 / L;
     const int l = threadIdx.x & (L - 1), g = threadIdx.x / L;
     const int nv = ncol / W;
+    const V *R[4];
+    V x0[4];
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
+        const int r = min(row0 + q * G + g, rend - 1);
+        R[q] = reinterpret_cast<const V *>(F + f.fo + (int64_t)r * f.ld + c0);
+        if (nv > 0) x0[q] = R[q][min(l, nv - 1)];
+    }
+    stage();
+    __syncthreads();
+    double acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = (l < nv) ? dot16<T>(x0[q], xs + l * W) : 0.0;
+    for (int c = l + L; c < nv; c += L) {
+        V x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = R[q][c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += dot16<T>(x[q], xs + c * W);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        for (int o = L >> 1; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
         const int r = row0 + q * G + g;
-        double acc = 0.0;
-        if (r < rend) {
-            const V *R = reinterpret_cast<const V *>(F + f.fo + (int64_t)r * f.ld + c0);
-            for (int c = l; c < nv; c += L) acc += dot16<T>(R[c], xs + c * W);
-        }
-        for (int o = L >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-        if (l == 0 && r < rend) out(r, acc);
+        if (l == 0 && r < rend) out(r, acc[q]);
     }
 }
 
-// Forward row products: u = F[:, 0:nsp] v_s; z = u on the pivots (Zf), v_b += u.
+// Forward front products: u = F[:, 0:nsp] v_s (v gathered by k_mf_fgather);
+// z = u on the pivots (Zf), v_b + u on the couplings (Vf, read by the parent).
 template <typename T>
 __global__ void __launch_bounds__(256) k_mf_fgemv(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
                                                   int64_t g0, const T *__restrict__ F, T *__restrict__ Vf,
@@ -779,12 +801,15 @@ __global__ void __launch_bounds__(256) k_mf_fgemv(const MfFront *__restrict__ fr
     T *xs = reinterpret_cast<T *>(smem);
     const int2 fb = blk[g0 + blockIdx.x];
     const MfFront f = fr[fb.x];
-    for (int c = threadIdx.x; c < f.nsp; c += blockDim.x) xs[c] = Vf[f.vo + c];
-    __syncthreads();
-    mf_rows<T>(f, F, fb.y, f.nf, 0, f.nsp, xs, [&](int r, double acc) {
-        if (r < f.nsp) Zf[f.zo + r] = (T)acc;
-        else Vf[f.vo + r] = (T)((double)Vf[f.vo + r] + acc);
-    });
+    mf_rows<T>(
+        f, F, fb.y, f.nf, 0, f.nsp, xs,
+        [&] {
+            for (int a = threadIdx.x; a < f.nsp; a += blockDim.x) xs[a] = Vf[f.vo + a];
+        },
+        [&](int r, double acc) {
+            if (r < f.nsp) Zf[f.zo + r] = (T)acc;
+            else Vf[f.vo + r] = (T)((double)Vf[f.vo + r] + acc);
+        });
 }
 
 // Backward row products: x_s = z - W x_b, x_b gathered from the solution.
@@ -798,13 +823,14 @@ __global__ void __launch_bounds__(256) k_mf_bwd(const MfFront *__restrict__ fr, 
     const int2 fb = blk[b0 + blockIdx.x];
     const MfFront f = fr[fb.x];
     const int nbp = round_up(f.nb, kMfAlign);
-    for (int q = threadIdx.x; q < nbp; q += blockDim.x) xs[q] = q < f.nb ? x[fu[f.uo + f.nsp + q]] : T(0);
-    __syncthreads();
     // the W block is read up to the 4-rounded coupling count: those columns
     // lie inside the row stride and meet zeros in xs
-    mf_rows<T>(f, F, fb.y, f.ns, f.nsp, nbp, xs, [&](int r, double acc) {
-        x[fu[f.uo + r]] = (T)((double)Zf[f.zo + r] - acc);
-    });
+    mf_rows<T>(
+        f, F, fb.y, f.ns, f.nsp, nbp, xs,
+        [&] {
+            for (int q = threadIdx.x; q < nbp; q += blockDim.x) xs[q] = q < f.nb ? x[fu[f.uo + f.nsp + q]] : T(0);
+        },
+        [&](int r, double acc) { x[fu[f.uo + r]] = (T)((double)Zf[f.zo + r] - acc); });
 }
 
 // Column products for the transposed solve: out(c) = sum_r F[r0 + r][c] ws[r]
@@ -866,37 +892,130 @@ __global__ void k_mf_zero_int(int *p) { *p = 0; }
 
 }  // namespace
 
+// ------------------------------------------------------------ probe -----
+MfProbe::~MfProbe()
+{
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+int MfProbe::begin(hipStream_t s)
+{
+    while (pool.size() < used + 2) {
+        hipEvent_t e;
+        V2D_HIP(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    V2D_HIP(hipEventRecord(pool[used], s));
+    return 0;
+}
+
+int MfProbe::end(hipStream_t s, int group, double fl, double by)
+{
+    V2D_HIP(hipEventRecord(pool[used + 1], s));
+    ev.push_back(pool[used]);
+    ev.push_back(pool[used + 1]);
+    grp.push_back(group);
+    used += 2;
+    flops[group] += fl;
+    bytes[group] += by;
+    ++launches[group];
+    return 0;
+}
+
+int MfProbe::flush()
+{
+    if (!ev.empty()) V2D_HIP(hipEventSynchronize(ev.back()));
+    for (size_t q = 0; q < grp.size(); ++q) {
+        float t = 0.f;
+        V2D_HIP(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
+        ms[grp[q]] += t;
+    }
+    ev.clear();
+    grp.clear();
+    used = 0;
+    return 0;
+}
+
+#define MF_BEGIN()                                                                           \
+    do {                                                                                     \
+        if (probe && probe->begin(s)) return -1;                                             \
+    } while (0)
+#define MF_END(g, fl, by)                                                                    \
+    do {                                                                                     \
+        if (probe && probe->end(s, MfProbe::g, (fl), (by))) return -1;                       \
+    } while (0)
+
 template <typename T>
-int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s)
+int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfProbe *probe)
 {
     if (!W.F) {
         set_error("multifrontal: work not allocated");
         return -1;
     }
     W.factored = false;
+    const double ts = sizeof(T);
     hipLaunchKernelGGL(k_mf_zero_int, dim3(1), dim3(1), 0, s, W.err);
     for (int h = 0; h < P.nheight; ++h) {
         const MfLevel &L = P.lv[h];
         const int64_t nr = L.r1 - L.r0;
-        if (nr > 0)
+        // algorithmic counts of the height (host, only when timed)
+        double asm_b = 0, small_fl = 0, small_b = 0;
+        if (probe)
+            for (int f = L.f0; f < L.f1; ++f) {
+                const MfFront &F = P.fr[f];
+                asm_b += ts * (double)F.nf * F.ld;
+                for (int k = 0; k < F.nch; ++k) {
+                    const MfFront &C = P.fr[P.ch[F.co + k]];
+                    asm_b += ts * (double)C.nb * C.nb;
+                }
+                if (!F.big) {
+                    small_fl += 2.0 * F.ns * (double)F.nf * F.nf;
+                    small_b += 2.0 * ts * (double)F.nf * F.nf;
+                }
+            }
+        if (nr > 0) {
+            MF_BEGIN();
             hipLaunchKernelGGL(k_mf_assemble<T>, dim3((unsigned)((nr + 3) / 4)), dim3(256), 0, s, P.d_fr,
                                P.d_rows + L.r0, nr, P.d_ch, P.d_map, W.F);
-        if (L.a1 > L.a0)
+            MF_END(kAssemble, 0.0, asm_b);
+        }
+        if (L.a1 > L.a0) {
+            MF_BEGIN();
             hipLaunchKernelGGL(k_mf_scatter<T>, dim3(grid_for(L.a1 - L.a0)), dim3(256), 0, s, L.a0, L.a1,
                                P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
+            MF_END(kScatter, 0.0, (2 * ts + 12.0) * (double)(L.a1 - L.a0));
+        }
         if (L.fbig > L.f0) {
             const dim3 g((unsigned)(L.fbig - L.f0));
+            MF_BEGIN();
             if (L.nfmax_small <= 64)
                 hipLaunchKernelGGL((k_mf_small<T, 4>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
             else if (L.nfmax_small <= 96)
                 hipLaunchKernelGGL((k_mf_small<T, 6>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
             else
                 hipLaunchKernelGGL((k_mf_small<T, 9>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
+            MF_END(kSmall, small_fl, small_b);
         }
         V2D_LAUNCH();
-        if (L.f1 > L.fbig &&
-            gj_sweep_table<T>(W.jobs + L.job0, L.f1 - L.fbig, L.npmax_big, L.nfmax_big, W.flags, W.err, W.epoch, s))
-            return -1;
+        const int nj = L.f1 - L.fbig;
+        if (nj <= 0 || L.npmax_big <= 0) continue;
+        MF_BEGIN();
+        if (gj_table_begin<T>(W.jobs + L.job0, nj, L.nfmax_big, s)) return -1;
+        MF_END(kGjBegin, 0.0, 0.0);
+        for (int k0 = 0; k0 < L.npmax_big; k0 += kNB) {
+            double fl = 0, by = 0;
+            if (probe)
+                for (int f = L.fbig; f < L.f1; ++f) {
+                    const MfFront &F = P.fr[f];
+                    if (k0 >= F.nsp) continue;
+                    const double n = F.nf, kb = kNB;
+                    fl += 2.0 * (n - kb) * n * kb;
+                    by += ts * (2.0 * (n - kb) * n + (n - kb) * kb + kb * n);
+                }
+            MF_BEGIN();
+            if (gj_table_step<T>(W.jobs + L.job0, nj, L.nfmax_big, k0, W.flags, W.err, W.epoch, s)) return -1;
+            MF_END(kGjStep, fl, by);
+        }
     }
     int e = 0;
     V2D_HIP(hipMemcpyAsync(&e, W.err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -911,7 +1030,7 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s)
 }
 
 template <typename T>
-int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hipStream_t s)
+int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hipStream_t s, MfProbe *probe)
 {
     if (!W.factored) {
         set_error("multifrontal: solve before a factorisation");
@@ -921,16 +1040,31 @@ int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hip
         set_error("multifrontal: right-hand side and solution must be distinct");
         return -1;
     }
+    const double ts = sizeof(T);
     for (int h = 0; h < P.nheight; ++h) {
         const MfLevel &L = P.lv[h];
         int nspmax = 0;
-        for (int f = L.f0; f < L.f1; ++f) nspmax = std::max(nspmax, P.fr[f].nsp);
-        if (L.r1 > L.r0)
+        double fb = 0, gb = 0;
+        for (int f = L.f0; f < L.f1; ++f) {
+            const MfFront &F = P.fr[f];
+            nspmax = std::max(nspmax, F.nsp);
+            if (probe) {
+                fb += ts * ((double)F.nf * F.nsp + F.nsp + 2.0 * F.nf);
+                gb += ts * (2.0 * F.nf) + 4.0 * F.nf * (1 + F.nch);
+            }
+        }
+        if (L.r1 > L.r0) {
+            MF_BEGIN();
             hipLaunchKernelGGL(k_mf_fgather<T>, dim3(grid_for(L.r1 - L.r0)), dim3(256), 0, s, P.d_fr, P.d_rows,
-                               L.r0, L.r1, P.d_ch, P.d_map, P.d_fu, b, W.Vf);
-        if (!trans && L.g1 > L.g0)
+                               L.r0, L.r1, P.d_cvo, P.d_map, P.d_fu, b, W.Vf);
+            MF_END(kGather, 0.0, gb);
+        }
+        if (!trans && L.g1 > L.g0) {
+            MF_BEGIN();
             hipLaunchKernelGGL(k_mf_fgemv<T>, dim3((unsigned)(L.g1 - L.g0)), dim3(256), sizeof(T) * nspmax, s,
                                P.d_fr, P.d_gemv, L.g0, W.F, W.Vf, W.Zf);
+            MF_END(kFwd, 0.0, fb);
+        }
         if (trans && L.c1 > L.c0)
             hipLaunchKernelGGL(k_mf_fgemv_t<T>, dim3((unsigned)(L.c1 - L.c0)), dim3(256),
                                sizeof(double) * 256 + sizeof(T) * nspmax, s, P.d_fr, P.d_colt, L.c0, W.F, W.Vf,
@@ -940,10 +1074,18 @@ int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hip
     for (int h = P.nheight - 1; h >= 0; --h) {
         const MfLevel &L = P.lv[h];
         int nbmax = 0;
-        for (int f = L.f0; f < L.f1; ++f) nbmax = std::max(nbmax, round_up(P.fr[f].nb, kMfAlign));
-        if (!trans && L.b1 > L.b0)
+        double bb = 0;
+        for (int f = L.f0; f < L.f1; ++f) {
+            const MfFront &F = P.fr[f];
+            nbmax = std::max(nbmax, round_up(F.nb, kMfAlign));
+            if (probe) bb += ts * ((double)F.ns * round_up(F.nb, kMfAlign) + F.nb + 2.0 * F.ns);
+        }
+        if (!trans && L.b1 > L.b0) {
+            MF_BEGIN();
             hipLaunchKernelGGL(k_mf_bwd<T>, dim3((unsigned)(L.b1 - L.b0)), dim3(256), sizeof(T) * std::max(nbmax, 1),
                                s, P.d_fr, P.d_bwd, L.b0, W.F, W.Zf, P.d_fu, x);
+            MF_END(kBwd, 0.0, bb);
+        }
         if (trans && L.p1 > L.p0)
             hipLaunchKernelGGL(k_mf_bwd_t<T>, dim3((unsigned)(L.p1 - L.p0)), dim3(256),
                                sizeof(double) * 256 + sizeof(T) * std::max(nbmax, 1), s, P.d_fr, P.d_pcolt, L.p0,
@@ -957,9 +1099,11 @@ template int mf_alloc<float>(const MfPattern &, MfWork<float> &, hipStream_t);
 template int mf_alloc<double>(const MfPattern &, MfWork<double> &, hipStream_t);
 template void mf_free<float>(MfWork<float> &, hipStream_t);
 template void mf_free<double>(MfWork<double> &, hipStream_t);
-template int mf_factor<float>(const MfPattern &, MfWork<float> &, const float *, hipStream_t);
-template int mf_factor<double>(const MfPattern &, MfWork<double> &, const double *, hipStream_t);
-template int mf_solve<float>(const MfPattern &, MfWork<float> &, const float *, float *, bool, hipStream_t);
-template int mf_solve<double>(const MfPattern &, MfWork<double> &, const double *, double *, bool, hipStream_t);
+template int mf_factor<float>(const MfPattern &, MfWork<float> &, const float *, hipStream_t, MfProbe *);
+template int mf_factor<double>(const MfPattern &, MfWork<double> &, const double *, hipStream_t, MfProbe *);
+template int mf_solve<float>(const MfPattern &, MfWork<float> &, const float *, float *, bool, hipStream_t,
+                             MfProbe *);
+template int mf_solve<double>(const MfPattern &, MfWork<double> &, const double *, double *, bool, hipStream_t,
+                              MfProbe *);
 
 }  // namespace v2d

@@ -170,6 +170,23 @@ class DeviceContext:
                 "entries", "positions")
         return dict(zip(keys, (int(v) for v in out)), flops=fl.value)
 
+    MF_GROUPS = ("assemble", "scatter", "small", "gj_begin", "gj_step", "gather", "fwd", "bwd")
+    MF_KERNELS = {"assemble": "k_mf_assemble", "scatter": "k_mf_scatter", "small": "k_mf_small",
+                  "gj_begin": "k_gj_diag_b + k_gj_rowpanel_b", "gj_step": "k_gj_step_mfma_b",
+                  "gather": "k_mf_fgather", "fwd": "k_mf_fgemv", "bwd": "k_mf_bwd"}
+
+    def profile_mf(self) -> dict:
+        """Factorise the last assembled Jacobian and solve once with the
+        multifrontal LU, every launch timed with HIP events on the context's
+        stream: per kernel group ms, launches, algorithmic flops and bytes, and
+        the factorisation / solve spans."""
+        out = np.zeros(34)
+        check(lib.v2d_ctx_profile_mf(self._h, ptr(out)))
+        g = {k: dict(ms=float(out[4 * i]), launches=int(out[4 * i + 1]), flops=float(out[4 * i + 2]),
+                     bytes=float(out[4 * i + 3]), kernel=self.MF_KERNELS[k])
+             for i, k in enumerate(self.MF_GROUPS)}
+        return dict(groups=g, factor_ms=float(out[32]), solve_ms=float(out[33]))
+
     def mf_fronts(self) -> np.ndarray:
         """The factored multifrontal front storage (FP64 copy), for inspection."""
         out = np.empty(self.mf_info()["front_elems"])
