@@ -386,7 +386,7 @@ def main():
         kernels = mf_kernels(mx, "mixed", nsol)
         dom = max((k for k in kernels if "bound" in kernels[k]), key=lambda k: kernels[k]["ms_per_solve"])
         d = kernels[dom]
-        kname = d["kernel"] + "<float>"
+        kname = d["kernel"] + ("<float, TableBatch<float>>" if dom == "gj_step" else "<float>")
         ent = traffic.get(kname) or {}
         alg = d["bytes"] / d["launches"]
         tr = ent["measured_over_algorithmic"] * alg if "measured_over_algorithmic" in ent else None

@@ -9,7 +9,8 @@ newton:   the headline solve: pure Newton from the impulsive start to res <= 1e-
           and ||R||_2 / sqrt(2N) < 1e-10 (mixed: the benchmark's GMRES and
           forcing settings), after one warm-up solve.
 factor:   one factorisation of the steady cylinder Jacobian (initial state,
-          Re 40) after one warm-up factorisation.
+          Re 40) after one warm-up factorisation (multifrontal: and one solve,
+          both event-timed per kernel group, with their algorithmic counts).
 assembly: repeated FP32 device assemblies of the same case.
 copy:     a plain device-to-device copy of a known size (calibrates FETCH_SIZE /
           WRITE_SIZE against bytes that are certainly moved once).
@@ -74,12 +75,15 @@ def main(argv=None):
     ctx.init_state()
     ctx.assemble(a.re)
     if a.what == "factor" and a.solver == "multifrontal":
-        import time
+        # a warm-up factorisation, then one event-timed factorisation and solve:
+        # the algorithmic counts of every kernel group of that pair
         ctx.factor()
-        t0 = time.perf_counter()
-        ctx.factor()
-        print(f"factor {a.size}x{jmax} {a.precision} multifrontal: "
-              f"{(time.perf_counter() - t0) * 1e3:.1f} ms, {ctx.mf_info()['flops']:.3e} flop", flush=True)
+        p = ctx.profile_mf()
+        print(f"factor {a.size}x{jmax} {a.precision} multifrontal: factorisation {p['factor_ms']:.2f} ms, "
+              f"solve {p['solve_ms']:.3f} ms, {ctx.mf_info()['flops']:.3e} flop", flush=True)
+        for k, g in p["groups"].items():
+            print(f"  {g['kernel']:32s} launches {g['launches']:5d}  {g['ms']:9.3f} ms  "
+                  f"algorithmic_bytes={g['bytes']:.0f} algorithmic_flops={g['flops']:.0f}", flush=True)
         return 0
     if a.what == "factor":
         ctx.factor()
