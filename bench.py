@@ -331,7 +331,8 @@ def main():
     ap.add_argument("--refactor-every", type=int, default=0,
                     help="mixed: refactor the FP32 block LU every N Newton steps "
                          "(0 = only when GMRES on the stored one misses its tolerance)")
-    ap.add_argument("--gmres-max", type=int, default=30)
+    ap.add_argument("--gmres-max", type=int, default=10,
+                    help="mixed: GMRES budget on a stored FP32 factorisation before refactoring")
     ap.add_argument("--gmres-rtol", type=float, default=1e-6)
     ap.add_argument("--forcing", type=float, default=0.25,
                     help="mixed: Eisenstat-Walker GMRES tolerance capped at this value (0: off)")

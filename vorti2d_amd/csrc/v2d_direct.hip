@@ -331,6 +331,7 @@ struct TableBatch {
     int nj;
     int kind;
     int par;
+    int tpj;   // grid tiles per job (the largest job's tile count)
     __device__ __forceinline__ Job<T> job(int z) const
     {
         Job<T> J = jt[z];
@@ -339,6 +340,20 @@ struct TableBatch {
             t = J.rowk; J.rowk = J.rowk_w; J.rowk_w = t;
         }
         return J;
+    }
+    // block -> (job, tile): every job's first tile (its pivot tile) first,
+    // then each job's remaining tiles contiguous, so that the workgroups in
+    // flight together share their job's panels in the caches
+    __device__ __forceinline__ void tile(int b, int &z, int &lin) const
+    {
+        if (b < nj) {
+            z = b;
+            lin = 0;
+        } else {
+            const int q = b - nj;
+            z = q / (tpj - 1);
+            lin = 1 + q % (tpj - 1);
+        }
     }
 };
 
@@ -982,14 +997,20 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
     constexpr int VW = Vec8<T>::w;
     constexpr int LT = NB + 4;    // padded LDS row stride
     __shared__ T sT[NB * LT];     // next pivot block, or a tile stored transposed
-    const int z = blockIdx.x % B.nj;
+    constexpr bool kTable = !std::is_same<BT, Batch<T>>::value;
+    int z, lin;
+    if constexpr (kTable) {
+        B.tile(blockIdx.x, z, lin);
+    } else {
+        z = blockIdx.x % B.nj;
+        lin = blockIdx.x / B.nj;
+    }
     const Job<T> &J = B.job(z);
     T *__restrict__ A = J.S;
     const int n = J.n, ld = J.ld, np = J.np;
     if (k0 >= np) return;
     const int kb = min(NB, np - k0);
     const int nt = (n + NB - 1) / NB;
-    int lin = blockIdx.x / B.nj;
     if (lin >= nt * nt) return;
     const int k1 = k0 + NB;
     if (k1 < np) {
@@ -1020,78 +1041,170 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
     const bool colK = c0 == k0;            // the pivot columns: old values are zero
     const bool cfull = c0 + NB <= n;       // no column of the tile past n
     // ---------------------------------------------------- tile update ----
-    const T *pa[2], *pb[2];
+    C acc[2][2], old[2][2];
+    if constexpr (kTable && sizeof(T) == 4) {
+        // Job tables (many fronts per launch, the bulk update dominates): every
+        // load of the tile goes out before the first matrix-core product --
+        // the panels of kPf 16-wide k-slices at a time (all four in FP32), then
+        // the old tile from clamped rows and columns (zero selected outside
+        // the matrix), so a tile waits for one memory round trip instead of
+        // one per slice and per sub-tile.  FP32 only: in FP64 the registers
+        // this takes halve the occupancy and the launch is slower.
+        constexpr int kPf = 4;
+        const V *qa[2], *qb[2];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi) pa[bi] = J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk;
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj) pb[bj] = J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk;
-    C old[2][2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            if (M::kSwap && cfull && !colK) {
-                const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
-                if (r < n) {
-                    const float4 x = *reinterpret_cast<const float4 *>(A + (int64_t)r * ld + c);
-                    old[bi][bj][0] = x.x; old[bi][bj][1] = x.y; old[bi][bj][2] = x.z; old[bi][bj][3] = x.w;
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) old[bi][bj][v] = T(0);
-                }
-            } else {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
-                    old[bi][bj][v] = (!colK && r < n && c < n) ? A[(int64_t)r * ld + c] : T(0);
-                }
-            }
-        }
-    C acc[2][2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < 2; ++bi)
+            qa[bi] = reinterpret_cast<const V *>(J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk);
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
-    const bool tail = kb < NB;
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        T av[2][4], bv[2][4];
+            qb[bj] = reinterpret_cast<const V *>(J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk);
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-            for (int u = 0; u < 4 / VW; ++u) {
-                const V x = reinterpret_cast<const V *>(pa[bi] + 16 * h)[u];
+            for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-                for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = -reinterpret_cast<const T *>(&x)[e];
+                for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
+        const bool tail = kb < NB;
+#pragma unroll
+        for (int h0 = 0; h0 < 4; h0 += kPf) {
+            V xa[kPf][2][4 / VW], xb[kPf][2][4 / VW];
+#pragma unroll
+            for (int hh = 0; hh < kPf; ++hh)
+#pragma unroll
+                for (int b2 = 0; b2 < 2; ++b2)
+#pragma unroll
+                    for (int u = 0; u < 4 / VW; ++u) {
+                        xa[hh][b2][u] = qa[b2][(16 * (h0 + hh)) / VW + u];
+                        xb[hh][b2][u] = qb[b2][(16 * (h0 + hh)) / VW + u];
+                    }
+            if (h0 == 0) {
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        if (M::kSwap) {
+                            const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                            const float4 x = *reinterpret_cast<const float4 *>(
+                                A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4));
+                            const bool in = !colK && r < n && c < n;
+                            old[bi][bj][0] = in ? x.x : T(0);
+                            old[bi][bj][1] = in ? x.y : T(0);
+                            old[bi][bj][2] = in ? x.z : T(0);
+                            old[bi][bj][3] = in ? x.w : T(0);
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
+                                const T x = A[(int64_t)min(r, n - 1) * ld + min(c, n - 1)];
+                                old[bi][bj][v] = (!colK && r < n && c < n) ? x : T(0);
+                            }
+                        }
+                    }
             }
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+            for (int hh = 0; hh < kPf; ++hh) {
+                const int h = h0 + hh;
+                T av[2][4], bv[2][4];
 #pragma unroll
-            for (int u = 0; u < 4 / VW; ++u) {
-                const V x = reinterpret_cast<const V *>(pb[bj] + 16 * h)[u];
+                for (int b2 = 0; b2 < 2; ++b2)
 #pragma unroll
-                for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = reinterpret_cast<const T *>(&x)[e];
+                    for (int u = 0; u < 4 / VW; ++u)
+#pragma unroll
+                        for (int e = 0; e < VW; ++e) {
+                            av[b2][u * VW + e] = -reinterpret_cast<const T *>(&xa[hh][b2][u])[e];
+                            bv[b2][u * VW + e] = reinterpret_cast<const T *>(&xb[hh][b2][u])[e];
+                        }
+                if (tail) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
             }
-        if (tail) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
+    } else {
+        const T *pa[2], *pb[2];
+    #pragma unroll
+        for (int bi = 0; bi < 2; ++bi) pa[bi] = J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk;
+    #pragma unroll
+        for (int bj = 0; bj < 2; ++bj) pb[bj] = J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk;
+    #pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+    #pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                if (M::kSwap && cfull && !colK) {
+                    const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                    if (r < n) {
+                        const float4 x = *reinterpret_cast<const float4 *>(A + (int64_t)r * ld + c);
+                        old[bi][bj][0] = x.x; old[bi][bj][1] = x.y; old[bi][bj][2] = x.z; old[bi][bj][3] = x.w;
+                    } else {
+    #pragma unroll
+                        for (int v = 0; v < 4; ++v) old[bi][bj][v] = T(0);
+                    }
+                } else {
+    #pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
+                        old[bi][bj][v] = (!colK && r < n && c < n) ? A[(int64_t)r * ld + c] : T(0);
+                    }
+                }
+            }
+    #pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+    #pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+    #pragma unroll
+                for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
+        const bool tail = kb < NB;
+    #pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            T av[2][4], bv[2][4];
+    #pragma unroll
             for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+    #pragma unroll
+                for (int u = 0; u < 4 / VW; ++u) {
+                    const V x = reinterpret_cast<const V *>(pa[bi] + 16 * h)[u];
+    #pragma unroll
+                    for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = -reinterpret_cast<const T *>(&x)[e];
+                }
+    #pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+    #pragma unroll
+                for (int u = 0; u < 4 / VW; ++u) {
+                    const V x = reinterpret_cast<const V *>(pb[bj] + 16 * h)[u];
+    #pragma unroll
+                    for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = reinterpret_cast<const T *>(&x)[e];
+                }
+            if (tail) {
+    #pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+            }
+    #pragma unroll
+            for (int e = 0; e < 4; ++e)
+    #pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+    #pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+        }
+    #pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+    #pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+    #pragma unroll
+                for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
     }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
     // ------------------------------------------------ ordinary tiles ----
     if (!is_piv && !is_prow) {
 #pragma unroll
@@ -1330,7 +1443,7 @@ int gj_table_begin(const Job<T> *jobs, int nj, int nmax, hipStream_t s)
         return -1;
     }
     const unsigned npr = (unsigned)((nmax + RP - 1) / RP);
-    TableBatch<T> B{jobs, nj, kUpdateMfma64, 0};
+    TableBatch<T> B{jobs, nj, kUpdateMfma64, 0, 0};
     hipLaunchKernelGGL((k_gj_diag_b<T, TableBatch<T>>), dim3(1, 1, (unsigned)nj), dim3(256), 0, s, B, 0);
     hipLaunchKernelGGL((k_gj_rowpanel_b<T, TableBatch<T>>), dim3(npr, 2, (unsigned)nj), dim3(256), 0, s, B, 0);
     V2D_LAUNCH();
@@ -1341,7 +1454,11 @@ template <typename T>
 int gj_table_step(const Job<T> *jobs, int nj, int nmax, int k0, int *flags, int *err, int &epoch, hipStream_t s)
 {
     const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
-    TableBatch<T> B{jobs, nj, kUpdateMfma64, (k0 / NB) & 1};
+    TableBatch<T> B{jobs, nj, kUpdateMfma64, (k0 / NB) & 1, (int)(nt * nt)};
+    if (nt * nt < 2) {
+        set_error("Gauss-Jordan table: a job of one tile has no step");
+        return -1;
+    }
     hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>>), dim3(nt * nt * (unsigned)nj), dim3(256), 0, s, B, k0,
                        ++epoch, flags, err);
     V2D_LAUNCH();

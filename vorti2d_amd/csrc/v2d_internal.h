@@ -354,6 +354,8 @@ struct MfFront {
     int64_t zo;    // pivot-part vector, nsp entries
     int64_t uo;    // unknowns of the front rows (field-blocked global index, -1: padding)
     int64_t mo;    // child maps: nch arrays of nf entries (child's struct row or -1)
+    int64_t ao;    // first of the front's assembly positions
+    int na;        // number of them
     int ns, nsp, nb, nf, ld;
     int nch, co;   // children: ch[co .. co + nch)
     int big;       // 1: swept by the dataflow Gauss-Jordan steps (nsp a multiple of 64)
@@ -367,6 +369,7 @@ struct MfLevel {
     int64_t c0, c1;          // transposed column blocks (64 columns of all front columns)
     int64_t p0, p1;          // transposed column blocks of the pivot columns
     int64_t a0, a1;          // assembly positions
+    int64_t rbig, abig;      // rows and positions of the big fronts: [rbig, r1), [abig, a1)
     int nfmax_small, nfmax_big, npmax_big;
     int64_t pool;            // Gauss-Jordan panel elements of the big fronts
     int64_t job0;            // first job of the level in the job table

@@ -428,7 +428,9 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
             }
             if (!pv.empty()) P.aseg.push_back((int)P.avid.size());
             for (size_t q = 0; q < st[s].size(); ++q) idx[st[s][q]] = -1;
+            P.fr[f].ao = f ? pend[f - 1] : 0;
             pend[f] = (int64_t)P.aoff.size();
+            P.fr[f].na = (int)(pend[f] - P.fr[f].ao);
         }
         P.npos = (int64_t)P.aoff.size();
         // level ranges of the positions
@@ -477,6 +479,19 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
             }
         }
         L.r1 = (int64_t)P.rows.size();
+        L.rbig = L.r1;
+        L.abig = L.a1;
+        {
+            int64_t r = L.r0;
+            for (int f = L.f0; f < L.f1; ++f) {
+                if (P.fr[f].big) {
+                    L.rbig = r;
+                    L.abig = P.fr[f].ao;
+                    break;
+                }
+                r += P.fr[f].nf;
+            }
+        }
         L.g1 = (int64_t)P.gemv.size();
         L.b1 = (int64_t)P.bwd.size();
         L.c1 = (int64_t)P.colt.size();
@@ -638,29 +653,77 @@ __global__ void __launch_bounds__(256) k_mf_scatter(int64_t u0, int64_t u1, cons
     F[aoff[u]] += acc;
 }
 
-// Partial Gauss-Jordan of one small front (pivots 0 .. ns-1, nf <= 16 NI), one
-// workgroup per front, the front in registers: thread (ty, tx) of a 16 x 16
-// grid holds rows ty + 16 i and columns tx + 16 j.  Per pivot p the owners of
+// A small front assembled and swept in one workgroup: its rows are built in
+// LDS in chunks of at most kSmallLds bytes -- identity on the padding pivots,
+// the children's contribution blocks gathered in list order, then the
+// Jacobian entries it owns (the same sums in the same order as k_mf_assemble
+// then k_mf_scatter) -- and loaded into registers: thread (ty, tx) of a
+// 16 x 16 grid holds rows ty + 16 i and columns tx + 16 j (nf <= 16 NI).
+// Then partial Gauss-Jordan on pivots 0 .. ns-1: per pivot p the owners of
 // row p and of column p stage them in LDS (two buffers, alternating, so one
-// barrier per pivot), then every entry takes the rank-1 update and the pivot
-// row and column are set: A[p][c] /= A[p][p], A[r][p] = -A[r][p] / A[p][p],
+// barrier per pivot), every entry takes the rank-1 update and the pivot row
+// and column are set: A[p][c] /= A[p][p], A[r][p] = -A[r][p] / A[p][p],
 // A[p][p] = 1 / A[p][p].
+constexpr int kSmallLds = 32768;
+
 template <typename T, int NI>
-__global__ void __launch_bounds__(256) k_mf_small(const MfFront *__restrict__ fr, int f0, T *__restrict__ F)
+__global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restrict__ fr, int f0,
+                                                        const int *__restrict__ ch, const int *__restrict__ map,
+                                                        const int64_t *__restrict__ aoff,
+                                                        const int *__restrict__ aseg, const int *__restrict__ avid,
+                                                        const T *__restrict__ val, T *__restrict__ F)
 {
+    __shared__ T sF[kSmallLds / sizeof(T)];
     __shared__ T prow[2][16 * NI], pcol[2][16 * NI];
     const MfFront f = fr[f0 + blockIdx.x];
     const int nf = f.nf, ns = f.ns;
     const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-    T *g = F + f.fo;
+    const int cr = min(nf, (int)(kSmallLds / sizeof(T)) / nf);
     T a[NI][NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < NI; ++j) {
-            const int r = ty + 16 * i, c = tx + 16 * j;
-            a[i][j] = (r < nf && c < nf) ? g[(int64_t)r * f.ld + c] : T(0);
+        for (int j = 0; j < NI; ++j) a[i][j] = T(0);
+    for (int r0 = 0; r0 < nf; r0 += cr) {
+        const int r1 = min(nf, r0 + cr), ne = (r1 - r0) * nf;
+        // every element e is zeroed and summed by the same thread
+        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+            const int r = r0 + e / nf, c = e % nf;
+            sF[e] = (r >= ns && r < f.nsp && c == r) ? T(1) : T(0);
         }
+        for (int k = 0; k < f.nch; ++k) {
+            const int *m = map + f.mo + (int64_t)k * nf;
+            const MfFront c = fr[ch[f.co + k]];
+            for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+                const int ia = m[r0 + e / nf];
+                if (ia < 0) continue;
+                const int ib = m[e % nf];
+                if (ib < 0) continue;
+                sF[e] += F[c.fo + (int64_t)(c.nsp + ia) * c.ld + c.nsp + ib];
+            }
+        }
+        __syncthreads();
+        for (int64_t u = f.ao + threadIdx.x; u < f.ao + f.na; u += blockDim.x) {
+            const int64_t q = aoff[u] - f.fo;
+            const int r = (int)(q / f.ld), c = (int)(q % f.ld);
+            if (r < r0 || r >= r1) continue;
+            T acc = T(0);
+            for (int e = aseg[u]; e < aseg[u + 1]; ++e) acc += val[avid[e]];
+            sF[(r - r0) * nf + c] += acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int r = ty + 16 * i;
+            if (r >= r0 && r < r1)
+#pragma unroll
+                for (int j = 0; j < NI; ++j) {
+                    const int c = tx + 16 * j;
+                    if (c < nf) a[i][j] = sF[(r - r0) * nf + c];
+                }
+        }
+        __syncthreads();
+    }
     int par = 0;
 #pragma unroll
     for (int I = 0; I < NI; ++I) {
@@ -694,6 +757,7 @@ __global__ void __launch_bounds__(256) k_mf_small(const MfFront *__restrict__ fr
             par ^= 1;
         }
     }
+    T *g = F + f.fo;
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
@@ -701,6 +765,11 @@ __global__ void __launch_bounds__(256) k_mf_small(const MfFront *__restrict__ fr
             const int r = ty + 16 * i, c = tx + 16 * j;
             if (r < nf && c < nf) g[(int64_t)r * f.ld + c] = a[i][j];
         }
+    // the row stride's tail (columns nf .. ld) stays zero, as k_mf_assemble leaves it
+    for (int e = threadIdx.x; e < nf * (f.ld - nf); e += blockDim.x) {
+        const int r = e / (f.ld - nf), c = nf + e % (f.ld - nf);
+        g[(int64_t)r * f.ld + c] = T(0);
+    }
 }
 
 // Forward gather: v = b on the pivots, 0 on padding and couplings, plus the
@@ -963,38 +1032,44 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
         if (probe)
             for (int f = L.f0; f < L.f1; ++f) {
                 const MfFront &F = P.fr[f];
-                asm_b += ts * (double)F.nf * F.ld;
+                double cb = 0;
                 for (int k = 0; k < F.nch; ++k) {
                     const MfFront &C = P.fr[P.ch[F.co + k]];
-                    asm_b += ts * (double)C.nb * C.nb;
+                    cb += ts * (double)C.nb * C.nb;
                 }
-                if (!F.big) {
+                if (F.big) {
+                    asm_b += ts * (double)F.nf * F.ld + cb;
+                } else {
                     small_fl += 2.0 * F.ns * (double)F.nf * F.nf;
-                    small_b += 2.0 * ts * (double)F.nf * F.nf;
+                    small_b += ts * (double)F.nf * F.ld + cb + (ts + 12.0) * F.na;
                 }
             }
-        if (nr > 0) {
-            MF_BEGIN();
-            hipLaunchKernelGGL(k_mf_assemble<T>, dim3((unsigned)((nr + 3) / 4)), dim3(256), 0, s, P.d_fr,
-                               P.d_rows + L.r0, nr, P.d_ch, P.d_map, W.F);
-            MF_END(kAssemble, 0.0, asm_b);
-        }
-        if (L.a1 > L.a0) {
-            MF_BEGIN();
-            hipLaunchKernelGGL(k_mf_scatter<T>, dim3(grid_for(L.a1 - L.a0)), dim3(256), 0, s, L.a0, L.a1,
-                               P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
-            MF_END(kScatter, 0.0, (2 * ts + 12.0) * (double)(L.a1 - L.a0));
-        }
         if (L.fbig > L.f0) {
             const dim3 g((unsigned)(L.fbig - L.f0));
             MF_BEGIN();
             if (L.nfmax_small <= 64)
-                hipLaunchKernelGGL((k_mf_small<T, 4>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
+                hipLaunchKernelGGL((k_mf_small_fused<T, 4>), g, dim3(256), 0, s, P.d_fr, L.f0, P.d_ch, P.d_map,
+                                   P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
             else if (L.nfmax_small <= 96)
-                hipLaunchKernelGGL((k_mf_small<T, 6>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
+                hipLaunchKernelGGL((k_mf_small_fused<T, 6>), g, dim3(256), 0, s, P.d_fr, L.f0, P.d_ch, P.d_map,
+                                   P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
             else
-                hipLaunchKernelGGL((k_mf_small<T, 9>), g, dim3(256), 0, s, P.d_fr, L.f0, W.F);
+                hipLaunchKernelGGL((k_mf_small_fused<T, 9>), g, dim3(256), 0, s, P.d_fr, L.f0, P.d_ch, P.d_map,
+                                   P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
             MF_END(kSmall, small_fl, small_b);
+        }
+        const int64_t nrb = L.r1 - L.rbig;
+        if (nrb > 0) {
+            MF_BEGIN();
+            hipLaunchKernelGGL(k_mf_assemble<T>, dim3((unsigned)((nrb + 3) / 4)), dim3(256), 0, s, P.d_fr,
+                               P.d_rows + L.rbig, nrb, P.d_ch, P.d_map, W.F);
+            MF_END(kAssemble, 0.0, asm_b);
+        }
+        if (L.a1 > L.abig) {
+            MF_BEGIN();
+            hipLaunchKernelGGL(k_mf_scatter<T>, dim3(grid_for(L.a1 - L.abig)), dim3(256), 0, s, L.abig, L.a1,
+                               P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
+            MF_END(kScatter, 0.0, (2 * ts + 12.0) * (double)(L.a1 - L.abig));
         }
         V2D_LAUNCH();
         const int nj = L.f1 - L.fbig;

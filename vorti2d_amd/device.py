@@ -171,7 +171,7 @@ class DeviceContext:
         return dict(zip(keys, (int(v) for v in out)), flops=fl.value)
 
     MF_GROUPS = ("assemble", "scatter", "small", "gj_begin", "gj_step", "gather", "fwd", "bwd")
-    MF_KERNELS = {"assemble": "k_mf_assemble", "scatter": "k_mf_scatter", "small": "k_mf_small",
+    MF_KERNELS = {"assemble": "k_mf_assemble", "scatter": "k_mf_scatter", "small": "k_mf_small_fused",
                   "gj_begin": "k_gj_diag_b + k_gj_rowpanel_b", "gj_step": "k_gj_step_mfma_b",
                   "gather": "k_mf_fgather", "fwd": "k_mf_fgemv", "bwd": "k_mf_bwd"}
 
