@@ -370,6 +370,8 @@ struct MfLevel {
     int64_t p0, p1;          // transposed column blocks of the pivot columns
     int64_t a0, a1;          // assembly positions
     int64_t rbig, abig;      // rows and positions of the big fronts: [rbig, r1), [abig, a1)
+    int64_t s0, s1;          // assembly blocks of the big fronts
+    int nch2;                // 1: every big front of the height has two children
     int nfmax_small, nfmax_big, npmax_big;
     int64_t pool;            // Gauss-Jordan panel elements of the big fronts
     int64_t job0;            // first job of the level in the job table
@@ -390,11 +392,13 @@ struct MfPattern {
     std::vector<int64_t> aoff;
     std::vector<int> aseg, avid;
     std::vector<int2> rows, gemv, bwd, colt, pcolt;
+    std::vector<int2> asmb;     // assembly blocks of the big fronts: (front, first row)
     std::vector<int64_t> cvo;   // per child-list entry: the child's coupling part in Vf
     // device copies
     MfFront *d_fr = nullptr;
     int *d_fu = nullptr, *d_ch = nullptr, *d_map = nullptr;
     int2 *d_rows = nullptr, *d_gemv = nullptr, *d_bwd = nullptr, *d_colt = nullptr, *d_pcolt = nullptr;
+    int2 *d_asmb = nullptr;
     int64_t *d_cvo = nullptr;
     int64_t *d_aoff = nullptr;
     int *d_aseg = nullptr, *d_avid = nullptr;

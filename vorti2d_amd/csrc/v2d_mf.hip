@@ -64,6 +64,15 @@ __host__ __device__ inline int row_lanes(int ncol)
 }
 __host__ __device__ inline int block_rows(int ncol) { return 4 * (256 / row_lanes(ncol)); }
 
+// Rows of one assembly block of a big front of row stride ld (a multiple of
+// 4): about kAsmItems items of 4 columns.
+constexpr int kAsmItems = 4096;
+__host__ __device__ inline int asm_rows(int ld)
+{
+    const int r = kAsmItems / (ld / 4);
+    return r > 1 ? r : 1;
+}
+
 
 // Symmetrised node graph of the Jacobian pattern (CSR, no self loops).
 struct NodeGraph {
@@ -461,6 +470,8 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
         L.b0 = (int64_t)P.bwd.size();
         L.c0 = (int64_t)P.colt.size();
         L.p0 = (int64_t)P.pcolt.size();
+        L.s0 = (int64_t)P.asmb.size();
+        L.nch2 = 1;
         L.nfmax_small = L.nfmax_big = L.npmax_big = 0;
         L.pool = 0;
         for (int f = L.f0; f < L.f1; ++f) {
@@ -471,6 +482,8 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
             for (int c = 0; c < F.nf; c += 64) P.colt.push_back(make_int2(f, c));
             for (int c = 0; c < F.ns; c += 64) P.pcolt.push_back(make_int2(f, c));
             if (F.big) {
+                for (int r = 0; r < F.nf; r += asm_rows(F.ld)) P.asmb.push_back(make_int2(f, r));
+                if (F.nch != 2) L.nch2 = 0;
                 L.nfmax_big = std::max(L.nfmax_big, F.nf);
                 L.npmax_big = std::max(L.npmax_big, F.nsp);
                 L.pool += 2 * (int64_t)F.nf * kNB + 2 * (int64_t)round_up(F.nf, kNB) * kNB + kNB * kNB;
@@ -496,6 +509,7 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
         L.b1 = (int64_t)P.bwd.size();
         L.c1 = (int64_t)P.colt.size();
         L.p1 = (int64_t)P.pcolt.size();
+        L.s1 = (int64_t)P.asmb.size();
         pool_max = std::max(pool_max, L.pool);
     }
     P.pool_elems = pool_max;
@@ -528,7 +542,7 @@ int mf_upload(MfPattern &P, hipStream_t s)
     if (P.on_device) return 0;
     if (upload(P.d_fr, P.fr, s) || upload(P.d_fu, P.fu, s) || upload(P.d_ch, P.ch, s) ||
         upload(P.d_map, P.map, s) || upload(P.d_rows, P.rows, s) || upload(P.d_cvo, P.cvo, s) || upload(P.d_gemv, P.gemv, s) ||
-        upload(P.d_bwd, P.bwd, s) || upload(P.d_colt, P.colt, s) || upload(P.d_pcolt, P.pcolt, s) ||
+        upload(P.d_bwd, P.bwd, s) || upload(P.d_colt, P.colt, s) || upload(P.d_pcolt, P.pcolt, s) || upload(P.d_asmb, P.asmb, s) ||
         upload(P.d_aoff, P.aoff, s) || upload(P.d_aseg, P.aseg, s) || upload(P.d_avid, P.avid, s))
         return -1;
     V2D_HIP(hipStreamSynchronize(s));   // the host vectors may change after this returns
@@ -539,10 +553,10 @@ int mf_upload(MfPattern &P, hipStream_t s)
 void mf_free_pattern(MfPattern &P, hipStream_t s)
 {
     void *ptrs[] = {P.d_fr, P.d_fu, P.d_ch, P.d_map, P.d_rows, P.d_cvo, P.d_gemv, P.d_bwd,
-                    P.d_colt, P.d_pcolt, P.d_aoff, P.d_aseg, P.d_avid};
+                    P.d_colt, P.d_pcolt, P.d_asmb, P.d_aoff, P.d_aseg, P.d_avid};
     for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
     P.d_fr = nullptr; P.d_fu = P.d_ch = P.d_map = nullptr;
-    P.d_rows = P.d_gemv = P.d_bwd = P.d_colt = P.d_pcolt = nullptr;
+    P.d_rows = P.d_gemv = P.d_bwd = P.d_colt = P.d_pcolt = P.d_asmb = nullptr;
     P.d_cvo = nullptr;
     P.d_aoff = nullptr; P.d_aseg = P.d_avid = nullptr;
     P.on_device = false;
@@ -607,36 +621,101 @@ void mf_free(MfWork<T> &W, hipStream_t s)
 // ------------------------------------------------------------ kernels ----
 namespace {
 
-// F rows of the fronts of one height: the children's contribution blocks
-// gathered (children in list order), identity on the padding pivots; one
-// wavefront per row, four rows per workgroup.
-template <typename T>
-__global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__ fr, const int2 *__restrict__ rows,
-                                                     int64_t nrows, const int *__restrict__ ch,
-                                                     const int *__restrict__ map, T *__restrict__ F)
+// F rows of the big fronts of one height: identity on the padding pivots plus
+// the children's contribution blocks, summed in child-list order.  A
+// workgroup writes asm_rows(ld) rows of one front as items of 4 consecutive
+// columns (one 16-byte store in FP32), U items per thread and round.  Each
+// round issues every map load of its items, then every contribution load,
+// before the first sum, so a thread waits for two memory round trips per
+// round rather than two per child and column.  NC = 2 (every big front of
+// the O-grid dissection has two children) keeps both children's loads in one
+// batch; NC = 0 takes any child count, one child at a time.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
+                                                     const int *__restrict__ ch, const int *__restrict__ map,
+                                                     T *__restrict__ F)
 {
-    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= nrows) return;
-    const int lane = threadIdx.x & 63;
-    const int2 fa = rows[t];
-    const MfFront f = fr[fa.x];
-    const int a = fa.y;
-    T *dst = F + f.fo + (int64_t)a * f.ld;
-    const bool pad = a >= f.ns && a < f.nsp;
-    for (int b = lane; b < f.ld; b += 64) {
-        T acc = (pad && b == a) ? T(1) : T(0);
-        if (b < f.nf) {
-            for (int k = 0; k < f.nch; ++k) {
-                const int *m = map + f.mo + (int64_t)k * f.nf;
-                const int ia = m[a];
-                if (ia < 0) continue;
-                const int ib = m[b];
-                if (ib < 0) continue;
-                const MfFront c = fr[ch[f.co + k]];
-                acc += F[c.fo + (int64_t)(c.nsp + ia) * c.ld + c.nsp + ib];
+    constexpr int U = 4;
+    constexpr int KC = NC > 0 ? NC : 1;
+    const int2 fb = blk[blockIdx.x];
+    const MfFront f = fr[fb.x];
+    const int nf = f.nf, ld4 = f.ld >> 2;
+    const int a0 = fb.y;
+    const int items = min(asm_rows(f.ld), nf - a0) * ld4;
+    const float inv = 1.0f / (float)ld4;
+    T *dst0 = F + f.fo + (int64_t)a0 * f.ld;
+    const int nk = NC > 0 ? NC : f.nch;
+    for (int e0 = 0; e0 < items; e0 += 256 * U) {
+        int r[U], c[U];
+        bool ok[U];
+        T acc[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * 256 + (int)threadIdx.x;
+            ok[u] = e < items;
+            int q = (int)((float)e * inv);   // e / ld4, corrected below
+            if (q * ld4 > e) --q;
+            else if ((q + 1) * ld4 <= e) ++q;
+            r[u] = q;
+            c[u] = 4 * (e - q * ld4);
+            const int a = a0 + q;
+            const bool pad = a >= f.ns && a < f.nsp;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] = (pad && c[u] + v == a) ? T(1) : T(0);
+        }
+        for (int k0 = 0; k0 < nk; k0 += KC) {
+            int ia[KC][U], ib[KC][U][4];
+            const T *cb[KC];
+            int cld[KC];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+                const MfFront cf = fr[ch[f.co + k0 + kk]];
+                cb[kk] = F + cf.fo + (int64_t)cf.nsp * cf.ld + cf.nsp;
+                cld[kk] = cf.ld;
+                const int *m = map + f.mo + (int64_t)(k0 + kk) * nf;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ia[kk][u] = ok[u] ? m[a0 + r[u]] : -1;
+                    // nf is even: columns c, c + 1 exist together, as do c + 2, c + 3
+                    const int2 lo = (ok[u] && c[u] < nf) ? *reinterpret_cast<const int2 *>(m + c[u]) : make_int2(-1, -1);
+                    const int2 hi = (ok[u] && c[u] + 2 < nf) ? *reinterpret_cast<const int2 *>(m + c[u] + 2)
+                                                             : make_int2(-1, -1);
+                    ib[kk][u][0] = lo.x;
+                    ib[kk][u][1] = lo.y;
+                    ib[kk][u][2] = hi.x;
+                    ib[kk][u][3] = hi.y;
+                }
+            }
+            T x[KC][U][4];
+            bool in[KC][U][4];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        in[kk][u][v] = ia[kk][u] >= 0 && ib[kk][u][v] >= 0;
+                        x[kk][u][v] = in[kk][u][v] ? cb[kk][ia[kk][u] * cld[kk] + ib[kk][u][v]] : T(0);
+                    }
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (in[kk][u][v]) acc[u][v] += x[kk][u][v];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!ok[u]) continue;
+            T *d = dst0 + (int64_t)r[u] * f.ld + c[u];
+            if constexpr (sizeof(T) == 4) {
+                *reinterpret_cast<float4 *>(d) = make_float4(acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
+            } else {
+                reinterpret_cast<double2 *>(d)[0] = make_double2(acc[u][0], acc[u][1]);
+                reinterpret_cast<double2 *>(d)[1] = make_double2(acc[u][2], acc[u][3]);
             }
         }
-        dst[b] = acc;
     }
 }
 
@@ -1058,11 +1137,15 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
                                    P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
             MF_END(kSmall, small_fl, small_b);
         }
-        const int64_t nrb = L.r1 - L.rbig;
-        if (nrb > 0) {
+        if (L.s1 > L.s0) {
+            const dim3 g((unsigned)(L.s1 - L.s0));
             MF_BEGIN();
-            hipLaunchKernelGGL(k_mf_assemble<T>, dim3((unsigned)((nrb + 3) / 4)), dim3(256), 0, s, P.d_fr,
-                               P.d_rows + L.rbig, nrb, P.d_ch, P.d_map, W.F);
+            if (L.nch2)
+                hipLaunchKernelGGL((k_mf_assemble<T, 2>), g, dim3(256), 0, s, P.d_fr, P.d_asmb + L.s0, P.d_ch,
+                                   P.d_map, W.F);
+            else
+                hipLaunchKernelGGL((k_mf_assemble<T, 0>), g, dim3(256), 0, s, P.d_fr, P.d_asmb + L.s0, P.d_ch,
+                                   P.d_map, W.F);
             MF_END(kAssemble, 0.0, asm_b);
         }
         if (L.a1 > L.abig) {
