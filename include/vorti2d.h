@@ -194,6 +194,15 @@ int v2d_ctx_get_sweep_kernel(v2d_ctx *ctx, int *kind, int64_t *fallbacks);
 enum { V2D_SOLVER_BLOCK_LU = 0, V2D_SOLVER_MULTIFRONTAL = 1 };
 int v2d_ctx_set_linear_solver(v2d_ctx *ctx, int kind);
 int v2d_ctx_get_linear_solver(v2d_ctx *ctx, int *kind);
+/* Schedule of the multifrontal factorisation's Gauss-Jordan steps (no
+ * counterpart in the reference; a tuning choice of this library): paired != 0
+ * (the default) runs them in pairs, a step that updates only the tiles which
+ * form the next pivot panels followed by a step that applies both steps'
+ * panels to every other tile in one rank-128 product (one pass over each
+ * front per two steps); 0 runs every step over the whole front.  The two
+ * schedules give the same factorisation up to rounding.  Takes effect at the
+ * next factorisation. */
+int v2d_ctx_set_step_pairing(v2d_ctx *ctx, int paired);
 /* Sizes of the multifrontal analysis of a context's grid (built on first
  * call): out[0..7] = fronts, tree heights, fronts swept by the matrix-core
  * Gauss-Jordan steps, front storage (elements), front-vector elements,

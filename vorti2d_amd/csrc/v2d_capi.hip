@@ -929,6 +929,14 @@ int v2d_ctx_set_linear_solver(v2d_ctx *ctx, int kind)
     return 0;
 }
 
+int v2d_ctx_set_step_pairing(v2d_ctx *ctx, int paired)
+{
+    Ctx *c = ctx->c.get();
+    V2D_HIP(hipStreamSynchronize(c->s));
+    c->md.paired = c->mw.paired = paired != 0;
+    return 0;
+}
+
 int v2d_ctx_get_linear_solver(v2d_ctx *ctx, int *kind)
 {
     *kind = ctx->c->solver;

@@ -29,6 +29,20 @@ namespace v2d {
 
 constexpr int NB = 64;   // Gauss-Jordan block width == update tile edge
 
+// Global-address-space views of generic pointers: the job tables hand the
+// kernels plain pointers, which compile to flat memory instructions, and a
+// flat load is waited for together with every LDS access (lgkmcnt); global
+// loads are counted in order, so later loads stay in flight.
+template <typename X> using gptr = __attribute__((address_space(1))) X *;
+template <typename X> __device__ __forceinline__ gptr<X> glb(X *p) { return (gptr<X>)p; }
+// Native 16-byte vectors (HIP's float4 is a class type, which cannot be
+// copied through an address-space-qualified pointer).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T> struct NVec;
+template <> struct NVec<float> { using type = f32x4; };
+template <> struct NVec<double> { using type = f64x2; };
+
 // ----------------------------------------------- persistent solve sweeps ---
 // Level shapes in closed form (levels 0 and nlev-1 hold two j rows, the others
 // one); the pattern build checks them against its own tables.
@@ -322,23 +336,30 @@ struct Batch {
     __device__ __forceinline__ const Job<T> &job(int z) const { return j[z]; }
 };
 
-// Many jobs from a device table (the fronts of one elimination-tree level);
-// par = 1 swaps the read and write panel buffers of every job (the dataflow
-// steps alternate them).
+// Many jobs from a device table (the fronts of one elimination-tree level).
+// The panels rotate over three buffers per job: step s reads panel s from
+// buffer s mod 3 (rot), writes panel s + 1 into the next buffer and, in a
+// heavy step, also reads panel s - 1 from the previous one (returned as
+// colk_x / rowk_x).
 template <typename T>
 struct TableBatch {
     const Job<T> *jt;
     int nj;
     int kind;
-    int par;
-    int tpj;   // grid tiles per job (the largest job's tile count)
+    int rot;       // buffer of the step's read panels
+    int tpj;       // grid tiles per job (the largest job's tile count)
+    int compact;   // 1: the grid covers the next pivot's row and column blocks only
     __device__ __forceinline__ Job<T> job(int z) const
     {
         Job<T> J = jt[z];
-        if (par) {
-            T *t = J.colk; J.colk = J.colk_w; J.colk_w = t;
-            t = J.rowk; J.rowk = J.rowk_w; J.rowk_w = t;
-        }
+        T *const c0 = J.colk, *const c1 = J.colk_w, *const c2 = J.colk_x;
+        T *const r0 = J.rowk, *const r1 = J.rowk_w, *const r2 = J.rowk_x;
+        J.colk = rot == 0 ? c0 : rot == 1 ? c1 : c2;
+        J.rowk = rot == 0 ? r0 : rot == 1 ? r1 : r2;
+        J.colk_w = rot == 0 ? c1 : rot == 1 ? c2 : c0;
+        J.rowk_w = rot == 0 ? r1 : rot == 1 ? r2 : r0;
+        J.colk_x = rot == 0 ? c2 : rot == 1 ? c0 : c1;
+        J.rowk_x = rot == 0 ? r2 : rot == 1 ? r0 : r1;
         return J;
     }
     // block -> (job, tile): every job's first tile (its pivot tile) first,
@@ -987,17 +1008,19 @@ template <> struct Mfma<double> {
 // The pivot-row work thus overlaps the bulk update of the step.
 constexpr int kSpinLimit = 1 << 21;
 
-template <typename T, typename BT = Batch<T>>
+template <typename T, typename BT = Batch<T>, int MODE = kStepFull>
 __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
                                                         int *__restrict__ flags, int *__restrict__ err)
 {
     using M = Mfma<T>;
     using C = typename M::C;
     using V = typename Vec8<T>::type;
+    using NV = typename NVec<T>::type;
     constexpr int VW = Vec8<T>::w;
     constexpr int LT = NB + 4;    // padded LDS row stride
     __shared__ T sT[NB * LT];     // next pivot block, or a tile stored transposed
     constexpr bool kTable = !std::is_same<BT, Batch<T>>::value;
+    static_assert(kTable || MODE == kStepFull, "the level batches take full steps only");
     int z, lin;
     if constexpr (kTable) {
         B.tile(blockIdx.x, z, lin);
@@ -1011,21 +1034,42 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
     if (k0 >= np) return;
     const int kb = min(NB, np - k0);
     const int nt = (n + NB - 1) / NB;
-    if (lin >= nt * nt) return;
     const int k1 = k0 + NB;
-    if (k1 < np) {
-        const int piv = (k1 / NB) * nt + k1 / NB;
-        if (lin == 0) lin = piv;
-        else if (lin <= piv) lin -= 1;
+    int i0, c0;
+    bool compact = false;
+    if constexpr (kTable) compact = B.compact && k1 < np;
+    if (compact) {
+        // only the next pivot's column block (pivot tile first) and row block
+        const int kt = k1 / NB;
+        if (lin >= 2 * nt - 1) return;
+        if (lin < nt) {
+            i0 = (lin == 0 ? kt : (lin - 1 < kt ? lin - 1 : lin)) * NB;
+            c0 = k1;
+        } else {
+            const int q = lin - nt;
+            i0 = k1;
+            c0 = (q < kt ? q : q + 1) * NB;
+        }
+    } else {
+        if (lin >= nt * nt) return;
+        if (k1 < np) {
+            const int piv = (k1 / NB) * nt + k1 / NB;
+            if (lin == 0) lin = piv;
+            else if (lin <= piv) lin -= 1;
+        }
+        i0 = (lin / nt) * NB;
+        c0 = (lin % nt) * NB;
+        // a light or first step leaves the tiles outside the next pivot's row
+        // and column blocks to the following heavy step
+        if ((MODE == kStepLight || MODE == kStepBegin) && k1 < np && i0 != k1 && c0 != k1) return;
     }
-    const int i0 = (lin / nt) * NB, c0 = (lin % nt) * NB;
     if (i0 == k0) {
         // the current pivot rows are not updated; their part of column tile
         // k1 is still the next column panel's rows k0 .. k0 + 63
         if (k1 < np && c0 == k1) {
             for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
                 const int r = idx / NB, t = idx % NB;
-                J.colk_w[(int64_t)(k0 + r) * NB + t] = (k1 + t < np) ? A[(int64_t)(k0 + r) * ld + k1 + t] : T(0);
+                glb(J.colk_w)[(int64_t)(k0 + r) * NB + t] = (k1 + t < np) ? glb(A)[(int64_t)(k0 + r) * ld + k1 + t] : T(0);
             }
         }
         return;
@@ -1040,70 +1084,69 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
     const int rw = i0 + (w >> 1) * 32, cw = c0 + (w & 1) * 32;
     const bool colK = c0 == k0;            // the pivot columns: old values are zero
     const bool cfull = c0 + NB <= n;       // no column of the tile past n
+    // A heavy step also applies the previous step's panels (colk_x, rowk_x),
+    // whose update the light step deferred: every tile outside the previous
+    // pivot rows (kp) and this step's pivot columns, with old values zero in
+    // the previous pivot columns.
+    bool oldZero = colK, useP = false;
+    if constexpr (MODE == kStepHeavy) {
+        const int kp = k0 - NB;
+        const bool rowP = i0 == kp, colP = c0 == kp;
+        useP = !rowP && !colK;
+        oldZero = colK || (colP && !rowP);
+    }
     // ---------------------------------------------------- tile update ----
     C acc[2][2], old[2][2];
-    if constexpr (kTable && sizeof(T) == 4) {
-        // Job tables (many fronts per launch, the bulk update dominates): every
-        // load of the tile goes out before the first matrix-core product --
-        // the panels of kPf 16-wide k-slices at a time (all four in FP32), then
-        // the old tile from clamped rows and columns (zero selected outside
-        // the matrix), so a tile waits for one memory round trip instead of
-        // one per slice and per sub-tile.  FP32 only: in FP64 the registers
-        // this takes halve the occupancy and the launch is slower.
-        constexpr int kPf = 4;
-        const V *qa[2], *qb[2];
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-            qa[bi] = reinterpret_cast<const V *>(J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk);
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
-            qb[bj] = reinterpret_cast<const V *>(J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk);
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+            for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
+    if constexpr (kTable && sizeof(T) == 4) {
+        // Job tables (many fronts per launch, the bulk update dominates): every
+        // load of a panel's product goes out before its first matrix-core
+        // product -- the four 16-wide k-slices -- with the old tile (clamped
+        // rows and columns, zero selected outside the matrix) behind the
+        // first panel, so a tile waits for one memory round trip per panel.
+        // FP32 only: in FP64 the registers this takes halve the occupancy and
+        // the launch is slower.
+        const bool tail = kb < NB;
+        auto product = [&](const T *ck, const T *rk, bool msk, bool with_old) {
+            gptr<const NV> qa[2], qb[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+                qa[bi] = glb(reinterpret_cast<const NV *>(ck + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk));
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj)
+                qb[bj] = glb(reinterpret_cast<const NV *>(rk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk));
+            NV xa[4][2][4 / VW], xb[4][2][4 / VW];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
-        const bool tail = kb < NB;
-#pragma unroll
-        for (int h0 = 0; h0 < 4; h0 += kPf) {
-            V xa[kPf][2][4 / VW], xb[kPf][2][4 / VW];
-#pragma unroll
-            for (int hh = 0; hh < kPf; ++hh)
+            for (int hh = 0; hh < 4; ++hh)
 #pragma unroll
                 for (int b2 = 0; b2 < 2; ++b2)
 #pragma unroll
                     for (int u = 0; u < 4 / VW; ++u) {
-                        xa[hh][b2][u] = qa[b2][(16 * (h0 + hh)) / VW + u];
-                        xb[hh][b2][u] = qb[b2][(16 * (h0 + hh)) / VW + u];
+                        xa[hh][b2][u] = qa[b2][(16 * hh) / VW + u];
+                        xb[hh][b2][u] = qb[b2][(16 * hh) / VW + u];
                     }
-            if (h0 == 0) {
+            if (with_old) {
 #pragma unroll
                 for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
                     for (int bj = 0; bj < 2; ++bj) {
-                        if (M::kSwap) {
-                            const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
-                            const float4 x = *reinterpret_cast<const float4 *>(
-                                A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4));
-                            const bool in = !colK && r < n && c < n;
-                            old[bi][bj][0] = in ? x.x : T(0);
-                            old[bi][bj][1] = in ? x.y : T(0);
-                            old[bi][bj][2] = in ? x.z : T(0);
-                            old[bi][bj][3] = in ? x.w : T(0);
-                        } else {
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) {
-                                const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
-                                const T x = A[(int64_t)min(r, n - 1) * ld + min(c, n - 1)];
-                                old[bi][bj][v] = (!colK && r < n && c < n) ? x : T(0);
-                            }
-                        }
+                        const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                        const f32x4 x = *glb(reinterpret_cast<const f32x4 *>(
+                            A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
+                        const bool in = !oldZero && r < n && c < n;
+                        old[bi][bj][0] = in ? x[0] : T(0);
+                        old[bi][bj][1] = in ? x[1] : T(0);
+                        old[bi][bj][2] = in ? x[2] : T(0);
+                        old[bi][bj][3] = in ? x[3] : T(0);
                     }
             }
 #pragma unroll
-            for (int hh = 0; hh < kPf; ++hh) {
-                const int h = h0 + hh;
+            for (int h = 0; h < 4; ++h) {
                 T av[2][4], bv[2][4];
 #pragma unroll
                 for (int b2 = 0; b2 < 2; ++b2)
@@ -1111,10 +1154,10 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
                     for (int u = 0; u < 4 / VW; ++u)
 #pragma unroll
                         for (int e = 0; e < VW; ++e) {
-                            av[b2][u * VW + e] = -reinterpret_cast<const T *>(&xa[hh][b2][u])[e];
-                            bv[b2][u * VW + e] = reinterpret_cast<const T *>(&xb[hh][b2][u])[e];
+                            av[b2][u * VW + e] = -reinterpret_cast<const T *>(&xa[h][b2][u])[e];
+                            bv[b2][u * VW + e] = reinterpret_cast<const T *>(&xb[h][b2][u])[e];
                         }
-                if (tail) {
+                if (msk) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
                         if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
@@ -1126,6 +1169,26 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
 #pragma unroll
                         for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
             }
+        };
+        if constexpr (MODE == kStepBegin) {
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                    const f32x4 x =
+                        *glb(reinterpret_cast<const f32x4 *>(A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
+                    const bool in = r < n && c < n;
+                    old[bi][bj][0] = in ? x[0] : T(0);
+                    old[bi][bj][1] = in ? x[1] : T(0);
+                    old[bi][bj][2] = in ? x[2] : T(0);
+                    old[bi][bj][3] = in ? x[3] : T(0);
+                }
+        } else if (useP) {
+            product(J.colk_x, J.rowk_x, false, true);
+            product(J.colk, J.rowk, tail, false);
+        } else {
+            product(J.colk, J.rowk, tail, true);
         }
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
@@ -1134,20 +1197,15 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
     } else {
-        const T *pa[2], *pb[2];
-    #pragma unroll
-        for (int bi = 0; bi < 2; ++bi) pa[bi] = J.colk + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk;
-    #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) pb[bj] = J.rowk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk;
     #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
     #pragma unroll
             for (int bj = 0; bj < 2; ++bj) {
-                if (M::kSwap && cfull && !colK) {
+                if (M::kSwap && cfull && !oldZero) {
                     const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                     if (r < n) {
-                        const float4 x = *reinterpret_cast<const float4 *>(A + (int64_t)r * ld + c);
-                        old[bi][bj][0] = x.x; old[bi][bj][1] = x.y; old[bi][bj][2] = x.z; old[bi][bj][3] = x.w;
+                        const f32x4 x = *glb(reinterpret_cast<const f32x4 *>(A + (int64_t)r * ld + c));
+                        old[bi][bj][0] = x[0]; old[bi][bj][1] = x[1]; old[bi][bj][2] = x[2]; old[bi][bj][3] = x[3];
                     } else {
     #pragma unroll
                         for (int v = 0; v < 4; ++v) old[bi][bj][v] = T(0);
@@ -1156,47 +1214,51 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
     #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
-                        old[bi][bj][v] = (!colK && r < n && c < n) ? A[(int64_t)r * ld + c] : T(0);
+                        old[bi][bj][v] = (!oldZero && r < n && c < n) ? glb(A)[(int64_t)r * ld + c] : T(0);
                     }
                 }
             }
+        auto product = [&](const T *ck, const T *rk, bool msk) {
+            gptr<const T> pa[2], pb[2];
     #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+            for (int bi = 0; bi < 2; ++bi) pa[bi] = glb(ck + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk);
     #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
+            for (int bj = 0; bj < 2; ++bj) pb[bj] = glb(rk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk);
     #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
-        const bool tail = kb < NB;
-    #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            T av[2][4], bv[2][4];
-    #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-    #pragma unroll
-                for (int u = 0; u < 4 / VW; ++u) {
-                    const V x = reinterpret_cast<const V *>(pa[bi] + 16 * h)[u];
-    #pragma unroll
-                    for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = -reinterpret_cast<const T *>(&x)[e];
-                }
-    #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-    #pragma unroll
-                for (int u = 0; u < 4 / VW; ++u) {
-                    const V x = reinterpret_cast<const V *>(pb[bj] + 16 * h)[u];
-    #pragma unroll
-                    for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = reinterpret_cast<const T *>(&x)[e];
-                }
-            if (tail) {
-    #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
-            }
-    #pragma unroll
-            for (int e = 0; e < 4; ++e)
+            for (int h = 0; h < 4; ++h) {
+                T av[2][4], bv[2][4];
     #pragma unroll
                 for (int bi = 0; bi < 2; ++bi)
     #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+                    for (int u = 0; u < 4 / VW; ++u) {
+                        const NV x = reinterpret_cast<gptr<const NV>>(pa[bi] + 16 * h)[u];
+    #pragma unroll
+                        for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = -reinterpret_cast<const T *>(&x)[e];
+                    }
+    #pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+    #pragma unroll
+                    for (int u = 0; u < 4 / VW; ++u) {
+                        const NV x = reinterpret_cast<gptr<const NV>>(pb[bj] + 16 * h)[u];
+    #pragma unroll
+                        for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = reinterpret_cast<const T *>(&x)[e];
+                    }
+                if (msk) {
+    #pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                }
+    #pragma unroll
+                for (int e = 0; e < 4; ++e)
+    #pragma unroll
+                    for (int bi = 0; bi < 2; ++bi)
+    #pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+            }
+        };
+        if constexpr (MODE != kStepBegin) {
+            if (useP) product(J.colk_x, J.rowk_x, false);
+            product(J.colk, J.rowk, kb < NB);
         }
     #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
@@ -1214,18 +1276,18 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
                 if (M::kSwap && cfull) {
                     const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                     if (r < n) {
-                        float4 x;
-                        x.x = acc[bi][bj][0]; x.y = acc[bi][bj][1]; x.z = acc[bi][bj][2]; x.w = acc[bi][bj][3];
-                        *reinterpret_cast<float4 *>(A + (int64_t)r * ld + c) = x;
-                        if (is_pcol) *reinterpret_cast<float4 *>(J.colk_w + (int64_t)r * NB + (c - k1)) = x;
+                        f32x4 x;
+                        x[0] = acc[bi][bj][0]; x[1] = acc[bi][bj][1]; x[2] = acc[bi][bj][2]; x[3] = acc[bi][bj][3];
+                        if (MODE != kStepBegin) *glb(reinterpret_cast<f32x4 *>(A + (int64_t)r * ld + c)) = x;
+                        if (is_pcol) *glb(reinterpret_cast<f32x4 *>(J.colk_w + (int64_t)r * NB + (c - k1))) = x;
                     }
                 } else {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int r = rw + bi * 16 + M::row(li, lk, v), c = cw + bj * 16 + M::col(li, lk, v);
                         if (r < n) {
-                            if (c < n) A[(int64_t)r * ld + c] = acc[bi][bj][v];
-                            if (is_pcol) J.colk_w[(int64_t)r * NB + (c - k1)] = acc[bi][bj][v];
+                            if (MODE != kStepBegin && c < n) glb(A)[(int64_t)r * ld + c] = acc[bi][bj][v];
+                            if (is_pcol) glb(J.colk_w)[(int64_t)r * NB + (c - k1)] = acc[bi][bj][v];
                         }
                     }
                 }
@@ -1261,8 +1323,8 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int r = rg * 16 + q;
-            if (r < kb2 && c < kb2) A[(int64_t)(k1 + r) * ld + k1 + c] = g[q];
-            rk[q] = g[q];
+            if (r < kb2 && c < kb2) glb(A)[(int64_t)(k1 + r) * ld + k1 + c] = g[q];
+            glb(rk)[q] = g[q];
         }
         return;
     }
@@ -1334,8 +1396,8 @@ __global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = rw2 + bi * 16 + M::drow(lk, v);
-                if (r < kb2 && c0 + c < n) A[(int64_t)(k1 + r) * ld + c0 + c] = d[bi][bj][v];
-                J.rowk_w[(int64_t)(c0 + c) * NB + r] = d[bi][bj][v];
+                if (r < kb2 && c0 + c < n) glb(A)[(int64_t)(k1 + r) * ld + c0 + c] = d[bi][bj][v];
+                glb(J.rowk_w)[(int64_t)(c0 + c) * NB + r] = d[bi][bj][v];
             }
         }
 }
@@ -1432,55 +1494,71 @@ static int gj_invert_b(const Batch<T> &B, DirectWork<T> &W, LaunchProbe *probe, 
 
 // Gauss-Jordan sweeps of a device table of jobs (MFMA dataflow steps): every
 // job's pivots [0, np) are eliminated from its n x n matrix in place, np a
-// multiple of 64 at most npmax, n at most nmax.  flags holds one word per job.
-// gj_table_begin inverts every first pivot block and forms the step-0 panels;
-// gj_table_step runs the step at pivot k0 (k0 = 0, 64, .. < npmax in order).
+// multiple of 64, n at most nmax.  flags holds one word per job.
+// gj_table_begin inverts every first pivot block and forms the step-0 panels
+// and pivot rows (a first step, compact grid); gj_table_step runs the step at
+// pivot k0 (k0 = 0, 64, .. in order) in the given mode.
 template <typename T>
-int gj_table_begin(const Job<T> *jobs, int nj, int nmax, hipStream_t s)
+int gj_table_begin(const Job<T> *jobs, int nj, int nmax, int *flags, int *err, int &epoch, hipStream_t s)
 {
     if (nj > 65535) {
         set_error("Gauss-Jordan table: more than 65535 jobs in one launch");
         return -1;
     }
-    const unsigned npr = (unsigned)((nmax + RP - 1) / RP);
-    TableBatch<T> B{jobs, nj, kUpdateMfma64, 0, 0};
-    hipLaunchKernelGGL((k_gj_diag_b<T, TableBatch<T>>), dim3(1, 1, (unsigned)nj), dim3(256), 0, s, B, 0);
-    hipLaunchKernelGGL((k_gj_rowpanel_b<T, TableBatch<T>>), dim3(npr, 2, (unsigned)nj), dim3(256), 0, s, B, 0);
-    V2D_LAUNCH();
-    return 0;
-}
-
-template <typename T>
-int gj_table_step(const Job<T> *jobs, int nj, int nmax, int k0, int *flags, int *err, int &epoch, hipStream_t s)
-{
     const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
-    TableBatch<T> B{jobs, nj, kUpdateMfma64, (k0 / NB) & 1, (int)(nt * nt)};
-    if (nt * nt < 2) {
+    if (nt < 2) {
         set_error("Gauss-Jordan table: a job of one tile has no step");
         return -1;
     }
-    hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>>), dim3(nt * nt * (unsigned)nj), dim3(256), 0, s, B, k0,
-                       ++epoch, flags, err);
+    TableBatch<T> B{jobs, nj, kUpdateMfma64, 2, (int)(2 * nt - 1), 1};
+    hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>, kStepBegin>), dim3((2 * nt - 1) * (unsigned)nj), dim3(256),
+                       0, s, B, -NB, ++epoch, flags, err);
     V2D_LAUNCH();
     return 0;
 }
 
 template <typename T>
-int gj_sweep_table(const Job<T> *jobs, int nj, int npmax, int nmax, int *flags, int *err, int &epoch,
-                   hipStream_t s)
+int gj_table_step(const Job<T> *jobs, int nj, int nmax, int k0, int mode, bool compact, int *flags, int *err,
+                  int &epoch, hipStream_t s)
 {
-    if (nj <= 0 || npmax <= 0) return 0;
-    if (gj_table_begin(jobs, nj, nmax, s)) return -1;
-    for (int k0 = 0; k0 < npmax; k0 += NB)
-        if (gj_table_step(jobs, nj, nmax, k0, flags, err, epoch, s)) return -1;
+    const unsigned nt = (unsigned)((nmax + NB - 1) / NB);
+    if (nt < 2) {
+        set_error("Gauss-Jordan table: a job of one tile has no step");
+        return -1;
+    }
+    compact = compact && mode == kStepLight;
+    const unsigned tpj = compact ? 2 * nt - 1 : nt * nt;
+    TableBatch<T> B{jobs, nj, kUpdateMfma64, (k0 / NB) % 3, (int)tpj, compact ? 1 : 0};
+    const dim3 g(tpj * (unsigned)nj);
+    switch (mode) {
+    case kStepFull:
+        hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>, kStepFull>), g, dim3(256), 0, s, B, k0, ++epoch, flags,
+                           err);
+        break;
+    case kStepLight:
+        hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>, kStepLight>), g, dim3(256), 0, s, B, k0, ++epoch,
+                           flags, err);
+        break;
+    case kStepHeavy:
+        if (k0 < NB) {
+            set_error("Gauss-Jordan table: a heavy step needs a previous step");
+            return -1;
+        }
+        hipLaunchKernelGGL((k_gj_step_mfma_b<T, TableBatch<T>, kStepHeavy>), g, dim3(256), 0, s, B, k0, ++epoch,
+                           flags, err);
+        break;
+    default:
+        set_error("Gauss-Jordan table: unknown step mode");
+        return -1;
+    }
+    V2D_LAUNCH();
     return 0;
 }
-template int gj_table_begin<float>(const Job<float> *, int, int, hipStream_t);
-template int gj_table_begin<double>(const Job<double> *, int, int, hipStream_t);
-template int gj_table_step<float>(const Job<float> *, int, int, int, int *, int *, int &, hipStream_t);
-template int gj_table_step<double>(const Job<double> *, int, int, int, int *, int *, int &, hipStream_t);
-template int gj_sweep_table<float>(const Job<float> *, int, int, int, int *, int *, int &, hipStream_t);
-template int gj_sweep_table<double>(const Job<double> *, int, int, int, int *, int *, int &, hipStream_t);
+
+template int gj_table_begin<float>(const Job<float> *, int, int, int *, int *, int &, hipStream_t);
+template int gj_table_begin<double>(const Job<double> *, int, int, int *, int *, int &, hipStream_t);
+template int gj_table_step<float>(const Job<float> *, int, int, int, int, bool, int *, int *, int &, hipStream_t);
+template int gj_table_step<double>(const Job<double> *, int, int, int, int, bool, int *, int *, int &, hipStream_t);
 
 // ------------------------------------------------------------ factor ------
 // Twisted block factorisation: the bottom chain eliminates levels 0..M-1 from
@@ -1504,6 +1582,7 @@ static Job<T> make_job(const DirectPattern &P, DirectWork<T> &W, int l, int slot
     J.rowk = df ? W.rowk + (int64_t)slot * P.ldmax * NB : nullptr;
     J.colk_w = df ? W.colk2 + (int64_t)slot * P.nmax * NB : nullptr;
     J.rowk_w = df ? W.rowk2 + (int64_t)slot * P.ldmax * NB : nullptr;
+    J.colk_x = J.rowk_x = nullptr;
     return J;
 }
 

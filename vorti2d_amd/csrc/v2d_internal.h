@@ -247,6 +247,13 @@ int direct_copy_pattern(DirectPattern &D, const DirectPattern &S, hipStream_t s)
 // Gauss-Jordan update kernels: VALU with 64- or 128-wide output tiles, or
 // MFMA 16x16x4 (FP32 / FP64 matrix cores) on 64-wide tiles.
 enum UpdateKind { kUpdateValu64 = 0, kUpdateValu128 = 1, kUpdateMfma64 = 2 };
+// Launch modes of the dataflow Gauss-Jordan step over job tables.  Full: every
+// tile takes the step's rank-64 update.  Light: only the tiles of the next
+// pivot's row and column blocks (they form the next panels); the others wait
+// for the heavy step that follows, which applies both steps' panels in one
+// rank-128 update.  Begin: no update; the first pivot block is inverted and
+// the first panels and pivot rows formed.
+enum StepMode { kStepFull = 0, kStepLight = 1, kStepHeavy = 2, kStepBegin = 3 };
 
 // Optional timing of the Gauss-Jordan update kernel on the factorisation
 // stream: an event pair brackets each level's run of back-to-back update
@@ -303,17 +310,19 @@ struct Job {
     T *colk, *dinv, *tmat; // per-job scratch (tmat: block LU only)
     T *rowk;               // transposed pivot-row panel (MFMA update only, else null)
     T *colk_w, *rowk_w;    // panels of the next step (MFMA dataflow step only)
+    T *colk_x, *rowk_x;    // job tables: third panel buffers (panels rotate over three)
 };
 
-// Sweeps of a device table of jobs with the MFMA dataflow steps (colk, colk_w:
-// n x 64; rowk, rowk_w: 64-rounded n x 64; dinv 64 x 64 per job).
+// Sweeps of a device table of jobs with the MFMA dataflow steps (colk, colk_w,
+// colk_x: n x 64; rowk, rowk_w, rowk_x: 64-rounded n x 64; dinv 64 x 64 per
+// job).
 template <typename T>
-int gj_sweep_table(const Job<T> *jobs, int nj, int npmax, int nmax, int *flags, int *err, int &epoch,
-                   hipStream_t s);
+int gj_table_begin(const Job<T> *jobs, int nj, int nmax, int *flags, int *err, int &epoch, hipStream_t s);
+// mode: a StepMode; compact (light steps only): no job of the table ends at
+// this step, so the grid covers only the next pivot's row and column blocks.
 template <typename T>
-int gj_table_begin(const Job<T> *jobs, int nj, int nmax, hipStream_t s);
-template <typename T>
-int gj_table_step(const Job<T> *jobs, int nj, int nmax, int k0, int *flags, int *err, int &epoch, hipStream_t s);
+int gj_table_step(const Job<T> *jobs, int nj, int nmax, int k0, int mode, bool compact, int *flags, int *err,
+                  int &epoch, hipStream_t s);
 
 template <typename T>
 int direct_factor(const DirectPattern &P, DirectWork<T> &W, const T *val, hipStream_t s);
@@ -433,6 +442,7 @@ struct MfWork {
     int *flags = nullptr, *err = nullptr;
     int epoch = 0;
     bool factored = false;
+    bool paired = true;       // Gauss-Jordan steps in light / heavy pairs (else every step full)
 };
 template <typename T>
 int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s);

@@ -486,7 +486,7 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
                 if (F.nch != 2) L.nch2 = 0;
                 L.nfmax_big = std::max(L.nfmax_big, F.nf);
                 L.npmax_big = std::max(L.npmax_big, F.nsp);
-                L.pool += 2 * (int64_t)F.nf * kNB + 2 * (int64_t)round_up(F.nf, kNB) * kNB + kNB * kNB;
+                L.pool += 3 * (int64_t)F.nf * kNB + 3 * (int64_t)round_up(F.nf, kNB) * kNB + kNB * kNB;
             } else {
                 L.nfmax_small = std::max(L.nfmax_small, F.nf);
             }
@@ -592,6 +592,8 @@ int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s)
             J.colk_w = W.pool + off; off += (int64_t)F.nf * kNB;
             J.rowk = W.pool + off; off += r64 * kNB;
             J.rowk_w = W.pool + off; off += r64 * kNB;
+            J.colk_x = W.pool + off; off += (int64_t)F.nf * kNB;
+            J.rowk_x = W.pool + off; off += r64 * kNB;
             J.dinv = W.pool + off; off += kNB * kNB;
             J.tmat = nullptr;
             jobs.push_back(J);
@@ -615,7 +617,9 @@ void mf_free(MfWork<T> &W, hipStream_t s)
 {
     void *ptrs[] = {W.F, W.Vf, W.Zf, W.pool, W.jobs, W.flags};
     for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
+    const bool paired = W.paired;
     W = MfWork<T>{};
+    W.paired = paired;
 }
 
 // ------------------------------------------------------------ kernels ----
@@ -1157,21 +1161,38 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
         V2D_LAUNCH();
         const int nj = L.f1 - L.fbig;
         if (nj <= 0 || L.npmax_big <= 0) continue;
+        // Steps in pairs: a light step (the next panels only) and a heavy step
+        // (every other tile takes both steps' panels), a last unpaired step full.
+        const int nsteps = L.npmax_big / kNB;
+        std::vector<char> ends(nsteps, 0);   // a job's last step (no compact light grid there)
+        for (int f = L.fbig; f < L.f1; ++f) ends[P.fr[f].nsp / kNB - 1] = 1;
         MF_BEGIN();
-        if (gj_table_begin<T>(W.jobs + L.job0, nj, L.nfmax_big, s)) return -1;
+        if (gj_table_begin<T>(W.jobs + L.job0, nj, L.nfmax_big, W.flags, W.err, W.epoch, s)) return -1;
         MF_END(kGjBegin, 0.0, 0.0);
-        for (int k0 = 0; k0 < L.npmax_big; k0 += kNB) {
+        for (int st = 0; st < nsteps; ++st) {
+            const int k0 = st * kNB;
+            const int mode = !W.paired ? kStepFull
+                             : (st & 1) ? kStepHeavy
+                                        : (st + 1 < nsteps ? kStepLight : kStepFull);
             double fl = 0, by = 0;
             if (probe)
                 for (int f = L.fbig; f < L.f1; ++f) {
                     const MfFront &F = P.fr[f];
                     if (k0 >= F.nsp) continue;
                     const double n = F.nf, kb = kNB;
-                    fl += 2.0 * (n - kb) * n * kb;
-                    by += ts * (2.0 * (n - kb) * n + (n - kb) * kb + kb * n);
+                    const bool last = k0 + kNB >= F.nsp;
+                    // elements of a step's update (rows outside the pivots, every
+                    // column) and of a light step's share of it
+                    const double e_step = (n - kb) * n, e_light = kb * n + kb * (n - 2 * kb);
+                    double e = e_step;
+                    if (mode == kStepLight && !last) e = e_light;
+                    if (mode == kStepHeavy) e = e_step + (e_step - e_light);
+                    fl += 2.0 * kb * e;
+                    by += ts * (2.0 * e + 2.0 * kb * n * (mode == kStepHeavy ? 2 : 1));
                 }
             MF_BEGIN();
-            if (gj_table_step<T>(W.jobs + L.job0, nj, L.nfmax_big, k0, W.flags, W.err, W.epoch, s)) return -1;
+            if (gj_table_step<T>(W.jobs + L.job0, nj, L.nfmax_big, k0, mode, !ends[st], W.flags, W.err, W.epoch, s))
+                return -1;
             MF_END(kGjStep, fl, by);
         }
     }

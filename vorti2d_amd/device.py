@@ -156,6 +156,13 @@ class DeviceContext:
         factorisation."""
         check(lib.v2d_ctx_set_linear_solver(self._h, LINEAR_SOLVERS.get(kind, kind)))
 
+    def set_step_pairing(self, paired: bool):
+        """Multifrontal factorisation: Gauss-Jordan steps in light / heavy pairs
+        (True, the default; the bulk of every front is read and written once
+        per two steps) or every step over the whole front (False).  Same
+        factorisation up to rounding; applies from the next factorisation."""
+        check(lib.v2d_ctx_set_step_pairing(self._h, 1 if paired else 0))
+
     def linear_solver(self) -> str:
         k = C.c_int()
         check(lib.v2d_ctx_get_linear_solver(self._h, C.byref(k)))
