@@ -304,7 +304,7 @@ def mf_kernels(run, prec, nsolves):
         per_solve = g["ms"] * (app if k in ("gather", "fwd", "bwd") else fac)
         e = {"kernel": g["kernel"], "launches": g["launches"], "ms": g["ms"],
              "avg_launch_us": g["ms"] / g["launches"] * 1e3, "ms_per_solve": per_solve}
-        if k == "gj_step":
+        if k in ("gj_full", "gj_light", "gj_heavy"):
             tf = g["flops"] / (g["ms"] * 1e-3) / 1e12
             e.update(bound="mfma", achieved=tf, peak=peak_tf, unit="TFLOP/s", frac=tf / peak_tf,
                      flops=g["flops"], bytes=g["bytes"])
@@ -387,7 +387,7 @@ def main():
         kernels = mf_kernels(mx, "mixed", nsol)
         dom = max((k for k in kernels if "bound" in kernels[k]), key=lambda k: kernels[k]["ms_per_solve"])
         d = kernels[dom]
-        kname = d["kernel"] + ("<float, TableBatch<float>>" if dom == "gj_step" else "<float>")
+        kname = d["kernel"]
         ent = traffic.get(kname) or {}
         alg = d["bytes"] / d["launches"]
         tr = ent["measured_over_algorithmic"] * alg if "measured_over_algorithmic" in ent else None

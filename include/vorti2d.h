@@ -213,13 +213,13 @@ int v2d_ctx_mf_info(v2d_ctx *ctx, int64_t *out, double *flops);
 
 /* Factorise the last assembled Jacobian and solve one system with the
  * multifrontal LU, timing every launch with HIP events on the context's
- * stream.  out[4 g .. 4 g + 3] for the kernel groups g = 0..7 (front assembly,
- * entry scatter, small-front sweep, first Gauss-Jordan pivot blocks and
- * panels, Gauss-Jordan dataflow steps, forward gather, forward front products,
- * backward front products): milliseconds, launches, algorithmic flops,
- * algorithmic bytes; out[32], out[33]: the factorisation's and the solve's
- * event spans (ms).  The factorisation is kept.  No counterpart in the
- * reference. */
+ * stream.  out[4 g .. 4 g + 3] for the kernel groups g = 0..9 (front assembly,
+ * entry scatter, small-front sweep, first Gauss-Jordan step (pivot inverse,
+ * pivot rows and panels), full Gauss-Jordan steps, light steps, heavy steps,
+ * forward gather, forward front products, backward front products):
+ * milliseconds, launches, algorithmic flops, algorithmic bytes; out[40],
+ * out[41]: the factorisation's and the solve's event spans (ms).  The
+ * factorisation is kept.  No counterpart in the reference. */
 int v2d_ctx_profile_mf(v2d_ctx *ctx, double *out);
 
 /* Copy of the factored fronts (front storage elements, FP64; each front's

@@ -6,8 +6,8 @@
     python profile_kernels.py copy     [--mib 256] [--launches 5]
 
 newton:   the headline solve: pure Newton from the impulsive start to res <= 1e-10
-          and ||R||_2 / sqrt(2N) < 1e-10 (mixed: the benchmark's GMRES and
-          forcing settings), after one warm-up solve.
+          and ||R||_2 / sqrt(2N) < 1e-10 (mixed: the benchmark's GMRES budget,
+          tolerance and forcing), after one warm-up solve.
 factor:   one factorisation of the steady cylinder Jacobian (initial state,
           Re 40) after one warm-up factorisation (multifrontal: and one solve,
           both event-timed per kernel group, with their algorithmic counts).
@@ -31,6 +31,8 @@ def main(argv=None):
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="newton, mixed: launch the preconditioner-apply kernels one by one "
                          "(kernel tracing does not follow captured-graph replays)")
+    ap.add_argument("--gmres-max", type=int, default=10,
+                    help="newton, mixed: GMRES budget on a stored factorisation (the benchmark's)")
     ap.add_argument("--size", type=int, default=1025)
     ap.add_argument("--jmax", type=int, default=0,
                     help="radial size (default: --size); the factor launches of a "
@@ -59,7 +61,7 @@ def main(argv=None):
         import numpy as np
         tol = 1e-10
         if a.precision == "mixed":
-            ctx.set_solver(refactor_every=0, gmres_rtol=1e-6, gmres_max_iter=30)
+            ctx.set_solver(refactor_every=0, gmres_rtol=1e-6, gmres_max_iter=a.gmres_max)
             ctx.set_forcing(0.25, tol * np.sqrt(2 * ctx.N))
             ctx.set_graph(a.graph)
         for k in range(2):

@@ -417,7 +417,7 @@ struct MfPattern {
 // Per-kernel timing of a factorisation and a solve (HIP events around every
 // launch on the stream) with the algorithmic counts of what was timed.
 struct MfProbe {
-    enum { kAssemble, kScatter, kSmall, kGjBegin, kGjStep, kGather, kFwd, kBwd, kGroups };
+    enum { kAssemble, kScatter, kSmall, kGjBegin, kGjFull, kGjLight, kGjHeavy, kGather, kFwd, kBwd, kGroups };
     double ms[kGroups] = {}, flops[kGroups] = {}, bytes[kGroups] = {};
     int64_t launches[kGroups] = {};
     std::vector<hipEvent_t> ev;     // pairs, recorded

@@ -1193,7 +1193,11 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
             MF_BEGIN();
             if (gj_table_step<T>(W.jobs + L.job0, nj, L.nfmax_big, k0, mode, !ends[st], W.flags, W.err, W.epoch, s))
                 return -1;
-            MF_END(kGjStep, fl, by);
+            if (probe && probe->end(s, mode == kStepHeavy ? MfProbe::kGjHeavy
+                                       : mode == kStepLight ? MfProbe::kGjLight
+                                                            : MfProbe::kGjFull,
+                                    fl, by))
+                return -1;
         }
     }
     int e = 0;

@@ -177,22 +177,31 @@ class DeviceContext:
                 "entries", "positions")
         return dict(zip(keys, (int(v) for v in out)), flops=fl.value)
 
-    MF_GROUPS = ("assemble", "scatter", "small", "gj_begin", "gj_step", "gather", "fwd", "bwd")
-    MF_KERNELS = {"assemble": "k_mf_assemble", "scatter": "k_mf_scatter", "small": "k_mf_small_fused",
-                  "gj_begin": "k_gj_diag_b + k_gj_rowpanel_b", "gj_step": "k_gj_step_mfma_b",
-                  "gather": "k_mf_fgather", "fwd": "k_mf_fgemv", "bwd": "k_mf_bwd"}
+    MF_GROUPS = ("assemble", "scatter", "small", "gj_begin", "gj_full", "gj_light", "gj_heavy", "gather", "fwd",
+                 "bwd")
+    # kernel of each group, as rocprofv3 names it ({T}: float or double);
+    # "small" and "assemble" are several instances of one template
+    MF_KERNELS = {"assemble": "k_mf_assemble<{T}, *>", "scatter": "k_mf_scatter<{T}>",
+                  "small": "k_mf_small_fused<{T}, *>",
+                  "gj_begin": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 3>",
+                  "gj_full": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 0>",
+                  "gj_light": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 1>",
+                  "gj_heavy": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 2>",
+                  "gather": "k_mf_fgather<{T}>", "fwd": "k_mf_fgemv<{T}>", "bwd": "k_mf_bwd<{T}>"}
 
     def profile_mf(self) -> dict:
         """Factorise the last assembled Jacobian and solve once with the
         multifrontal LU, every launch timed with HIP events on the context's
         stream: per kernel group ms, launches, algorithmic flops and bytes, and
         the factorisation / solve spans."""
-        out = np.zeros(34)
+        ng = len(self.MF_GROUPS)
+        out = np.zeros(4 * ng + 2)
         check(lib.v2d_ctx_profile_mf(self._h, ptr(out)))
+        t = "double" if self.precision == PRECISIONS["fp64"] else "float"
         g = {k: dict(ms=float(out[4 * i]), launches=int(out[4 * i + 1]), flops=float(out[4 * i + 2]),
-                     bytes=float(out[4 * i + 3]), kernel=self.MF_KERNELS[k])
+                     bytes=float(out[4 * i + 3]), kernel=self.MF_KERNELS[k].format(T=t))
              for i, k in enumerate(self.MF_GROUPS)}
-        return dict(groups=g, factor_ms=float(out[32]), solve_ms=float(out[33]))
+        return dict(groups=g, factor_ms=float(out[4 * ng]), solve_ms=float(out[4 * ng + 1]))
 
     def mf_fronts(self) -> np.ndarray:
         """The factored multifrontal front storage (FP64 copy), for inspection."""
