@@ -48,7 +48,8 @@ namespace {
 
 constexpr int kMfLeaf = 32;      // most nodes of a leaf supernode
 constexpr int kMfSmall = 144;    // largest front swept in registers by one workgroup (16 x 9)
-constexpr int kMfAlign = 4;      // row stride and pivot padding of small fronts
+constexpr int kMfAlign = 4;      // pivot padding of small fronts
+constexpr int kMfRow = 32;       // row stride of every front, in elements (128-byte FP32 rows)
 constexpr int kNB = 64;          // Gauss-Jordan block of the large fronts
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -312,7 +313,7 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
         F.big = is_big(s) ? 1 : 0;
         F.nsp = round_up(F.ns, F.big ? kNB : kMfAlign);
         F.nf = F.nsp + F.nb;
-        F.ld = round_up(F.nf, kMfAlign);
+        F.ld = round_up(F.nf, kMfRow);
         F.nch = (int)D.kids[s].size();
         F.co = (int)P.ch.size();
         for (int c : D.kids[s]) P.ch.push_back(fid[c]);
