@@ -144,7 +144,7 @@ static int ensure_mf_pattern(Ctx *c)
     auto p = std::make_unique<MfPattern>();
     if (mf_analyse(*p, c->imax, c->jmax, c->hflags.data())) return -1;
     if (mf_upload(*p, c->s)) return -1;
-    c->mfp = std::move(p);
+    c->mfp = std::shared_ptr<MfPattern>(std::move(p));
     return 0;
 }
 
@@ -617,8 +617,9 @@ int v2d_force_coeffs(int imax, int jmax, double re, double dksi, double deta,
 
 // ------------------------------------------------------------ context -----
 // A context on the same mesh, far-field setting and state as src: the mesh
-// and state are copied on the device and the block-LU pattern is copied, not
-// rebuilt on the host.
+// and state are copied on the device, the block-LU pattern is copied and the
+// multifrontal analysis (with its device tables) is shared, not rebuilt on the
+// host.
 static int ctx_bc_copy(Ctx *c, const Ctx *src)
 {
     c->ff_bc = src->ff_bc; c->ca = src->ca; c->sa = src->sa;
@@ -630,6 +631,7 @@ static int ctx_bc_copy(Ctx *c, const Ctx *src)
     V2D_HIP(hipMemcpyAsync(c->pre_ome, src->pre_ome, sizeof(int) * c->imax, hipMemcpyDeviceToDevice, c->s));
     c->drop_graph();
     c->drop_mf();
+    c->mfp = src->mfp;
     c->solver = src->solver;
     if (direct_copy_pattern(c->pat, src->pat, c->s)) return -1;
     c->factored = false;

@@ -60,14 +60,22 @@ struct Ctx {
     // solver is allocated at its first factorisation
     int solver = V2D_SOLVER_MULTIFRONTAL;
     bool blu_ready = false;
-    std::unique_ptr<MfPattern> mfp;   // multifrontal analysis, device tables uploaded
+    // multifrontal analysis, device tables uploaded; shared (read only) with
+    // the contexts created like this one on the same grid and far field
+    std::shared_ptr<MfPattern> mfp;
     MfWork<double> md;
     MfWork<float> mw;
     void drop_mf()
     {
         mf_free(md, s);
         mf_free(mw, s);
-        if (mfp) mf_free_pattern(*mfp, s);
+        if (mfp) {
+            if (mfp.use_count() == 1) {
+                mf_free_pattern(*mfp, s);   // stream-ordered after this context's work
+            } else {
+                (void)hipStreamSynchronize(s);   // other owners keep the tables: finish our reads
+            }
+        }
         mfp.reset();
     }
     // GMRES (mixed) work, FP32 vectors
