@@ -770,30 +770,65 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
         for (int j = 0; j < NI; ++j) a[i][j] = T(0);
     for (int r0 = 0; r0 < nf; r0 += cr) {
         const int r1 = min(nf, r0 + cr), ne = (r1 - r0) * nf;
-        // every element e is zeroed and summed by the same thread
-        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-            const int r = r0 + e / nf, c = e % nf;
-            sF[e] = (r >= ns && r < f.nsp && c == r) ? T(1) : T(0);
-        }
-        for (int k = 0; k < f.nch; ++k) {
-            const int *m = map + f.mo + (int64_t)k * nf;
-            const MfFront c = fr[ch[f.co + k]];
-            for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-                const int ia = m[r0 + e / nf];
-                if (ia < 0) continue;
-                const int ib = m[e % nf];
-                if (ib < 0) continue;
-                sF[e] += F[c.fo + (int64_t)(c.nsp + ia) * c.ld + c.nsp + ib];
+        // Every element e is set and summed by one thread: identity on the
+        // padding pivots, then the children's contributions in list order.
+        // Rounds of U elements per thread issue every map load, then every
+        // contribution load (clamped addresses, the absent ones dropped by a
+        // select: no load behind a branch), before the first sum.
+        constexpr int U = 4;
+        for (int e0 = 0; e0 < ne; e0 += 256 * U) {
+            int er[U], ec[U];
+            T acc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = min(e0 + u * 256 + (int)threadIdx.x, ne - 1);
+                er[u] = r0 + e / nf;
+                ec[u] = e % nf;
+                acc[u] = (er[u] >= ns && er[u] < f.nsp && ec[u] == er[u]) ? T(1) : T(0);
+            }
+            for (int k = 0; k < f.nch; ++k) {
+                const int *m = map + f.mo + (int64_t)k * nf;
+                const MfFront c = fr[ch[f.co + k]];
+                const T *cb = F + c.fo + (int64_t)c.nsp * c.ld + c.nsp;
+                int ia[U], ib[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ia[u] = m[er[u]];
+                    ib[u] = m[ec[u]];
+                }
+                T x[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) x[u] = cb[max(ia[u], 0) * c.ld + max(ib[u], 0)];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (ia[u] >= 0 && ib[u] >= 0) acc[u] += x[u];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + u * 256 + (int)threadIdx.x;
+                if (e < ne) sF[e] = acc[u];
             }
         }
         __syncthreads();
-        for (int64_t u = f.ao + threadIdx.x; u < f.ao + f.na; u += blockDim.x) {
-            const int64_t q = aoff[u] - f.fo;
-            const int r = (int)(q / f.ld), c = (int)(q % f.ld);
-            if (r < r0 || r >= r1) continue;
-            T acc = T(0);
-            for (int e = aseg[u]; e < aseg[u + 1]; ++e) acc += val[avid[e]];
-            sF[(r - r0) * nf + c] += acc;
+        // the Jacobian entries the front owns, U positions per thread and round
+        for (int64_t u0 = f.ao; u0 < f.ao + f.na; u0 += 256 * U) {
+            int64_t q[U];
+            int s0[U], s1[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t uu = min(u0 + u * 256 + (int64_t)threadIdx.x, f.ao + f.na - 1);
+                q[u] = aoff[uu] - f.fo;
+                s0[u] = aseg[uu];
+                s1[u] = aseg[uu + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int r = (int)(q[u] / f.ld), c = (int)(q[u] % f.ld);
+                if (u0 + u * 256 + (int64_t)threadIdx.x >= f.ao + f.na || r < r0 || r >= r1) continue;
+                T acc = T(0);
+                for (int e = s0[u]; e < s1[u]; ++e) acc += val[avid[e]];
+                sF[(r - r0) * nf + c] += acc;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -869,10 +904,20 @@ __global__ void __launch_bounds__(256) k_mf_fgather(const MfFront *__restrict__ 
     const int2 fa = rows[t];
     const MfFront f = fr[fa.x];
     const int a = fa.y;
-    T v = a < f.ns ? b[fu[f.uo + a]] : T(0);
-    for (int k = 0; k < f.nch; ++k) {
+    // clamped loads, absent terms dropped by a select (no load behind a branch)
+    const T bv = b[fu[f.uo + min(a, f.ns - 1)]];
+    T v = a < f.ns ? bv : T(0);
+    int k = 0;
+    for (; k + 2 <= f.nch; k += 2) {
+        const int m0 = map[f.mo + (int64_t)k * f.nf + a], m1 = map[f.mo + (int64_t)(k + 1) * f.nf + a];
+        const T x0 = Vf[cvo[f.co + k] + max(m0, 0)], x1 = Vf[cvo[f.co + k + 1] + max(m1, 0)];
+        if (m0 >= 0) v += x0;
+        if (m1 >= 0) v += x1;
+    }
+    for (; k < f.nch; ++k) {
         const int m = map[f.mo + (int64_t)k * f.nf + a];
-        if (m >= 0) v += Vf[cvo[f.co + k] + m];
+        const T x = Vf[cvo[f.co + k] + max(m, 0)];
+        if (m >= 0) v += x;
     }
     Vf[f.vo + a] = v;
 }
@@ -898,30 +943,37 @@ __device__ __forceinline__ double wave_sum64(double v)
     return v;
 }
 
-// Row products of one front against a vector staged in LDS: out(r) =
-// sum_c F[r][c0 + c] xs[c], in FP64, for the rows row0 + q G + g (q < 4) of a
-// block below rend, G = 256 / L rows per pass, L = row_lanes(ncol) lanes per
-// row (16-byte vectors, summed across the group by shuffles).  The four rows
-// of a lane group are read together, and their first vectors are loaded
-// before stage() fills xs (they do not depend on it).  ncol and c0 are
-// multiples of 4 (the vectors stay aligned); rows past rend read row rend - 1
-// and are dropped.
-template <typename T, typename Stage, typename Out>
+// Row products of one front against a vector staged in LDS: out(r, p, s)
+// with s = sum_c F[r][c0 + c] xs[c] in FP64, for the rows row0 + q G + g
+// (q < 4) of a block below rend, G = 256 / L rows per pass, L =
+// row_lanes(ncol) lanes per row (16-byte vectors, summed across the group by
+// shuffles).  The four rows of a lane group are read together: their first
+// vectors and pre(r), the per-row operands out() needs, are loaded before
+// stage() fills xs (they do not depend on it).  No load sits behind a branch
+// (addresses are clamped instead: a load in a branch is waited for before
+// the branch joins, which would serialise the rows), so a block waits for one
+// memory round trip before its sums.  ncol and c0 are multiples of 4 (the
+// vectors stay aligned); rows past rend read row rend - 1 and are dropped.
+template <typename T, typename Stage, typename Pre, typename Out>
 __device__ __forceinline__ void mf_rows(const MfFront &f, const T *F, int row0, int rend, int c0, int ncol,
-                                        const T *xs, Stage &&stage, Out &&out)
+                                        const T *xs, Stage &&stage, Pre &&pre, Out &&out)
 {
     using V = typename V16<T>::type;
     constexpr int W = V16<T>::w;
     const int L = row_lanes(ncol), G = 256 / L;
     const int l = threadIdx.x & (L - 1), g = threadIdx.x / L;
     const int nv = ncol / W;
+    const int cb = nv > 0 ? c0 : 0;              // no columns: read the row start
+    const int lv = nv > 0 ? min(l, nv - 1) : 0;
     const V *R[4];
     V x0[4];
+    decltype(pre(0)) pv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int r = min(row0 + q * G + g, rend - 1);
-        R[q] = reinterpret_cast<const V *>(F + f.fo + (int64_t)r * f.ld + c0);
-        if (nv > 0) x0[q] = R[q][min(l, nv - 1)];
+        R[q] = reinterpret_cast<const V *>(F + f.fo + (int64_t)r * f.ld + cb);
+        x0[q] = R[q][lv];
+        pv[q] = pre(r);
     }
     stage();
     __syncthreads();
@@ -939,7 +991,7 @@ __device__ __forceinline__ void mf_rows(const MfFront &f, const T *F, int row0, 
     for (int q = 0; q < 4; ++q) {
         for (int o = L >> 1; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
         const int r = row0 + q * G + g;
-        if (l == 0 && r < rend) out(r, acc[q]);
+        if (l == 0 && r < rend) out(r, pv[q], acc[q]);
     }
 }
 
@@ -959,11 +1011,19 @@ __global__ void __launch_bounds__(256) k_mf_fgemv(const MfFront *__restrict__ fr
         [&] {
             for (int a = threadIdx.x; a < f.nsp; a += blockDim.x) xs[a] = Vf[f.vo + a];
         },
-        [&](int r, double acc) {
+        [&](int r) { return Vf[f.vo + r]; },
+        [&](int r, T vb, double acc) {
             if (r < f.nsp) Zf[f.zo + r] = (T)acc;
-            else Vf[f.vo + r] = (T)((double)Vf[f.vo + r] + acc);
+            else Vf[f.vo + r] = (T)((double)vb + acc);
         });
 }
+
+// The per-row operands of a backward row: z and the row's unknown.
+template <typename T>
+struct ZU {
+    T z;
+    int u;
+};
 
 // Backward row products: x_s = z - W x_b, x_b gathered from the solution.
 template <typename T>
@@ -983,7 +1043,8 @@ __global__ void __launch_bounds__(256) k_mf_bwd(const MfFront *__restrict__ fr, 
         [&] {
             for (int q = threadIdx.x; q < nbp; q += blockDim.x) xs[q] = q < f.nb ? x[fu[f.uo + f.nsp + q]] : T(0);
         },
-        [&](int r, double acc) { x[fu[f.uo + r]] = (T)((double)Zf[f.zo + r] - acc); });
+        [&](int r) { return ZU<T>{Zf[f.zo + r], fu[f.uo + r]}; },
+        [&](int r, const ZU<T> &zu, double acc) { x[zu.u] = (T)((double)zu.z - acc); });
 }
 
 // Column products for the transposed solve: out(c) = sum_r F[r0 + r][c] ws[r]
