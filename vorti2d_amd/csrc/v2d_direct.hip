@@ -1009,7 +1009,7 @@ template <> struct Mfma<double> {
 constexpr int kSpinLimit = 1 << 21;
 
 template <typename T, typename BT = Batch<T>, int MODE = kStepFull>
-__global__ void __launch_bounds__(256) k_gj_step_mfma_b(BT B, int k0, int epoch,
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(BT B, int k0, int epoch,
                                                         int *__restrict__ flags, int *__restrict__ err)
 {
     using M = Mfma<T>;
