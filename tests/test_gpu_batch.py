@@ -80,9 +80,10 @@ def test_batch_adjoint_dcd_dalpha_vs_fd():
 
 def test_create_like_equals_fresh_context():
     """A context made by v2d_ctx_create_like (mesh, state and Jacobian pattern
-    copied on the device) computes what a fresh context on the same mesh and
-    state computes, bit for bit: the state itself, the FP64 force adjoint's
-    scalars and a Newton step.  The node-coordinate gradients agree to 1e-12:
+    copied on the device, the multifrontal analysis shared) computes what a
+    fresh context on the same mesh and state computes, bit for bit: the state
+    itself, the FP64 force adjoint's scalars (the force VJP gathers each wall
+    node's terms in a fixed order, no atomics) and a Newton step.  The node-coordinate gradients agree to 1e-12:
     the metric backward pass sums with FP64 atomics, so they are not
     bit-reproducible even between two calls on one context.  Outflow far
     field with a nonzero angle, so the outflow flags and their pattern rows

@@ -37,9 +37,73 @@ __global__ void k_bdf2_adjoint_rhs(int imax, int jmax, double inv2dt, const doub
 // J = wd Cd + wl Cl with Cd = 2 Fx / d, Cl = 2 Fy / d (body axes, fixed d):
 //     J = s * sum_i [ (wd y_i - wl x_i) dwdn_i ds_i - w_i (wd ny_i - wl nx_i) ds_i ],
 //     s = 2 / (d Re),
-// over the P = imax - 1 distinct wall nodes.  Gradients are accumulated with
-// atomics into g_ome (N), g_m (10 planes: beta, gamma, eta_x, eta_y, x, y at
-// the wall); out = {Cd, Cl, dJ/dRe}.  One workgroup.
+// over the P = imax - 1 distinct wall nodes.  Wall node w's terms reach its own
+// entries and its two neighbours' (the periodic xi difference of omega and
+// the arc lengths ds), so each thread recomputes the terms of w - 1 and w + 1
+// and adds the three contributions to node w in a fixed order: every entry of
+// g_ome (N) and g_m (10 planes: beta, gamma, eta_x, eta_y, x, y at the wall)
+// has one writer and the result does not depend on scheduling.
+// out = {Cd, Cl, dJ/dRe}.  One workgroup.
+struct WallTerms {
+    double a;              // g_oxi / (2 dksi): + to g_ome[w + 1], - to g_ome[w - 1]
+    double gome;           // to g_ome[w]
+    double gome1, gome2;   // to g_ome[imax + w], g_ome[2 imax + w]
+    double gb, gg, gex, gey;
+    double gx, gy;         // to g_x[w], g_y[w]
+    double hfx, hfy;       // to g_x[w + 1], g_y[w + 1]
+    double hbx, hby;       // minus these to g_x[w - 1], g_y[w - 1]
+    double fx, fy;         // the force sums' terms (times Re)
+};
+
+__device__ WallTerms wall_terms(int w, int imax, int P, double dksi, double deta, double s, double wd, double wl,
+                                const double *beta, const double *gama, const double *ex, const double *ey,
+                                const double *xp, const double *yp, const double *ome)
+{
+    WallTerms t;
+    const int wn = (w + 1) % P, wp = (w + P - 1) % P;
+    const double xw = xp[w], yw = yp[w], ow = ome[w];
+    const double bw = beta[w], gw = gama[w];
+    const double inv = 1.0 / sqrt(bw);
+    const double nx = ex[w] * inv, ny = ey[w] * inv;
+    const double oxi = (ome[wn] - ome[wp]) / (2.0 * dksi);
+    const double oeta = (-3.0 * ow + 4.0 * ome[imax + w] - ome[2 * imax + w]) / (2.0 * deta);
+    const double dwdn = (gw * oxi + bw * oeta) * inv;
+    const double dxf = xp[wn] - xw, dyf = yp[wn] - yw;
+    const double dxb = xw - xp[wp], dyb = yw - yp[wp];
+    const double segf = sqrt(dxf * dxf + dyf * dyf), segb = sqrt(dxb * dxb + dyb * dyb);
+    const double ds = 0.5 * (segf + segb);
+    t.fx = yw * dwdn * ds - ow * ny * ds;
+    t.fy = -xw * dwdn * ds + ow * nx * ds;
+    // --- pressure term  s (wd y - wl x) dwdn ds
+    const double cp = wd * yw - wl * xw;
+    const double g_dwdn = s * cp * ds;
+    const double g_oxi = g_dwdn * gw * inv, g_oeta = g_dwdn * bw * inv;
+    t.a = g_oxi / (2.0 * dksi);
+    t.gome1 = 4.0 * g_oeta / (2.0 * deta);
+    t.gome2 = -g_oeta / (2.0 * deta);
+    double gb = g_dwdn * (oeta * inv - 0.5 * dwdn / bw);
+    t.gg = g_dwdn * oxi * inv;
+    double gx = -s * wl * dwdn * ds, gy = s * wd * dwdn * ds;
+    double g_ds = s * cp * dwdn;
+    // --- friction term  -s w (wd ny - wl nx) ds,  n = (eta_x, eta_y) / sqrt(beta)
+    const double cf = wd * ny - wl * nx;
+    t.gome = -3.0 * g_oeta / (2.0 * deta) - s * cf * ds;
+    t.gey = -s * ow * wd * inv * ds;
+    t.gex = s * ow * wl * inv * ds;
+    gb += -s * ow * (wd * ey[w] - wl * ex[w]) * ds * (-0.5 * inv / bw);
+    t.gb = gb;
+    g_ds += -s * ow * cf;
+    // --- ds = (|X_wn - X_w| + |X_w - X_wp|) / 2
+    const double hf = 0.5 * g_ds / segf, hb = 0.5 * g_ds / segb;
+    t.gx = gx - hf * dxf + hb * dxb;
+    t.gy = gy - hf * dyf + hb * dyb;
+    t.hfx = hf * dxf;
+    t.hfy = hf * dyf;
+    t.hbx = hb * dxb;
+    t.hby = hb * dyb;
+    return t;
+}
+
 __global__ void __launch_bounds__(1024) k_forces_vjp(int imax, double re, double dksi, double deta,
                                                      const double *__restrict__ m, int64_t N,
                                                      const double *__restrict__ ome, double d,
@@ -56,53 +120,20 @@ __global__ void __launch_bounds__(1024) k_forces_vjp(int imax, double re, double
     double fx = 0.0, fy = 0.0;
     for (int w = threadIdx.x; w < P; w += blockDim.x) {
         const int wn = (w + 1) % P, wp = (w + P - 1) % P;
-        const double xw = xp[w], yw = yp[w], ow = ome[w];
-        const double bw = beta[w], gw = gama[w];
-        const double inv = 1.0 / sqrt(bw);
-        const double nx = ex[w] * inv, ny = ey[w] * inv;
-        const double oxi = (ome[wn] - ome[wp]) / (2.0 * dksi);
-        const double oeta = (-3.0 * ow + 4.0 * ome[imax + w] - ome[2 * imax + w]) / (2.0 * deta);
-        const double dwdn = (gw * oxi + bw * oeta) * inv;
-        const double dxf = xp[wn] - xw, dyf = yp[wn] - yw;
-        const double dxb = xw - xp[wp], dyb = yw - yp[wp];
-        const double segf = sqrt(dxf * dxf + dyf * dyf), segb = sqrt(dxb * dxb + dyb * dyb);
-        const double ds = 0.5 * (segf + segb);
-        // forward sums (Fx, Fy times Re)
-        fx += yw * dwdn * ds - ow * ny * ds;
-        fy += -xw * dwdn * ds + ow * nx * ds;
-        // --- pressure term  s (wd y - wl x) dwdn ds
-        const double cp = wd * yw - wl * xw;
-        const double g_dwdn = s * cp * ds;
-        const double g_oxi = g_dwdn * gw * inv, g_oeta = g_dwdn * bw * inv;
-        atomicAdd(&g_ome[wn], g_oxi / (2.0 * dksi));
-        atomicAdd(&g_ome[wp], -g_oxi / (2.0 * dksi));
-        atomicAdd(&g_ome[w], -3.0 * g_oeta / (2.0 * deta));
-        atomicAdd(&g_ome[imax + w], 4.0 * g_oeta / (2.0 * deta));
-        atomicAdd(&g_ome[2 * imax + w], -g_oeta / (2.0 * deta));
-        double gb = g_dwdn * (oeta * inv - 0.5 * dwdn / bw);
-        const double gg = g_dwdn * oxi * inv;
-        double gx = -s * wl * dwdn * ds, gy = s * wd * dwdn * ds;
-        double g_ds = s * cp * dwdn;
-        // --- friction term  -s w (wd ny - wl nx) ds,  n = (eta_x, eta_y) / sqrt(beta)
-        const double cf = wd * ny - wl * nx;
-        atomicAdd(&g_ome[w], -s * cf * ds);
-        const double gey = -s * ow * wd * inv * ds, gex = s * ow * wl * inv * ds;
-        gb += -s * ow * (wd * ey[w] - wl * ex[w]) * ds * (-0.5 * inv / bw);
-        g_ds += -s * ow * cf;
-        atomicAdd(&g_bt[w], gb);
-        atomicAdd(&g_ga[w], gg);
-        atomicAdd(&g_ex[w], gex);
-        atomicAdd(&g_ey[w], gey);
-        // --- ds = (|X_wn - X_w| + |X_w - X_wp|) / 2
-        const double hf = 0.5 * g_ds / segf, hb = 0.5 * g_ds / segb;
-        gx += -hf * dxf + hb * dxb;
-        gy += -hf * dyf + hb * dyb;
-        atomicAdd(&g_x[wn], hf * dxf);
-        atomicAdd(&g_y[wn], hf * dyf);
-        atomicAdd(&g_x[wp], -hb * dxb);
-        atomicAdd(&g_y[wp], -hb * dyb);
-        atomicAdd(&g_x[w], gx);
-        atomicAdd(&g_y[w], gy);
+        const WallTerms t = wall_terms(w, imax, P, dksi, deta, s, wd, wl, beta, gama, ex, ey, xp, yp, ome);
+        const WallTerms tp = wall_terms(wp, imax, P, dksi, deta, s, wd, wl, beta, gama, ex, ey, xp, yp, ome);
+        const WallTerms tn = wall_terms(wn, imax, P, dksi, deta, s, wd, wl, beta, gama, ex, ey, xp, yp, ome);
+        fx += t.fx;
+        fy += t.fy;
+        g_ome[w] += (t.gome + tp.a) - tn.a;
+        g_ome[imax + w] += t.gome1;
+        g_ome[2 * imax + w] += t.gome2;
+        g_bt[w] += t.gb;
+        g_ga[w] += t.gg;
+        g_ex[w] += t.gex;
+        g_ey[w] += t.gey;
+        g_x[w] += (t.gx + tp.hfx) - tn.hbx;
+        g_y[w] += (t.gy + tp.hfy) - tn.hby;
     }
     const double Fx = block_sum(fx, sh), Fy = block_sum(fy, sh);
     if (threadIdx.x == 0) {

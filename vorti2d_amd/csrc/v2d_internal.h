@@ -382,6 +382,7 @@ struct MfLevel {
     int64_t s0, s1;          // assembly blocks of the big fronts
     int nch2;                // 1: every big front of the height has two children
     int nfmax_small, nfmax_big, npmax_big;
+    int small_ni[10];        // first small front swept with a register tile of NI 16-row blocks
     int64_t pool;            // Gauss-Jordan panel elements of the big fronts
     int64_t job0;            // first job of the level in the job table
 };

@@ -48,6 +48,7 @@ namespace {
 
 constexpr int kMfLeaf = 32;      // most nodes of a leaf supernode
 constexpr int kMfSmall = 144;    // largest front swept in registers by one workgroup (16 x 9)
+constexpr int kSmallMinNI = 3, kSmallMaxNI = 9;   // register-tile sizes of the small-front kernel
 constexpr int kMfAlign = 4;      // pivot padding of small fronts
 constexpr int kMfRow = 32;       // row stride of every front, in elements (128-byte FP32 rows)
 constexpr int kNB = 64;          // Gauss-Jordan block of the large fronts
@@ -295,9 +296,15 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
     auto is_big = [&](int s) {
         return round_up(2 * (int)D.sep[s].size(), kMfAlign) + 2 * (int)st[s].size() > kMfSmall;
     };
+    // small fronts by size within a height, so that each register-tile size
+    // (16-row blocks, see k_mf_small_fused) covers a contiguous range
+    auto nf_small = [&](int s) {
+        return is_big(s) ? 0 : round_up(2 * (int)D.sep[s].size(), kMfAlign) + 2 * (int)st[s].size();
+    };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         if (height[a] != height[b]) return height[a] < height[b];
-        return (int)is_big(a) < (int)is_big(b);
+        if (is_big(a) != is_big(b)) return (int)is_big(a) < (int)is_big(b);
+        return nf_small(a) < nf_small(b);
     });
     std::vector<int> fid(ns_);
     for (int f = 0; f < ns_; ++f) fid[order[f]] = f;
@@ -474,6 +481,7 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
         L.s0 = (int64_t)P.asmb.size();
         L.nch2 = 1;
         L.nfmax_small = L.nfmax_big = L.npmax_big = 0;
+        for (int q = 0; q <= kSmallMaxNI; ++q) L.small_ni[q] = L.f0;
         L.pool = 0;
         for (int f = L.f0; f < L.f1; ++f) {
             const MfFront &F = P.fr[f];
@@ -491,6 +499,13 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
             } else {
                 L.nfmax_small = std::max(L.nfmax_small, F.nf);
             }
+        }
+        // first small front of each register-tile size: NI = ceil(nf / 16), at
+        // least kSmallMinNI (the small fronts are sorted by size)
+        for (int q = 0; q <= kSmallMaxNI; ++q) {
+            int f = L.f0;
+            while (f < L.fbig && std::max(kSmallMinNI, (P.fr[f].nf + 15) / 16) < q) ++f;
+            L.small_ni[q] = f;
         }
         L.r1 = (int64_t)P.rows.size();
         L.rbig = L.r1;
@@ -1190,17 +1205,25 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
                 }
             }
         if (L.fbig > L.f0) {
-            const dim3 g((unsigned)(L.fbig - L.f0));
             MF_BEGIN();
-            if (L.nfmax_small <= 64)
-                hipLaunchKernelGGL((k_mf_small_fused<T, 4>), g, dim3(256), 0, s, P.d_fr, L.f0, P.d_ch, P.d_map,
-                                   P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
-            else if (L.nfmax_small <= 96)
-                hipLaunchKernelGGL((k_mf_small_fused<T, 6>), g, dim3(256), 0, s, P.d_fr, L.f0, P.d_ch, P.d_map,
-                                   P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
-            else
-                hipLaunchKernelGGL((k_mf_small_fused<T, 9>), g, dim3(256), 0, s, P.d_fr, L.f0, P.d_ch, P.d_map,
-                                   P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
+            for (int ni = kSmallMinNI; ni <= kSmallMaxNI; ++ni) {
+                const int a0 = L.small_ni[ni], a1 = ni < kSmallMaxNI ? L.small_ni[ni + 1] : L.fbig;
+                if (a1 <= a0) continue;
+                const dim3 g((unsigned)(a1 - a0));
+#define V2D_SMALL(NI)                                                                                     \
+    hipLaunchKernelGGL((k_mf_small_fused<T, NI>), g, dim3(256), 0, s, P.d_fr, a0, P.d_ch, P.d_map, P.d_aoff, \
+                       P.d_aseg, P.d_avid, val, W.F)
+                switch (ni) {
+                case 3: V2D_SMALL(3); break;
+                case 4: V2D_SMALL(4); break;
+                case 5: V2D_SMALL(5); break;
+                case 6: V2D_SMALL(6); break;
+                case 7: V2D_SMALL(7); break;
+                case 8: V2D_SMALL(8); break;
+                default: V2D_SMALL(9); break;
+                }
+#undef V2D_SMALL
+            }
             MF_END(kSmall, small_fl, small_b);
         }
         if (L.s1 > L.s0) {
