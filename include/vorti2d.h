@@ -203,6 +203,14 @@ int v2d_ctx_get_linear_solver(v2d_ctx *ctx, int *kind);
  * schedules give the same factorisation up to rounding.  Takes effect at the
  * next factorisation. */
 int v2d_ctx_set_step_pairing(v2d_ctx *ctx, int paired);
+/* Multifrontal fronts of a single 64-pivot Gauss-Jordan step (no counterpart
+ * in the reference; a tuning choice of this library): fused != 0 (the
+ * default) assembles each such front, adds the Jacobian entries it owns and
+ * sweeps it in one workgroup of one launch per tree height; 0 takes them
+ * through the level assembly, entry scatter and step launches like the larger
+ * fronts.  Same factorisation up to rounding.  Takes effect at the next
+ * factorisation. */
+int v2d_ctx_set_front_fusion(v2d_ctx *ctx, int fused);
 /* Sizes of the multifrontal analysis of a context's grid (built on first
  * call): out[0..7] = fronts, tree heights, fronts swept by the matrix-core
  * Gauss-Jordan steps, front storage (elements), front-vector elements,
@@ -213,12 +221,13 @@ int v2d_ctx_mf_info(v2d_ctx *ctx, int64_t *out, double *flops);
 
 /* Factorise the last assembled Jacobian and solve one system with the
  * multifrontal LU, timing every launch with HIP events on the context's
- * stream.  out[4 g .. 4 g + 3] for the kernel groups g = 0..9 (front assembly,
+ * stream.  out[4 g .. 4 g + 3] for the kernel groups g = 0..10 (front assembly,
  * entry scatter, small-front sweep, first Gauss-Jordan step (pivot inverse,
  * pivot rows and panels), full Gauss-Jordan steps, light steps, heavy steps,
- * forward gather, forward front products, backward front products):
- * milliseconds, launches, algorithmic flops, algorithmic bytes; out[40],
- * out[41]: the factorisation's and the solve's event spans (ms).  The
+ * one-step fronts assembled and swept in one launch, forward gather, forward
+ * front products, backward front products): milliseconds, launches,
+ * algorithmic flops, algorithmic bytes; out[44], out[45]: the factorisation's
+ * and the solve's event spans (ms).  The
  * factorisation is kept.  No counterpart in the reference. */
 int v2d_ctx_profile_mf(v2d_ctx *ctx, double *out);
 

@@ -13,6 +13,10 @@ impulsive start).
   residual is about 1000x the device solve's;
 * the paired Gauss-Jordan step schedule (light / heavy steps) and the
   unpaired one give the same solution up to rounding (1e-12 in FP64);
+* the fronts of one 64-pivot step assembled and swept in one launch
+  (k_mf_front1, the default) and through the level assembly, entry scatter
+  and step launches give the same factored fronts (every entry of every
+  front, 1e-12 of the largest in FP64, 1e-5 in FP32) and solutions;
 * FP32 (the mixed path's preconditioner) within 1e-3 of SuperLU;
 * at 513^2 (multi-step Gauss-Jordan fronts, up to 17 pivot blocks) the FP64
   residual of the Newton right-hand side is below 1e-8 and within 4x of the
@@ -45,6 +49,40 @@ def _ctx(v, n, ff, prec, solver):
     ctx.set_linear_solver(solver)
     ctx.init_state()
     return ctx
+
+
+def _front_mask(n):
+    """Entries of the front storage that hold a front's rows and columns
+    (not the row padding nf .. ld-1), from the host analysis of the grid with
+    no far-field outflow coupling."""
+    from vorti2d_amd import mf
+    imax, jmax = n if isinstance(n, tuple) else (n, n)
+    a = mf.analyse(imax, jmax)
+    t = dict(zip(mf.FRONT_FIELDS, a["table"].T))
+    m = np.zeros(a["front_elems"], bool)
+    for fo, nf, ld in zip(t["fo"], t["nf"], t["ld"]):
+        m[fo:fo + nf * ld].reshape(nf, ld)[:, :nf] = True
+    return m
+
+
+def _fused_vs_steps(ctx, n, b, tol):
+    """Factor with and without the one-step front fusion; the largest
+    difference of the fronts (relative to the largest entry; ff 0 grids only)
+    and of the solutions."""
+    ctx.factor()
+    F, x = ctx.mf_fronts(), ctx.solve(b)
+    ctx.set_front_fusion(False)
+    ctx.factor()
+    Fu, xu = ctx.mf_fronts(), ctx.solve(b)
+    ctx.set_front_fusion(True)
+    df = 0.0
+    if n is not None:
+        m = _front_mask(n)
+        assert m.size == F.size
+        df = np.abs(F[m] - Fu[m]).max() / np.abs(Fu[m]).max()
+    dx = np.linalg.norm(x - xu) / np.linalg.norm(xu)
+    say(f"  one-step fronts fused vs by steps: fronts {df:.2e}, solutions {dx:.2e}")
+    assert df <= tol and dx <= tol
 
 
 def _system_after_one_step(ctx):
@@ -83,6 +121,7 @@ def test_fp64_solves_match_superlu(n, ff):
     d = max(np.linalg.norm(x - xu) / np.linalg.norm(xu), np.linalg.norm(xt - xtu) / np.linalg.norm(xtu))
     say(f"  paired vs unpaired steps {d:.2e}")
     assert d <= 1e-12
+    _fused_vs_steps(ctx, n if ff == 0 else None, b, 1e-12)
     blu = ctx.like("fp64")
     blu.set_linear_solver("block_lu")
     blu.assemble(40.0)
@@ -105,6 +144,7 @@ def test_fp32_solve_close_to_superlu():
     e = np.linalg.norm(x - xr) / np.linalg.norm(xr)
     say(f"multifrontal FP32 97x97: {e:.2e} relative to SuperLU")
     assert e <= 1e-3
+    _fused_vs_steps(ctx, 97, b, 1e-5)
     ctx.close()
 
 

@@ -50,6 +50,7 @@ _SIGS = {
     "v2d_ctx_set_forcing": ([vp, f64, f64], i32),
     "v2d_ctx_set_update_kernel": ([vp, i32], i32),
     "v2d_ctx_set_step_pairing": ([vp, i32], i32),
+    "v2d_ctx_set_front_fusion": ([vp, i32], i32),
     "v2d_ctx_set_sweep_kernel": ([vp, i32], i32),
     "v2d_ctx_set_sweep_spin_limit": ([vp, i32], i32),
     "v2d_ctx_get_sweep_kernel": ([vp, P(i32), P(i64)], i32),

@@ -939,6 +939,14 @@ int v2d_ctx_set_step_pairing(v2d_ctx *ctx, int paired)
     return 0;
 }
 
+int v2d_ctx_set_front_fusion(v2d_ctx *ctx, int fused)
+{
+    Ctx *c = ctx->c.get();
+    V2D_HIP(hipStreamSynchronize(c->s));
+    c->md.fused = c->mw.fused = fused != 0;
+    return 0;
+}
+
 int v2d_ctx_get_linear_solver(v2d_ctx *ctx, int *kind)
 {
     *kind = ctx->c->solver;

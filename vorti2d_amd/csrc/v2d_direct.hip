@@ -1402,6 +1402,301 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
         }
 }
 
+// ------------------------------------------------ one-step big fronts ----
+// A multifrontal front with a single 64-pivot step (nsp = 64, two children,
+// nf <= kFront1MaxNf) assembled and swept by one workgroup, with no launch and
+// no wait on another workgroup between the phases:
+//   1. the pivot rows [G W] (rows 0 .. 63, every column) and the pivot
+//      columns X (rows 64 .., columns 0 .. 63) are assembled into F as the
+//      level assembly does -- identity on the padding pivots, then the
+//      children's contribution blocks in list order -- and the Jacobian
+//      entries the front owns are added (the analysis checks that they all lie
+//      in these two panels);
+//   2. G is inverted in LDS (gj_tile) and stored as the pivot block;
+//   3. per 64-column tile: the pivot rows P = G^-1 W go to F and, transposed,
+//      to the job's row panel;
+//   4. per 64-row tile: X is staged in LDS and -X G^-1 stored as the pivot
+//      columns; per 64-column tile the trailing block C is gathered from the
+//      children straight into the matrix-core accumulators, takes - X P and is
+//      stored (zero in the padding columns nf .. ld-1).
+// The result is the assembly, scatter and Gauss-Jordan step launches' one,
+// [G^-1, G^-1 W; -X G^-1, C - X G^-1 W], up to rounding.  256 threads; every
+// global access is bounded by the front (rows < nf, columns < ld) or by a
+// child's contribution block (clamped map entries).
+template <typename T>
+__global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict__ fr, int f0,
+                                                      const Job<T> *__restrict__ jobs, const int *__restrict__ ch,
+                                                      const int *__restrict__ map, const int64_t *__restrict__ aoff,
+                                                      const int *__restrict__ aseg, const int *__restrict__ avid,
+                                                      const T *__restrict__ val, T *__restrict__ F)
+{
+    using M = Mfma<T>;
+    using C = typename M::C;
+    using NV = typename NVec<T>::type;
+    constexpr int VW = Vec8<T>::w;   // elements of one 16-byte access
+    constexpr int LT = NB + 4;       // padded LDS row stride
+    __shared__ T sD[NB * LT];        // the pivot block, then its inverse
+    __shared__ T sX[NB * LT];        // one 64-row tile of the pivot columns
+    __shared__ __align__(16) int smap[2][kFront1MaxNf];   // the children's maps, -1 past nf
+    const MfFront f = fr[f0 + blockIdx.x];
+    const Job<T> J = jobs[blockIdx.x];
+    T *__restrict__ A = F + f.fo;
+    const int nf = f.nf, ld = f.ld, ns = f.ns;
+    const int nt = (nf + NB - 1) / NB;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+    const int rw = (w >> 1) * 32, cw = (w & 1) * 32;   // the wave's 32 x 32 quarter of a tile
+    const T *cb[2];
+    int cld[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const MfFront c = fr[ch[f.co + k]];
+        cb[k] = F + c.fo + (int64_t)c.nsp * c.ld + c.nsp;
+        cld[k] = c.ld;
+    }
+    for (int a = t; a < kFront1MaxNf; a += 256) {
+        const int q = min(a, nf - 1);
+        const int m0 = map[f.mo + q], m1 = map[f.mo + nf + q];
+        smap[0][a] = a < nf ? m0 : -1;
+        smap[1][a] = a < nf ? m1 : -1;
+    }
+    __syncthreads();
+    // ------------------------------------------------ 1. the panels ----
+    {
+        const int ld4 = ld >> 2;
+        const int n1 = NB * ld4, nitems = n1 + (nf - NB) * (NB / 4);
+        constexpr int U = 4;
+        for (int e0 = 0; e0 < nitems; e0 += 256 * U) {
+            int r[U], c[U];
+            T acc[U][4];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = min(e0 + u * 256 + t, nitems - 1);
+                const int q = e / ld4, e2 = e - n1;
+                r[u] = e < n1 ? q : NB + (e2 >> 4);
+                c[u] = e < n1 ? 4 * (e - q * ld4) : 4 * (e2 & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = (r[u] >= ns && r[u] < NB && c[u] + v == r[u]) ? T(1) : T(0);
+            }
+            // every map read and contribution load of the round before the first sum
+            T x[2][U][4];
+            bool in[2][U][4];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int ia = smap[k][r[u]];
+                    const int4 ib = *reinterpret_cast<const int4 *>(&smap[k][c[u]]);
+                    const int ibv[4] = {ib.x, ib.y, ib.z, ib.w};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        in[k][u][v] = ia >= 0 && ibv[v] >= 0;
+                        x[k][u][v] = glb(cb[k])[max(ia, 0) * cld[k] + max(ibv[v], 0)];
+                    }
+                }
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (in[k][u][v]) acc[u][v] += x[k][u][v];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                T *d = A + (int64_t)r[u] * ld + c[u];
+                if constexpr (sizeof(T) == 4) {
+                    f32x4 o;
+                    o[0] = acc[u][0]; o[1] = acc[u][1]; o[2] = acc[u][2]; o[3] = acc[u][3];
+                    *glb(reinterpret_cast<f32x4 *>(d)) = o;
+                } else {
+                    f64x2 o0, o1;
+                    o0[0] = acc[u][0]; o0[1] = acc[u][1]; o1[0] = acc[u][2]; o1[1] = acc[u][3];
+                    glb(reinterpret_cast<f64x2 *>(d))[0] = o0;
+                    glb(reinterpret_cast<f64x2 *>(d))[1] = o1;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // the Jacobian entries the front owns (the sums of the entry scatter)
+    for (int64_t u = f.ao + t; u < f.ao + f.na; u += 256) {
+        T acc = T(0);
+        for (int e = aseg[u]; e < aseg[u + 1]; ++e) acc += val[avid[e]];
+        glb(F)[aoff[u]] += acc;
+    }
+    __syncthreads();
+    // ---------------------------------------------- 2. pivot inverse ----
+    for (int e = t; e < NB * NB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        sD[r * LT + c] = glb(A)[(int64_t)r * ld + c];
+    }
+    {
+        T g[16];
+        gj_tile<T>(sD, LT, NB, g, J.dinv);   // sD holds the inverse on return
+#pragma unroll
+        for (int q = 0; q < 16; ++q) glb(A)[(int64_t)(w * 16 + q) * ld + l] = g[q];
+    }
+    // ------------------------------------------------ 3. pivot rows ----
+    for (int ct = 1; ct < nt; ++ct) {
+        const int c0 = ct * NB;
+        T bw[2][16];
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int cc = c0 + cw + bj * 16 + li;
+            const int ccl = min(cc, ld - 1);
+#pragma unroll
+            for (int h = 0; h < 16; ++h) {
+                const T x = glb(A)[(int64_t)(4 * h + lk) * ld + ccl];
+                bw[bj][h] = cc < nf ? x : T(0);
+            }
+        }
+        __syncthreads();   // every wave holds its part of W before the pivot rows overwrite it
+        C d[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) d[bi][bj][v] = T(0);
+#pragma unroll
+                for (int h = 0; h < 16; ++h)
+                    d[bi][bj] = M::opd(sD[(rw + bi * 16 + li) * LT + 4 * h + lk], bw[bj][h], d[bi][bj]);
+            }
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                const int cc = c0 + cw + bj * 16 + li;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = rw + bi * 16 + M::drow(lk, v);
+                    if (cc < nf) glb(A)[(int64_t)r * ld + cc] = d[bi][bj][v];
+                    glb(J.rowk)[(int64_t)cc * NB + r] = d[bi][bj][v];
+                }
+            }
+    }
+    __syncthreads();
+    // -------------------------------- 4. pivot columns and the update ----
+    for (int it = 1; it < nt; ++it) {
+        const int r0 = it * NB;
+        for (int e = t; e < NB * (NB / VW); e += 256) {
+            const int r = e / (NB / VW), c = (e % (NB / VW)) * VW;
+            const NV x = *glb(reinterpret_cast<const NV *>(A + (int64_t)min(r0 + r, nf - 1) * ld + c));
+#pragma unroll
+            for (int q = 0; q < VW; ++q) sX[r * LT + c + q] = r0 + r < nf ? x[q] : T(0);
+        }
+        __syncthreads();
+        if (r0 + rw < nf) {   // the wave's rows reach into the front
+            // pivot columns -X G^-1
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    C y;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) y[v] = T(0);
+#pragma unroll
+                    for (int h = 0; h < 16; ++h)
+                        y = M::opd(sX[(rw + bi * 16 + li) * LT + 4 * h + lk],
+                                   sD[(4 * h + lk) * LT + cw + bj * 16 + li], y);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = r0 + rw + bi * 16 + M::drow(lk, v);
+                        if (r < nf) glb(A)[(int64_t)r * ld + cw + bj * 16 + li] = -y[v];
+                    }
+                }
+            // trailing blocks
+            for (int ct = 1; ct < nt; ++ct) {
+                const int c0 = ct * NB;
+                if (c0 + cw >= ld) break;   // the wave's columns lie past the row
+                // every gather of C and every load of the row panel before the first sum
+                T x[2][2][2][4];
+                bool in[2][2][2][4];
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int ia = smap[k][r0 + rw + bi * 16 + M::row(li, lk, v)];
+                                const int ib = smap[k][c0 + cw + bj * 16 + M::col(li, lk, v)];
+                                in[k][bi][bj][v] = ia >= 0 && ib >= 0;
+                                x[k][bi][bj][v] = glb(cb[k])[max(ia, 0) * cld[k] + max(ib, 0)];
+                            }
+                NV pb[4][2][4 / VW];
+#pragma unroll
+                for (int h = 0; h < 4; ++h)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                        for (int u = 0; u < 4 / VW; ++u)
+                            pb[h][bj][u] = *glb(reinterpret_cast<const NV *>(
+                                J.rowk + (int64_t)(c0 + cw + bj * 16 + li) * NB + 16 * h + 4 * lk + u * VW));
+                C acc[2][2];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            T s = T(0);
+#pragma unroll
+                            for (int k = 0; k < 2; ++k)
+                                if (in[k][bi][bj][v]) s += x[k][bi][bj][v];
+                            acc[bi][bj][v] = s;
+                        }
+                // C - X P, k = 16 h + 4 lk + e
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    T av[2][4], bv[2][4];
+#pragma unroll
+                    for (int bi = 0; bi < 2; ++bi) {
+                        const T *p = &sX[(rw + bi * 16 + li) * LT + 16 * h + 4 * lk];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) av[bi][e] = -p[e];
+                    }
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                        for (int u = 0; u < 4 / VW; ++u)
+#pragma unroll
+                            for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = pb[h][bj][u][e];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+                }
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        if constexpr (M::kSwap) {
+                            // one row, four consecutive columns per lane
+                            const int r = r0 + rw + bi * 16 + li, c = c0 + cw + bj * 16 + 4 * lk;
+                            if (r < nf && c < ld) {
+                                f32x4 o;
+#pragma unroll
+                                for (int v = 0; v < 4; ++v) o[v] = c + v < nf ? acc[bi][bj][v] : T(0);
+                                *glb(reinterpret_cast<f32x4 *>(A + (int64_t)r * ld + c)) = o;
+                            }
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int r = r0 + rw + bi * 16 + M::row(li, lk, v);
+                                const int c = c0 + cw + bj * 16 + M::col(li, lk, v);
+                                if (r < nf && c < ld) glb(A)[(int64_t)r * ld + c] = c < nf ? acc[bi][bj][v] : T(0);
+                            }
+                        }
+                    }
+            }
+        }
+        __syncthreads();   // sX is restaged by the next row tile
+    }
+}
+
 // ------------------------------------------------------ launch probe ------
 LaunchProbe::~LaunchProbe()
 {
@@ -1559,6 +1854,21 @@ template int gj_table_begin<float>(const Job<float> *, int, int, int *, int *, i
 template int gj_table_begin<double>(const Job<double> *, int, int, int *, int *, int &, hipStream_t);
 template int gj_table_step<float>(const Job<float> *, int, int, int, int, bool, int *, int *, int &, hipStream_t);
 template int gj_table_step<double>(const Job<double> *, int, int, int, int, bool, int *, int *, int &, hipStream_t);
+
+template <typename T>
+int mf_front1(const MfFront *fr, int f0, int nfront, const Job<T> *jobs, const int *ch, const int *map,
+              const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s)
+{
+    if (nfront <= 0) return 0;
+    hipLaunchKernelGGL(k_mf_front1<T>, dim3((unsigned)nfront), dim3(256), 0, s, fr, f0, jobs, ch, map, aoff, aseg,
+                       avid, val, F);
+    V2D_LAUNCH();
+    return 0;
+}
+template int mf_front1<float>(const MfFront *, int, int, const Job<float> *, const int *, const int *,
+                              const int64_t *, const int *, const int *, const float *, float *, hipStream_t);
+template int mf_front1<double>(const MfFront *, int, int, const Job<double> *, const int *, const int *,
+                               const int64_t *, const int *, const int *, const double *, double *, hipStream_t);
 
 // ------------------------------------------------------------ factor ------
 // Twisted block factorisation: the bottom chain eliminates levels 0..M-1 from

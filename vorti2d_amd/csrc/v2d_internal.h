@@ -370,6 +370,15 @@ struct MfFront {
     int big;       // 1: swept by the dataflow Gauss-Jordan steps (nsp a multiple of 64)
 };
 
+// Big fronts of one 64-pivot step with two children and nf <= kFront1MaxNf,
+// assembled (children, then the entries they own) and swept in one launch,
+// one workgroup per front: fronts f0 .. f0 + nfront - 1, jobs[q] the
+// Gauss-Jordan job (panels) of front f0 + q.
+constexpr int kFront1MaxNf = 512;
+template <typename T>
+int mf_front1(const MfFront *fr, int f0, int nfront, const Job<T> *jobs, const int *ch, const int *map,
+              const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s);
+
 struct MfLevel {
     int f0, f1, fbig;        // fronts [f0, f1); [fbig, f1) are big
     int64_t r0, r1;          // rows table range (every front row)
@@ -383,6 +392,8 @@ struct MfLevel {
     int nch2;                // 1: every big front of the height has two children
     int nfmax_small, nfmax_big, npmax_big;
     int small_ni[10];        // first small front swept with a register tile of NI 16-row blocks
+    int one_step;            // 1: the big fronts fit mf_front1 (one 64-pivot step, two children,
+                             //    nf <= kFront1MaxNf, every entry in the pivot rows or columns)
     int64_t pool;            // Gauss-Jordan panel elements of the big fronts
     int64_t job0;            // first job of the level in the job table
 };
@@ -418,7 +429,7 @@ struct MfPattern {
 // Per-kernel timing of a factorisation and a solve (HIP events around every
 // launch on the stream) with the algorithmic counts of what was timed.
 struct MfProbe {
-    enum { kAssemble, kScatter, kSmall, kGjBegin, kGjFull, kGjLight, kGjHeavy, kGather, kFwd, kBwd, kGroups };
+    enum { kAssemble, kScatter, kSmall, kGjBegin, kGjFull, kGjLight, kGjHeavy, kFront, kGather, kFwd, kBwd, kGroups };
     double ms[kGroups] = {}, flops[kGroups] = {}, bytes[kGroups] = {};
     int64_t launches[kGroups] = {};
     std::vector<hipEvent_t> ev;     // pairs, recorded
@@ -444,6 +455,7 @@ struct MfWork {
     int epoch = 0;
     bool factored = false;
     bool paired = true;       // Gauss-Jordan steps in light / heavy pairs (else every step full)
+    bool fused = true;        // one-step big fronts by mf_front1 (else assembly, scatter and steps)
 };
 template <typename T>
 int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s);

@@ -521,6 +521,17 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
                 r += P.fr[f].nf;
             }
         }
+        L.one_step = L.fbig < L.f1 && L.nch2 && L.npmax_big == kNB && L.nfmax_big <= kFront1MaxNf;
+        for (int f = L.fbig; f < L.f1 && L.one_step; ++f) {
+            const MfFront &F = P.fr[f];
+            for (int64_t u = F.ao; u < F.ao + F.na; ++u) {
+                const int64_t q = P.aoff[u] - F.fo;
+                if (q / F.ld >= F.nsp && q % F.ld >= F.nsp) {
+                    L.one_step = 0;
+                    break;
+                }
+            }
+        }
         L.g1 = (int64_t)P.gemv.size();
         L.b1 = (int64_t)P.bwd.size();
         L.c1 = (int64_t)P.colt.size();
@@ -633,9 +644,10 @@ void mf_free(MfWork<T> &W, hipStream_t s)
 {
     void *ptrs[] = {W.F, W.Vf, W.Zf, W.pool, W.jobs, W.flags};
     for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
-    const bool paired = W.paired;
+    const bool paired = W.paired, fused = W.fused;
     W = MfWork<T>{};
     W.paired = paired;
+    W.fused = fused;
 }
 
 // ------------------------------------------------------------ kernels ----
@@ -1187,8 +1199,10 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
     for (int h = 0; h < P.nheight; ++h) {
         const MfLevel &L = P.lv[h];
         const int64_t nr = L.r1 - L.r0;
+        // one-step big fronts: assembled, scattered and swept by mf_front1
+        const bool one = L.one_step && W.fused;
         // algorithmic counts of the height (host, only when timed)
-        double asm_b = 0, small_fl = 0, small_b = 0;
+        double asm_b = 0, small_fl = 0, small_b = 0, one_fl = 0, one_b = 0;
         if (probe)
             for (int f = L.f0; f < L.f1; ++f) {
                 const MfFront &F = P.fr[f];
@@ -1197,7 +1211,10 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
                     const MfFront &C = P.fr[P.ch[F.co + k]];
                     cb += ts * (double)C.nb * C.nb;
                 }
-                if (F.big) {
+                if (F.big && one) {
+                    one_fl += 2.0 * kNB * (double)(F.nf - kNB) * F.nf;
+                    one_b += ts * (double)F.nf * F.ld + cb + (ts + 12.0) * F.na;
+                } else if (F.big) {
                     asm_b += ts * (double)F.nf * F.ld + cb;
                 } else {
                     small_fl += 2.0 * F.ns * (double)F.nf * F.nf;
@@ -1226,7 +1243,7 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
             }
             MF_END(kSmall, small_fl, small_b);
         }
-        if (L.s1 > L.s0) {
+        if (!one && L.s1 > L.s0) {
             const dim3 g((unsigned)(L.s1 - L.s0));
             MF_BEGIN();
             if (L.nch2)
@@ -1237,7 +1254,7 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
                                    P.d_map, W.F);
             MF_END(kAssemble, 0.0, asm_b);
         }
-        if (L.a1 > L.abig) {
+        if (!one && L.a1 > L.abig) {
             MF_BEGIN();
             hipLaunchKernelGGL(k_mf_scatter<T>, dim3(grid_for(L.a1 - L.abig)), dim3(256), 0, s, L.abig, L.a1,
                                P.d_aoff, P.d_aseg, P.d_avid, val, W.F);
@@ -1246,6 +1263,14 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
         V2D_LAUNCH();
         const int nj = L.f1 - L.fbig;
         if (nj <= 0 || L.npmax_big <= 0) continue;
+        if (one) {
+            MF_BEGIN();
+            if (mf_front1<T>(P.d_fr, L.fbig, nj, W.jobs + L.job0, P.d_ch, P.d_map, P.d_aoff, P.d_aseg, P.d_avid,
+                             val, W.F, s))
+                return -1;
+            MF_END(kFront, one_fl, one_b);
+            continue;
+        }
         // Steps in pairs: a light step (the next panels only) and a heavy step
         // (every other tile takes both steps' panels), a last unpaired step full.
         const int nsteps = L.npmax_big / kNB;

@@ -163,6 +163,14 @@ class DeviceContext:
         factorisation up to rounding; applies from the next factorisation."""
         check(lib.v2d_ctx_set_step_pairing(self._h, 1 if paired else 0))
 
+    def set_front_fusion(self, fused: bool):
+        """Multifrontal factorisation: fronts of a single 64-pivot step
+        assembled and swept by one workgroup each in one launch per tree
+        height (True, the default) or through the level assembly, entry
+        scatter and step launches (False).  Same factorisation up to rounding;
+        applies from the next factorisation."""
+        check(lib.v2d_ctx_set_front_fusion(self._h, 1 if fused else 0))
+
     def linear_solver(self) -> str:
         k = C.c_int()
         check(lib.v2d_ctx_get_linear_solver(self._h, C.byref(k)))
@@ -177,8 +185,8 @@ class DeviceContext:
                 "entries", "positions")
         return dict(zip(keys, (int(v) for v in out)), flops=fl.value)
 
-    MF_GROUPS = ("assemble", "scatter", "small", "gj_begin", "gj_full", "gj_light", "gj_heavy", "gather", "fwd",
-                 "bwd")
+    MF_GROUPS = ("assemble", "scatter", "small", "gj_begin", "gj_full", "gj_light", "gj_heavy", "front1",
+                 "gather", "fwd", "bwd")
     # kernel of each group, as rocprofv3 names it ({T}: float or double);
     # "small" and "assemble" are several instances of one template
     MF_KERNELS = {"assemble": "k_mf_assemble<{T}, *>", "scatter": "k_mf_scatter<{T}>",
@@ -187,6 +195,7 @@ class DeviceContext:
                   "gj_full": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 0>",
                   "gj_light": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 1>",
                   "gj_heavy": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 2>",
+                  "front1": "k_mf_front1<{T}>",
                   "gather": "k_mf_fgather<{T}>", "fwd": "k_mf_fgemv<{T}>", "bwd": "k_mf_bwd<{T}>"}
 
     def profile_mf(self) -> dict:
