@@ -1408,33 +1408,40 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
 // no wait on another workgroup between the phases:
 //   1. the pivot rows [G W] (rows 0 .. 63, every column) and the pivot
 //      columns X (rows 64 .., columns 0 .. 63) are assembled into F as the
-//      level assembly does -- identity on the padding pivots, then the
-//      children's contribution blocks in list order -- and the Jacobian
-//      entries the front owns are added (the analysis checks that they all lie
-//      in these two panels);
-//   2. G is inverted in LDS (gj_tile) and stored as the pivot block;
+//      level assembly does -- identity on the padding pivots ns .. 63, then
+//      the children's contribution blocks in list order; only the rows and
+//      columns of the ns true pivots are gathered -- and the Jacobian entries
+//      the front owns are added (the analysis checks that they all lie in
+//      these two panels);
+//   2. G is inverted in LDS (gj_tile over its leading ns x ns block, the
+//      padding pivots keeping the identity) and stored as the pivot block;
 //   3. per 64-column tile: the pivot rows P = G^-1 W go to F and, transposed,
 //      to the job's row panel;
 //   4. per 64-row tile: X is staged in LDS and -X G^-1 stored as the pivot
 //      columns; per 64-column tile the trailing block C is gathered from the
 //      children straight into the matrix-core accumulators, takes - X P and is
 //      stored (zero in the padding columns nf .. ld-1).
-// The result is the assembly, scatter and Gauss-Jordan step launches' one,
+// KP (32 or 64, ns <= KP): the pivots the products run over.  The padding
+// pivots ns .. 63 have zero rows in W and zero columns in X, so with
+// ns <= 32 every product runs over 32 pivots, half the work.  The result is
+// the assembly, scatter and Gauss-Jordan step launches' one,
 // [G^-1, G^-1 W; -X G^-1, C - X G^-1 W], up to rounding.  256 threads; every
 // global access is bounded by the front (rows < nf, columns < ld) or by a
 // child's contribution block (clamped map entries).
-template <typename T>
+template <typename T, int KP>
 __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict__ fr, int f0,
                                                       const Job<T> *__restrict__ jobs, const int *__restrict__ ch,
                                                       const int *__restrict__ map, const int64_t *__restrict__ aoff,
                                                       const int *__restrict__ aseg, const int *__restrict__ avid,
                                                       const T *__restrict__ val, T *__restrict__ F)
 {
+    static_assert(KP == 32 || KP == 64, "32 or 64 pivots");
     using M = Mfma<T>;
     using C = typename M::C;
     using NV = typename NVec<T>::type;
     constexpr int VW = Vec8<T>::w;   // elements of one 16-byte access
     constexpr int LT = NB + 4;       // padded LDS row stride
+    constexpr int NBJ = KP / 32;     // 16-column blocks per wave of a 64 x KP product
     __shared__ T sD[NB * LT];        // the pivot block, then its inverse
     __shared__ T sX[NB * LT];        // one 64-row tile of the pivot columns
     __shared__ __align__(16) int smap[2][kFront1MaxNf];   // the children's maps, -1 past nf
@@ -1459,23 +1466,46 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
         smap[0][a] = a < nf ? m0 : -1;
         smap[1][a] = a < nf ? m1 : -1;
     }
-    __syncthreads();
     // ------------------------------------------------ 1. the panels ----
+    const int ld4 = ld >> 2, nq = (ns + 3) >> 2;   // 4-column items of a row; of the true pivot columns
     {
-        const int ld4 = ld >> 2;
-        const int n1 = NB * ld4, nitems = n1 + (nf - NB) * (NB / 4);
+        // no contribution: the padding pivot rows ns .. 63 (identity) and the
+        // padding pivot columns of X (zero)
+        const int nzq = max(NB / 4 - nq, 1);   // zero items of an X row (none when nq = 16: nz = na0)
+        const int na0 = (NB - ns) * ld4, nz = na0 + (nf - NB) * (NB / 4 - nq);
+        for (int e = t; e < nz; e += 256) {
+            const int e2 = e - na0, q = e / ld4;
+            const int r = e < na0 ? ns + q : NB + e2 / nzq;
+            const int c = e < na0 ? 4 * (e - q * ld4) : 4 * (nq + e2 % nzq);
+            T o[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) o[v] = c + v == r ? T(1) : T(0);
+            T *d = A + (int64_t)r * ld + c;
+            if constexpr (sizeof(T) == 4) {
+                f32x4 x;
+                x[0] = o[0]; x[1] = o[1]; x[2] = o[2]; x[3] = o[3];
+                *glb(reinterpret_cast<f32x4 *>(d)) = x;
+            } else {
+                f64x2 x0, x1;
+                x0[0] = o[0]; x0[1] = o[1]; x1[0] = o[2]; x1[1] = o[3];
+                glb(reinterpret_cast<f64x2 *>(d))[0] = x0;
+                glb(reinterpret_cast<f64x2 *>(d))[1] = x1;
+            }
+        }
+    }
+    __syncthreads();   // smap
+    {
+        // the true pivot rows (every column) and the true pivot columns of X
+        const int n1 = ns * ld4, nitems = n1 + (nf - NB) * nq;
         constexpr int U = 4;
         for (int e0 = 0; e0 < nitems; e0 += 256 * U) {
             int r[U], c[U];
-            T acc[U][4];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int e = min(e0 + u * 256 + t, nitems - 1);
                 const int q = e / ld4, e2 = e - n1;
-                r[u] = e < n1 ? q : NB + (e2 >> 4);
-                c[u] = e < n1 ? 4 * (e - q * ld4) : 4 * (e2 & 15);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[u][v] = (r[u] >= ns && r[u] < NB && c[u] + v == r[u]) ? T(1) : T(0);
+                r[u] = e < n1 ? q : NB + e2 / nq;
+                c[u] = e < n1 ? 4 * (e - q * ld4) : 4 * (e2 % nq);
             }
             // every map read and contribution load of the round before the first sum
             T x[2][U][4];
@@ -1493,13 +1523,16 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
                         x[k][u][v] = glb(cb[k])[max(ia, 0) * cld[k] + max(ibv[v], 0)];
                     }
                 }
+            T acc[U][4];
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
+            for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int u = 0; u < U; ++u)
+                for (int v = 0; v < 4; ++v) {
+                    acc[u][v] = T(0);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
+                    for (int k = 0; k < 2; ++k)
                         if (in[k][u][v]) acc[u][v] += x[k][u][v];
+                }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 T *d = A + (int64_t)r[u] * ld + c[u];
@@ -1517,11 +1550,38 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
         }
     }
     __syncthreads();
-    // the Jacobian entries the front owns (the sums of the entry scatter)
-    for (int64_t u = f.ao + t; u < f.ao + f.na; u += 256) {
-        T acc = T(0);
-        for (int e = aseg[u]; e < aseg[u + 1]; ++e) acc += val[avid[e]];
-        glb(F)[aoff[u]] += acc;
+    // the Jacobian entries the front owns (the sums of the entry scatter), U
+    // positions per thread and round with every load of the round issued
+    // before the first sum; every position holds at least one entry
+    {
+        constexpr int U = 4;
+        const int64_t uend = f.ao + f.na;
+        for (int64_t u0 = f.ao; u0 < uend; u0 += 256 * U) {
+            int64_t q[U];
+            int s0[U], s1[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t uc = min(u0 + u * 256 + t, uend - 1);
+                q[u] = aoff[uc];
+                s0[u] = aseg[uc];
+                s1[u] = aseg[uc + 1];
+            }
+            int v0[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v0[u] = avid[s0[u]];
+            T x[U], old[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                x[u] = T(0) + val[v0[u]];
+                old[u] = glb(F)[q[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                for (int e = s0[u] + 1; e < s1[u]; ++e) x[u] += val[avid[e]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (u0 + u * 256 + t < uend) glb(F)[q[u]] = old[u] + x[u];
+        }
     }
     __syncthreads();
     // ---------------------------------------------- 2. pivot inverse ----
@@ -1531,44 +1591,48 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
     }
     {
         T g[16];
-        gj_tile<T>(sD, LT, NB, g, J.dinv);   // sD holds the inverse on return
+        gj_tile<T>(sD, LT, ns, g, J.dinv);   // sD holds the whole inverse on return
 #pragma unroll
-        for (int q = 0; q < 16; ++q) glb(A)[(int64_t)(w * 16 + q) * ld + l] = g[q];
+        for (int q = 0; q < 16; ++q) glb(A)[(int64_t)(w * 16 + q) * ld + l] = sD[(w * 16 + q) * LT + l];
     }
     // ------------------------------------------------ 3. pivot rows ----
+    // P = G^-1 W over the KP leading pivot rows (the others are zero in W,
+    // hence in P, and already stored as such); the wave's block: the 32 rows
+    // rw3 .., 16 NBJ columns cw3 .. of the tile
+    const int rw3 = KP == 64 ? rw : 0, cw3 = KP == 64 ? cw : w * 16;
     for (int ct = 1; ct < nt; ++ct) {
         const int c0 = ct * NB;
-        T bw[2][16];
+        T bw[NBJ][KP / 4];
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const int cc = c0 + cw + bj * 16 + li;
+        for (int bj = 0; bj < NBJ; ++bj) {
+            const int cc = c0 + cw3 + bj * 16 + li;
             const int ccl = min(cc, ld - 1);
 #pragma unroll
-            for (int h = 0; h < 16; ++h) {
+            for (int h = 0; h < KP / 4; ++h) {
                 const T x = glb(A)[(int64_t)(4 * h + lk) * ld + ccl];
                 bw[bj][h] = cc < nf ? x : T(0);
             }
         }
         __syncthreads();   // every wave holds its part of W before the pivot rows overwrite it
-        C d[2][2];
+        C d[2][NBJ];
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) {
+            for (int bj = 0; bj < NBJ; ++bj) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) d[bi][bj][v] = T(0);
 #pragma unroll
-                for (int h = 0; h < 16; ++h)
-                    d[bi][bj] = M::opd(sD[(rw + bi * 16 + li) * LT + 4 * h + lk], bw[bj][h], d[bi][bj]);
+                for (int h = 0; h < KP / 4; ++h)
+                    d[bi][bj] = M::opd(sD[(rw3 + bi * 16 + li) * LT + 4 * h + lk], bw[bj][h], d[bi][bj]);
             }
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) {
-                const int cc = c0 + cw + bj * 16 + li;
+            for (int bj = 0; bj < NBJ; ++bj) {
+                const int cc = c0 + cw3 + bj * 16 + li;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int r = rw + bi * 16 + M::drow(lk, v);
+                    const int r = rw3 + bi * 16 + M::drow(lk, v);
                     if (cc < nf) glb(A)[(int64_t)r * ld + cc] = d[bi][bj][v];
                     glb(J.rowk)[(int64_t)cc * NB + r] = d[bi][bj][v];
                 }
@@ -1578,30 +1642,31 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
     // -------------------------------- 4. pivot columns and the update ----
     for (int it = 1; it < nt; ++it) {
         const int r0 = it * NB;
-        for (int e = t; e < NB * (NB / VW); e += 256) {
-            const int r = e / (NB / VW), c = (e % (NB / VW)) * VW;
+        for (int e = t; e < NB * (KP / VW); e += 256) {
+            const int r = e / (KP / VW), c = (e % (KP / VW)) * VW;
             const NV x = *glb(reinterpret_cast<const NV *>(A + (int64_t)min(r0 + r, nf - 1) * ld + c));
 #pragma unroll
             for (int q = 0; q < VW; ++q) sX[r * LT + c + q] = r0 + r < nf ? x[q] : T(0);
         }
         __syncthreads();
         if (r0 + rw < nf) {   // the wave's rows reach into the front
-            // pivot columns -X G^-1
+            // pivot columns -X G^-1 (columns KP .. 63 stay zero)
+            const int cy = KP == 64 ? cw : (w & 1) * 16;
 #pragma unroll
             for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
+                for (int bj = 0; bj < NBJ; ++bj) {
                     C y;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) y[v] = T(0);
 #pragma unroll
-                    for (int h = 0; h < 16; ++h)
+                    for (int h = 0; h < KP / 4; ++h)
                         y = M::opd(sX[(rw + bi * 16 + li) * LT + 4 * h + lk],
-                                   sD[(4 * h + lk) * LT + cw + bj * 16 + li], y);
+                                   sD[(4 * h + lk) * LT + cy + bj * 16 + li], y);
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int r = r0 + rw + bi * 16 + M::drow(lk, v);
-                        if (r < nf) glb(A)[(int64_t)r * ld + cw + bj * 16 + li] = -y[v];
+                        if (r < nf) glb(A)[(int64_t)r * ld + cy + bj * 16 + li] = -y[v];
                     }
                 }
             // trailing blocks
@@ -1624,9 +1689,9 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
                                 in[k][bi][bj][v] = ia >= 0 && ib >= 0;
                                 x[k][bi][bj][v] = glb(cb[k])[max(ia, 0) * cld[k] + max(ib, 0)];
                             }
-                NV pb[4][2][4 / VW];
+                NV pb[KP / 16][2][4 / VW];
 #pragma unroll
-                for (int h = 0; h < 4; ++h)
+                for (int h = 0; h < KP / 16; ++h)
 #pragma unroll
                     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
@@ -1648,7 +1713,7 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
                         }
                 // C - X P, k = 16 h + 4 lk + e
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
+                for (int h = 0; h < KP / 16; ++h) {
                     T av[2][4], bv[2][4];
 #pragma unroll
                     for (int bi = 0; bi < 2; ++bi) {
@@ -1856,18 +1921,22 @@ template int gj_table_step<float>(const Job<float> *, int, int, int, int, bool, 
 template int gj_table_step<double>(const Job<double> *, int, int, int, int, bool, int *, int *, int &, hipStream_t);
 
 template <typename T>
-int mf_front1(const MfFront *fr, int f0, int nfront, const Job<T> *jobs, const int *ch, const int *map,
+int mf_front1(const MfFront *fr, int f0, int nfront, int kp, const Job<T> *jobs, const int *ch, const int *map,
               const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s)
 {
     if (nfront <= 0) return 0;
-    hipLaunchKernelGGL(k_mf_front1<T>, dim3((unsigned)nfront), dim3(256), 0, s, fr, f0, jobs, ch, map, aoff, aseg,
-                       avid, val, F);
+    if (kp == 32)
+        hipLaunchKernelGGL((k_mf_front1<T, 32>), dim3((unsigned)nfront), dim3(256), 0, s, fr, f0, jobs, ch, map, aoff,
+                           aseg, avid, val, F);
+    else
+        hipLaunchKernelGGL((k_mf_front1<T, 64>), dim3((unsigned)nfront), dim3(256), 0, s, fr, f0, jobs, ch, map, aoff,
+                           aseg, avid, val, F);
     V2D_LAUNCH();
     return 0;
 }
-template int mf_front1<float>(const MfFront *, int, int, const Job<float> *, const int *, const int *,
+template int mf_front1<float>(const MfFront *, int, int, int, const Job<float> *, const int *, const int *,
                               const int64_t *, const int *, const int *, const float *, float *, hipStream_t);
-template int mf_front1<double>(const MfFront *, int, int, const Job<double> *, const int *, const int *,
+template int mf_front1<double>(const MfFront *, int, int, int, const Job<double> *, const int *, const int *,
                                const int64_t *, const int *, const int *, const double *, double *, hipStream_t);
 
 // ------------------------------------------------------------ factor ------

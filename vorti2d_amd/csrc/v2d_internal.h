@@ -373,10 +373,11 @@ struct MfFront {
 // Big fronts of one 64-pivot step with two children and nf <= kFront1MaxNf,
 // assembled (children, then the entries they own) and swept in one launch,
 // one workgroup per front: fronts f0 .. f0 + nfront - 1, jobs[q] the
-// Gauss-Jordan job (panels) of front f0 + q.
+// Gauss-Jordan job (panels) of front f0 + q; kp = 32 when every one of them
+// has ns <= 32 (the products then run over 32 pivots), else 64.
 constexpr int kFront1MaxNf = 512;
 template <typename T>
-int mf_front1(const MfFront *fr, int f0, int nfront, const Job<T> *jobs, const int *ch, const int *map,
+int mf_front1(const MfFront *fr, int f0, int nfront, int kp, const Job<T> *jobs, const int *ch, const int *map,
               const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s);
 
 struct MfLevel {
@@ -394,6 +395,7 @@ struct MfLevel {
     int small_ni[10];        // first small front swept with a register tile of NI 16-row blocks
     int one_step;            // 1: the big fronts fit mf_front1 (one 64-pivot step, two children,
                              //    nf <= kFront1MaxNf, every entry in the pivot rows or columns)
+    int one_kp;              // mf_front1's product width: 32 if every big front has ns <= 32, else 64
     int64_t pool;            // Gauss-Jordan panel elements of the big fronts
     int64_t job0;            // first job of the level in the job table
 };

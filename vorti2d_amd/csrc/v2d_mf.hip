@@ -522,6 +522,9 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
             }
         }
         L.one_step = L.fbig < L.f1 && L.nch2 && L.npmax_big == kNB && L.nfmax_big <= kFront1MaxNf;
+        L.one_kp = 32;
+        for (int f = L.fbig; f < L.f1; ++f)
+            if (P.fr[f].ns > 32) L.one_kp = 64;
         for (int f = L.fbig; f < L.f1 && L.one_step; ++f) {
             const MfFront &F = P.fr[f];
             for (int64_t u = F.ao; u < F.ao + F.na; ++u) {
@@ -1265,8 +1268,8 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
         if (nj <= 0 || L.npmax_big <= 0) continue;
         if (one) {
             MF_BEGIN();
-            if (mf_front1<T>(P.d_fr, L.fbig, nj, W.jobs + L.job0, P.d_ch, P.d_map, P.d_aoff, P.d_aseg, P.d_avid,
-                             val, W.F, s))
+            if (mf_front1<T>(P.d_fr, L.fbig, nj, L.one_kp, W.jobs + L.job0, P.d_ch, P.d_map, P.d_aoff, P.d_aseg,
+                             P.d_avid, val, W.F, s))
                 return -1;
             MF_END(kFront, one_fl, one_b);
             continue;
