@@ -354,7 +354,7 @@ def mf_kernels(run, prec, nsolves):
         per_solve = g["ms"] * (app if k in ("gather", "fwd", "bwd") else fac)
         e = {"kernel": g["kernel"], "launches": g["launches"], "ms": g["ms"],
              "avg_launch_us": g["ms"] / g["launches"] * 1e3, "ms_per_solve": per_solve}
-        if k in ("gj_full", "gj_light", "gj_heavy"):
+        if k in ("gj_full", "gj_light", "gj_heavy", "front1"):
             tf = g["flops"] / (g["ms"] * 1e-3) / 1e12
             e.update(bound="mfma", achieved=tf, peak=peak_tf, unit="TFLOP/s", frac=tf / peak_tf,
                      flops=g["flops"], bytes=g["bytes"])
