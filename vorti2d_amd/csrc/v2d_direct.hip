@@ -39,6 +39,11 @@ template <typename X> __device__ __forceinline__ gptr<X> glb(X *p) { return (gpt
 // copied through an address-space-qualified pointer).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// Two consecutive elements of T in one load (8 bytes in FP32, 16 in FP64).
+template <typename T> struct Pair2;
+template <> struct Pair2<float> { using type = f32x2; };
+template <> struct Pair2<double> { using type = f64x2; };
 template <typename T> struct NVec;
 template <> struct NVec<float> { using type = f32x4; };
 template <> struct NVec<double> { using type = f64x2; };
@@ -1403,9 +1408,11 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
 }
 
 // ------------------------------------------------ one-step big fronts ----
-// A multifrontal front with a single 64-pivot step (nsp = 64, two children,
-// nf <= kFront1MaxNf) assembled and swept by one workgroup, with no launch and
-// no wait on another workgroup between the phases:
+// Multifrontal fronts with a single 64-pivot step (nsp = 64, two children,
+// nf <= kFront1MaxNf) assembled and swept by two launches per tree height
+// instead of the level assembly, entry scatter and Gauss-Jordan step
+// launches, with no wait on another workgroup:
+// k_mf_front1_head, one workgroup per front:
 //   1. the pivot rows [G W] (rows 0 .. 63, every column) and the pivot
 //      columns X (rows 64 .., columns 0 .. 63) are assembled into F as the
 //      level assembly does -- identity on the padding pivots ns .. 63, then
@@ -1417,10 +1424,11 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
 //      padding pivots keeping the identity) and stored as the pivot block;
 //   3. per 64-column tile: the pivot rows P = G^-1 W go to F and, transposed,
 //      to the job's row panel;
-//   4. per 64-row tile: X is staged in LDS and -X G^-1 stored as the pivot
-//      columns; per 64-column tile the trailing block C is gathered from the
-//      children straight into the matrix-core accumulators, takes - X P and is
-//      stored (zero in the padding columns nf .. ld-1).
+// k_mf_front1_rows, one workgroup per front and 64-row tile below the pivots:
+//   4. X is staged in LDS and -X G^-1 stored as the pivot columns; per
+//      64-column tile the trailing block C is gathered from the children
+//      straight into the matrix-core accumulators, takes - X P and is stored
+//      (zero in the padding columns nf .. ld-1).
 // KP (32 or 64, ns <= KP): the pivots the products run over.  The padding
 // pivots ns .. 63 have zero rows in W and zero columns in X, so with
 // ns <= 32 every product runs over 32 pivots, half the work.  The result is
@@ -1429,7 +1437,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
 // global access is bounded by the front (rows < nf, columns < ld) or by a
 // child's contribution block (clamped map entries).
 template <typename T, int KP>
-__global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict__ fr, int f0,
+__global__ void __launch_bounds__(256) k_mf_front1_head(const MfFront *__restrict__ fr, int f0,
                                                       const Job<T> *__restrict__ jobs, const int *__restrict__ ch,
                                                       const int *__restrict__ map, const int64_t *__restrict__ aoff,
                                                       const int *__restrict__ aseg, const int *__restrict__ avid,
@@ -1439,11 +1447,11 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
     using M = Mfma<T>;
     using C = typename M::C;
     using NV = typename NVec<T>::type;
+    using P2 = typename Pair2<T>::type;
     constexpr int VW = Vec8<T>::w;   // elements of one 16-byte access
     constexpr int LT = NB + 4;       // padded LDS row stride
     constexpr int NBJ = KP / 32;     // 16-column blocks per wave of a 64 x KP product
     __shared__ T sD[NB * LT];        // the pivot block, then its inverse
-    __shared__ T sX[NB * LT];        // one 64-row tile of the pivot columns
     __shared__ __align__(16) int smap[2][kFront1MaxNf];   // the children's maps, -1 past nf
     const MfFront f = fr[f0 + blockIdx.x];
     const Job<T> J = jobs[blockIdx.x];
@@ -1451,7 +1459,7 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
     const int nf = f.nf, ld = f.ld, ns = f.ns;
     const int nt = (nf + NB - 1) / NB;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
-    const int rw = (w >> 1) * 32, cw = (w & 1) * 32;   // the wave's 32 x 32 quarter of a tile
+    const int rw = (w >> 1) * 32, cw = (w & 1) * 32;   // the wave's 32 x 32 quarter of a 64 x 64 product
     const T *cb[2];
     int cld[2];
 #pragma unroll
@@ -1507,7 +1515,10 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
                 r[u] = e < n1 ? q : NB + e2 / nq;
                 c[u] = e < n1 ? 4 * (e - q * ld4) : 4 * (e2 % nq);
             }
-            // every map read and contribution load of the round before the first sum
+            // every map read and contribution load of the round before the
+            // first sum; columns c, c + 1 (c even) are the two unknowns of one
+            // node, which a child holds at an even column and the next, so
+            // each pair is one load
             T x[2][U][4];
             bool in[2][U][4];
 #pragma unroll
@@ -1516,11 +1527,14 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
                 for (int u = 0; u < U; ++u) {
                     const int ia = smap[k][r[u]];
                     const int4 ib = *reinterpret_cast<const int4 *>(&smap[k][c[u]]);
-                    const int ibv[4] = {ib.x, ib.y, ib.z, ib.w};
+                    const int ibp[2] = {ib.x, ib.z};
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        in[k][u][v] = ia >= 0 && ibv[v] >= 0;
-                        x[k][u][v] = glb(cb[k])[max(ia, 0) * cld[k] + max(ibv[v], 0)];
+                    for (int p = 0; p < 2; ++p) {
+                        const bool ok = ia >= 0 && ibp[p] >= 0;
+                        const P2 y = *glb(reinterpret_cast<const P2 *>(cb[k] + max(ia, 0) * cld[k] + max(ibp[p], 0)));
+                        in[k][u][2 * p] = in[k][u][2 * p + 1] = ok;
+                        x[k][u][2 * p] = y[0];
+                        x[k][u][2 * p + 1] = y[1];
                     }
                 }
             T acc[U][4];
@@ -1638,127 +1652,188 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1(const MfFront *__restrict_
                 }
             }
     }
-    __syncthreads();
-    // -------------------------------- 4. pivot columns and the update ----
-    for (int it = 1; it < nt; ++it) {
-        const int r0 = it * NB;
-        for (int e = t; e < NB * (KP / VW); e += 256) {
-            const int r = e / (KP / VW), c = (e % (KP / VW)) * VW;
-            const NV x = *glb(reinterpret_cast<const NV *>(A + (int64_t)min(r0 + r, nf - 1) * ld + c));
+}
+
+template <typename T, int KP>
+__global__ void __launch_bounds__(256, 2) k_mf_front1_rows(const MfFront *__restrict__ fr, int f0, int nfront,
+                                                           const Job<T> *__restrict__ jobs,
+                                                           const int *__restrict__ ch, const int *__restrict__ map,
+                                                           T *__restrict__ F)
+{
+    using M = Mfma<T>;
+    using C = typename M::C;
+    using NV = typename NVec<T>::type;
+    constexpr int VW = Vec8<T>::w;   // elements of one 16-byte access
+    constexpr int LT = NB + 4;       // padded LDS row stride
+    constexpr int NBJ = KP / 32;     // 16-column blocks per wave of a 64 x KP product
+    __shared__ T sD[NB * LT];        // the inverse pivot block (leading KP x KP)
+    __shared__ T sX[NB * LT];        // the tile's pivot columns (leading KP)
+    __shared__ __align__(16) int smap[2][kFront1MaxNf];   // the children's maps, -1 past nf
+    // row tile major: with a multiple of 8 fronts every row tile of a front
+    // runs on its head's die (workgroups go round-robin over the 8 dies), so
+    // the pivot rows are read from that die's L2
+    const int fl = blockIdx.x % nfront, it = 1 + blockIdx.x / nfront;
+    const MfFront f = fr[f0 + fl];
+    const Job<T> J = jobs[fl];
+    T *__restrict__ A = F + f.fo;
+    const int nf = f.nf, ld = f.ld;
+    const int nt = (nf + NB - 1) / NB;
+    if (it >= nt) return;   // the front has fewer row tiles than the height's largest
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+    const int rw = (w >> 1) * 32, cw = (w & 1) * 32;   // the wave's 32 x 32 quarter of a tile
+    const int r0 = it * NB;
+    const T *cb[2];
+    int cld[2];
 #pragma unroll
-            for (int q = 0; q < VW; ++q) sX[r * LT + c + q] = r0 + r < nf ? x[q] : T(0);
+    for (int k = 0; k < 2; ++k) {
+        const MfFront c = fr[ch[f.co + k]];
+        cb[k] = F + c.fo + (int64_t)c.nsp * c.ld + c.nsp;
+        cld[k] = c.ld;
+    }
+    for (int a = t; a < kFront1MaxNf; a += 256) {
+        const int q = min(a, nf - 1);
+        const int m0 = map[f.mo + q], m1 = map[f.mo + nf + q];
+        smap[0][a] = a < nf ? m0 : -1;
+        smap[1][a] = a < nf ? m1 : -1;
+    }
+    for (int e = t; e < KP * (KP / VW); e += 256) {
+        const int r = e / (KP / VW), c = (e % (KP / VW)) * VW;
+        const NV x = *glb(reinterpret_cast<const NV *>(A + (int64_t)r * ld + c));
+#pragma unroll
+        for (int q = 0; q < VW; ++q) sD[r * LT + c + q] = x[q];
+    }
+    for (int e = t; e < NB * (KP / VW); e += 256) {
+        const int r = e / (KP / VW), c = (e % (KP / VW)) * VW;
+        const NV x = *glb(reinterpret_cast<const NV *>(A + (int64_t)min(r0 + r, nf - 1) * ld + c));
+#pragma unroll
+        for (int q = 0; q < VW; ++q) sX[r * LT + c + q] = r0 + r < nf ? x[q] : T(0);
+    }
+    __syncthreads();
+    if (r0 + rw >= nf) return;   // the wave's rows lie past the front (no barrier follows)
+    // pivot columns -X G^-1 (columns KP .. 63 stay zero)
+    const int cy = KP == 64 ? cw : (w & 1) * 16;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < NBJ; ++bj) {
+            C y;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) y[v] = T(0);
+#pragma unroll
+            for (int h = 0; h < KP / 4; ++h)
+                y = M::opd(sX[(rw + bi * 16 + li) * LT + 4 * h + lk],
+                           sD[(4 * h + lk) * LT + cy + bj * 16 + li], y);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = r0 + rw + bi * 16 + M::drow(lk, v);
+                if (r < nf) glb(A)[(int64_t)r * ld + cy + bj * 16 + li] = -y[v];
+            }
         }
-        __syncthreads();
-        if (r0 + rw < nf) {   // the wave's rows reach into the front
-            // pivot columns -X G^-1 (columns KP .. 63 stay zero)
-            const int cy = KP == 64 ? cw : (w & 1) * 16;
+    // trailing blocks
+    for (int ct = 1; ct < nt; ++ct) {
+        const int c0 = ct * NB;
+        if (c0 + cw >= ld) break;   // the wave's columns lie past the row
+        // every gather of C and every load of the row panel before the first
+        // sum; FP32 lanes hold four consecutive columns of a row, two node
+        // pairs (even column and the next, at an even child column and the
+        // next), one load each
+        T x[2][2][2][4];
+        bool in[2][2][2][4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
 #pragma unroll
             for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < NBJ; ++bj) {
-                    C y;
+                for (int bj = 0; bj < 2; ++bj) {
+                    if constexpr (M::kSwap) {
+                        const int ia = smap[k][r0 + rw + bi * 16 + li];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) y[v] = T(0);
-#pragma unroll
-                    for (int h = 0; h < KP / 4; ++h)
-                        y = M::opd(sX[(rw + bi * 16 + li) * LT + 4 * h + lk],
-                                   sD[(4 * h + lk) * LT + cy + bj * 16 + li], y);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int r = r0 + rw + bi * 16 + M::drow(lk, v);
-                        if (r < nf) glb(A)[(int64_t)r * ld + cy + bj * 16 + li] = -y[v];
-                    }
-                }
-            // trailing blocks
-            for (int ct = 1; ct < nt; ++ct) {
-                const int c0 = ct * NB;
-                if (c0 + cw >= ld) break;   // the wave's columns lie past the row
-                // every gather of C and every load of the row panel before the first sum
-                T x[2][2][2][4];
-                bool in[2][2][2][4];
-#pragma unroll
-                for (int k = 0; k < 2; ++k)
-#pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) {
-                                const int ia = smap[k][r0 + rw + bi * 16 + M::row(li, lk, v)];
-                                const int ib = smap[k][c0 + cw + bj * 16 + M::col(li, lk, v)];
-                                in[k][bi][bj][v] = ia >= 0 && ib >= 0;
-                                x[k][bi][bj][v] = glb(cb[k])[max(ia, 0) * cld[k] + max(ib, 0)];
-                            }
-                NV pb[KP / 16][2][4 / VW];
-#pragma unroll
-                for (int h = 0; h < KP / 16; ++h)
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                        for (int u = 0; u < 4 / VW; ++u)
-                            pb[h][bj][u] = *glb(reinterpret_cast<const NV *>(
-                                J.rowk + (int64_t)(c0 + cw + bj * 16 + li) * NB + 16 * h + 4 * lk + u * VW));
-                C acc[2][2];
-#pragma unroll
-                for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj)
+                        for (int p = 0; p < 2; ++p) {
+                            const int ib = smap[k][c0 + cw + bj * 16 + 4 * lk + 2 * p];
+                            const bool ok = ia >= 0 && ib >= 0;
+                            const f32x2 y =
+                                *glb(reinterpret_cast<const f32x2 *>(cb[k] + max(ia, 0) * cld[k] + max(ib, 0)));
+                            in[k][bi][bj][2 * p] = in[k][bi][bj][2 * p + 1] = ok;
+                            x[k][bi][bj][2 * p] = y[0];
+                            x[k][bi][bj][2 * p + 1] = y[1];
+                        }
+                    } else {
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            T s = T(0);
-#pragma unroll
-                            for (int k = 0; k < 2; ++k)
-                                if (in[k][bi][bj][v]) s += x[k][bi][bj][v];
-                            acc[bi][bj][v] = s;
+                            const int ia = smap[k][r0 + rw + bi * 16 + M::row(li, lk, v)];
+                            const int ib = smap[k][c0 + cw + bj * 16 + M::col(li, lk, v)];
+                            in[k][bi][bj][v] = ia >= 0 && ib >= 0;
+                            x[k][bi][bj][v] = glb(cb[k])[max(ia, 0) * cld[k] + max(ib, 0)];
                         }
-                // C - X P, k = 16 h + 4 lk + e
-#pragma unroll
-                for (int h = 0; h < KP / 16; ++h) {
-                    T av[2][4], bv[2][4];
-#pragma unroll
-                    for (int bi = 0; bi < 2; ++bi) {
-                        const T *p = &sX[(rw + bi * 16 + li) * LT + 16 * h + 4 * lk];
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) av[bi][e] = -p[e];
                     }
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                        for (int u = 0; u < 4 / VW; ++u)
-#pragma unroll
-                            for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = pb[h][bj][u][e];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-#pragma unroll
-                        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
                 }
+        NV pb[KP / 16][2][4 / VW];
+#pragma unroll
+        for (int h = 0; h < KP / 16; ++h)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int u = 0; u < 4 / VW; ++u)
+                    pb[h][bj][u] = *glb(reinterpret_cast<const NV *>(
+                        J.rowk + (int64_t)(c0 + cw + bj * 16 + li) * NB + 16 * h + 4 * lk + u * VW));
+        C acc[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    T s = T(0);
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                        if (in[k][bi][bj][v]) s += x[k][bi][bj][v];
+                    acc[bi][bj][v] = s;
+                }
+        // C - X P, k = 16 h + 4 lk + e
+#pragma unroll
+        for (int h = 0; h < KP / 16; ++h) {
+            T av[2][4], bv[2][4];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi) {
+                const T *p = &sX[(rw + bi * 16 + li) * LT + 16 * h + 4 * lk];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) av[bi][e] = -p[e];
+            }
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int u = 0; u < 4 / VW; ++u)
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) bv[bj][u * VW + e] = pb[h][bj][u][e];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
 #pragma unroll
                 for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        if constexpr (M::kSwap) {
-                            // one row, four consecutive columns per lane
-                            const int r = r0 + rw + bi * 16 + li, c = c0 + cw + bj * 16 + 4 * lk;
-                            if (r < nf && c < ld) {
-                                f32x4 o;
-#pragma unroll
-                                for (int v = 0; v < 4; ++v) o[v] = c + v < nf ? acc[bi][bj][v] : T(0);
-                                *glb(reinterpret_cast<f32x4 *>(A + (int64_t)r * ld + c)) = o;
-                            }
-                        } else {
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) {
-                                const int r = r0 + rw + bi * 16 + M::row(li, lk, v);
-                                const int c = c0 + cw + bj * 16 + M::col(li, lk, v);
-                                if (r < nf && c < ld) glb(A)[(int64_t)r * ld + c] = c < nf ? acc[bi][bj][v] : T(0);
-                            }
-                        }
-                    }
-            }
+                    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
         }
-        __syncthreads();   // sX is restaged by the next row tile
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                if constexpr (M::kSwap) {
+                    // one row, four consecutive columns per lane
+                    const int r = r0 + rw + bi * 16 + li, c = c0 + cw + bj * 16 + 4 * lk;
+                    if (r < nf && c < ld) {
+                        f32x4 o;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) o[v] = c + v < nf ? acc[bi][bj][v] : T(0);
+                        *glb(reinterpret_cast<f32x4 *>(A + (int64_t)r * ld + c)) = o;
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = r0 + rw + bi * 16 + M::row(li, lk, v);
+                        const int c = c0 + cw + bj * 16 + M::col(li, lk, v);
+                        if (r < nf && c < ld) glb(A)[(int64_t)r * ld + c] = c < nf ? acc[bi][bj][v] : T(0);
+                    }
+                }
+            }
     }
 }
 
@@ -1921,22 +1996,27 @@ template int gj_table_step<float>(const Job<float> *, int, int, int, int, bool, 
 template int gj_table_step<double>(const Job<double> *, int, int, int, int, bool, int *, int *, int &, hipStream_t);
 
 template <typename T>
-int mf_front1(const MfFront *fr, int f0, int nfront, int kp, const Job<T> *jobs, const int *ch, const int *map,
-              const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s)
+int mf_front1(const MfFront *fr, int f0, int nfront, int nfmax, int kp, const Job<T> *jobs, const int *ch,
+              const int *map, const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s)
 {
     if (nfront <= 0) return 0;
-    if (kp == 32)
-        hipLaunchKernelGGL((k_mf_front1<T, 32>), dim3((unsigned)nfront), dim3(256), 0, s, fr, f0, jobs, ch, map, aoff,
-                           aseg, avid, val, F);
-    else
-        hipLaunchKernelGGL((k_mf_front1<T, 64>), dim3((unsigned)nfront), dim3(256), 0, s, fr, f0, jobs, ch, map, aoff,
-                           aseg, avid, val, F);
+    const int nrt = (nfmax + NB - 1) / NB - 1;   // row tiles below the pivots of the largest front
+    const dim3 gh((unsigned)nfront), gr((unsigned)((int64_t)nfront * nrt));
+    if (kp == 32) {
+        hipLaunchKernelGGL((k_mf_front1_head<T, 32>), gh, dim3(256), 0, s, fr, f0, jobs, ch, map, aoff, aseg, avid,
+                           val, F);
+        hipLaunchKernelGGL((k_mf_front1_rows<T, 32>), gr, dim3(256), 0, s, fr, f0, nfront, jobs, ch, map, F);
+    } else {
+        hipLaunchKernelGGL((k_mf_front1_head<T, 64>), gh, dim3(256), 0, s, fr, f0, jobs, ch, map, aoff, aseg, avid,
+                           val, F);
+        hipLaunchKernelGGL((k_mf_front1_rows<T, 64>), gr, dim3(256), 0, s, fr, f0, nfront, jobs, ch, map, F);
+    }
     V2D_LAUNCH();
     return 0;
 }
-template int mf_front1<float>(const MfFront *, int, int, int, const Job<float> *, const int *, const int *,
+template int mf_front1<float>(const MfFront *, int, int, int, int, const Job<float> *, const int *, const int *,
                               const int64_t *, const int *, const int *, const float *, float *, hipStream_t);
-template int mf_front1<double>(const MfFront *, int, int, int, const Job<double> *, const int *, const int *,
+template int mf_front1<double>(const MfFront *, int, int, int, int, const Job<double> *, const int *, const int *,
                                const int64_t *, const int *, const int *, const double *, double *, hipStream_t);
 
 // ------------------------------------------------------------ factor ------

@@ -371,14 +371,15 @@ struct MfFront {
 };
 
 // Big fronts of one 64-pivot step with two children and nf <= kFront1MaxNf,
-// assembled (children, then the entries they own) and swept in one launch,
-// one workgroup per front: fronts f0 .. f0 + nfront - 1, jobs[q] the
-// Gauss-Jordan job (panels) of front f0 + q; kp = 32 when every one of them
-// has ns <= 32 (the products then run over 32 pivots), else 64.
+// assembled (children, then the entries they own) and swept in two launches:
+// one workgroup per front for the pivot block and pivot rows, one per front
+// and 64-row tile for the rest.  Fronts f0 .. f0 + nfront - 1 (nf <= nfmax),
+// jobs[q] the Gauss-Jordan job (panels) of front f0 + q; kp = 32 when every
+// one of them has ns <= 32 (the products then run over 32 pivots), else 64.
 constexpr int kFront1MaxNf = 512;
 template <typename T>
-int mf_front1(const MfFront *fr, int f0, int nfront, int kp, const Job<T> *jobs, const int *ch, const int *map,
-              const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s);
+int mf_front1(const MfFront *fr, int f0, int nfront, int nfmax, int kp, const Job<T> *jobs, const int *ch,
+              const int *map, const int64_t *aoff, const int *aseg, const int *avid, const T *val, T *F, hipStream_t s);
 
 struct MfLevel {
     int f0, f1, fbig;        // fronts [f0, f1); [fbig, f1) are big

@@ -672,6 +672,7 @@ __global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__
 {
     constexpr int U = 4;
     constexpr int KC = NC > 0 ? NC : 1;
+    using P2 = T __attribute__((ext_vector_type(2)));
     const int2 fb = blk[blockIdx.x];
     const MfFront f = fr[fb.x];
     const int nf = f.nf, ld4 = f.ld >> 2;
@@ -721,6 +722,9 @@ __global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__
                     ib[kk][u][3] = hi.y;
                 }
             }
+            // columns c, c + 1 (c even) are the two unknowns of one node, which
+            // a child holds at an even column and the next: one load per pair,
+            // at a clamped address (absent pairs dropped by the sum's select)
             T x[KC][U][4];
             bool in[KC][U][4];
 #pragma unroll
@@ -728,9 +732,13 @@ __global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__
 #pragma unroll
                 for (int u = 0; u < U; ++u)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        in[kk][u][v] = ia[kk][u] >= 0 && ib[kk][u][v] >= 0;
-                        x[kk][u][v] = in[kk][u][v] ? cb[kk][ia[kk][u] * cld[kk] + ib[kk][u][v]] : T(0);
+                    for (int p = 0; p < 2; ++p) {
+                        const bool okp = ia[kk][u] >= 0 && ib[kk][u][2 * p] >= 0;
+                        const P2 y = *reinterpret_cast<const P2 *>(cb[kk] + max(ia[kk][u], 0) * cld[kk] +
+                                                                   max(ib[kk][u][2 * p], 0));
+                        in[kk][u][2 * p] = in[kk][u][2 * p + 1] = okp;
+                        x[kk][u][2 * p] = y[0];
+                        x[kk][u][2 * p + 1] = y[1];
                     }
 #pragma unroll
             for (int kk = 0; kk < KC; ++kk)
@@ -1268,8 +1276,8 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
         if (nj <= 0 || L.npmax_big <= 0) continue;
         if (one) {
             MF_BEGIN();
-            if (mf_front1<T>(P.d_fr, L.fbig, nj, L.one_kp, W.jobs + L.job0, P.d_ch, P.d_map, P.d_aoff, P.d_aseg,
-                             P.d_avid, val, W.F, s))
+            if (mf_front1<T>(P.d_fr, L.fbig, nj, L.nfmax_big, L.one_kp, W.jobs + L.job0, P.d_ch, P.d_map, P.d_aoff,
+                             P.d_aseg, P.d_avid, val, W.F, s))
                 return -1;
             MF_END(kFront, one_fl, one_b);
             continue;
