@@ -1531,7 +1531,7 @@ __global__ void __launch_bounds__(256) k_mf_front1_head(const MfFront *__restric
 #pragma unroll
                     for (int p = 0; p < 2; ++p) {
                         const bool ok = ia >= 0 && ibp[p] >= 0;
-                        const P2 y = *glb(reinterpret_cast<const P2 *>(cb[k] + max(ia, 0) * cld[k] + max(ibp[p], 0)));
+                        const P2 y = ok ? *glb(reinterpret_cast<const P2 *>(cb[k] + ia * cld[k] + ibp[p])) : P2{T(0), T(0)};
                         in[k][u][2 * p] = in[k][u][2 * p + 1] = ok;
                         x[k][u][2 * p] = y[0];
                         x[k][u][2 * p + 1] = y[1];
@@ -1752,7 +1752,7 @@ __global__ void __launch_bounds__(256, 2) k_mf_front1_rows(const MfFront *__rest
                             const int ib = smap[k][c0 + cw + bj * 16 + 4 * lk + 2 * p];
                             const bool ok = ia >= 0 && ib >= 0;
                             const f32x2 y =
-                                *glb(reinterpret_cast<const f32x2 *>(cb[k] + max(ia, 0) * cld[k] + max(ib, 0)));
+                                ok ? *glb(reinterpret_cast<const f32x2 *>(cb[k] + ia * cld[k] + ib)) : f32x2{0.f, 0.f};
                             in[k][bi][bj][2 * p] = in[k][bi][bj][2 * p + 1] = ok;
                             x[k][bi][bj][2 * p] = y[0];
                             x[k][bi][bj][2 * p + 1] = y[1];

@@ -723,8 +723,8 @@ __global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__
                 }
             }
             // columns c, c + 1 (c even) are the two unknowns of one node, which
-            // a child holds at an even column and the next: one load per pair,
-            // at a clamped address (absent pairs dropped by the sum's select)
+            // a child holds at an even column and the next: one load per pair
+            // the child holds (the lanes of absent pairs load nothing)
             T x[KC][U][4];
             bool in[KC][U][4];
 #pragma unroll
@@ -734,8 +734,9 @@ __global__ void __launch_bounds__(256) k_mf_assemble(const MfFront *__restrict__
 #pragma unroll
                     for (int p = 0; p < 2; ++p) {
                         const bool okp = ia[kk][u] >= 0 && ib[kk][u][2 * p] >= 0;
-                        const P2 y = *reinterpret_cast<const P2 *>(cb[kk] + max(ia[kk][u], 0) * cld[kk] +
-                                                                   max(ib[kk][u][2 * p], 0));
+                        const P2 y = okp ? *reinterpret_cast<const P2 *>(cb[kk] + ia[kk][u] * cld[kk] +
+                                                                         ib[kk][u][2 * p])
+                                         : P2{T(0), T(0)};
                         in[kk][u][2 * p] = in[kk][u][2 * p + 1] = okp;
                         x[kk][u][2 * p] = y[0];
                         x[kk][u][2 * p + 1] = y[1];
@@ -814,15 +815,19 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
         // contribution load (clamped addresses, the absent ones dropped by a
         // select: no load behind a branch), before the first sum.
         constexpr int U = 4;
-        for (int e0 = 0; e0 < ne; e0 += 256 * U) {
+        using P2 = T __attribute__((ext_vector_type(2)));
+        const int np2 = ne >> 1;   // element pairs (2 q, 2 q + 1): nf is even, a pair lies in one row
+        for (int e0 = 0; e0 < np2; e0 += 256 * U) {
             int er[U], ec[U];
-            T acc[U];
+            T acc[U][2];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int e = min(e0 + u * 256 + (int)threadIdx.x, ne - 1);
+                const int e = 2 * min(e0 + u * 256 + (int)threadIdx.x, np2 - 1);
                 er[u] = r0 + e / nf;
                 ec[u] = e % nf;
-                acc[u] = (er[u] >= ns && er[u] < f.nsp && ec[u] == er[u]) ? T(1) : T(0);
+#pragma unroll
+                for (int v = 0; v < 2; ++v)
+                    acc[u][v] = (er[u] >= ns && er[u] < f.nsp && ec[u] + v == er[u]) ? T(1) : T(0);
             }
             for (int k = 0; k < f.nch; ++k) {
                 const int *m = map + f.mo + (int64_t)k * nf;
@@ -834,17 +839,27 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
                     ia[u] = m[er[u]];
                     ib[u] = m[ec[u]];
                 }
-                T x[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) x[u] = cb[max(ia[u], 0) * c.ld + max(ib[u], 0)];
+                // columns ec, ec + 1 are the two unknowns of one node, which the
+                // child holds at an even column and the next: one load per pair
+                P2 x[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if (ia[u] >= 0 && ib[u] >= 0) acc[u] += x[u];
+                    x[u] = (ia[u] >= 0 && ib[u] >= 0) ? *reinterpret_cast<const P2 *>(cb + ia[u] * c.ld + ib[u])
+                                                      : P2{T(0), T(0)};
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (ia[u] >= 0 && ib[u] >= 0) {
+                        acc[u][0] += x[u][0];
+                        acc[u][1] += x[u][1];
+                    }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int e = e0 + u * 256 + (int)threadIdx.x;
-                if (e < ne) sF[e] = acc[u];
+                if (e < np2) {
+                    sF[2 * e] = acc[u][0];
+                    sF[2 * e + 1] = acc[u][1];
+                }
             }
         }
         __syncthreads();
