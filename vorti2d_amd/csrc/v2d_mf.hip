@@ -844,8 +844,7 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
                 P2 x[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    x[u] = (ia[u] >= 0 && ib[u] >= 0) ? *reinterpret_cast<const P2 *>(cb + ia[u] * c.ld + ib[u])
-                                                      : P2{T(0), T(0)};
+                    x[u] = *reinterpret_cast<const P2 *>(cb + max(ia[u], 0) * c.ld + max(ib[u], 0));
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     if (ia[u] >= 0 && ib[u] >= 0) {
@@ -863,7 +862,9 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
             }
         }
         __syncthreads();
-        // the Jacobian entries the front owns, U positions per thread and round
+        // the Jacobian entries the front owns, U positions per thread and
+        // round, every load of the round issued before the first sum (every
+        // position holds at least one entry)
         for (int64_t u0 = f.ao; u0 < f.ao + f.na; u0 += 256 * U) {
             int64_t q[U];
             int s0[U], s1[U];
@@ -874,13 +875,20 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
                 s0[u] = aseg[uu];
                 s1[u] = aseg[uu + 1];
             }
+            int v0[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v0[u] = avid[s0[u]];
+            T acc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[u] = T(0) + val[v0[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                for (int e = s0[u] + 1; e < s1[u]; ++e) acc[u] += val[avid[e]];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int r = (int)(q[u] / f.ld), c = (int)(q[u] % f.ld);
-                if (u0 + u * 256 + (int64_t)threadIdx.x >= f.ao + f.na || r < r0 || r >= r1) continue;
-                T acc = T(0);
-                for (int e = s0[u]; e < s1[u]; ++e) acc += val[avid[e]];
-                sF[(r - r0) * nf + c] += acc;
+                if (u0 + u * 256 + (int64_t)threadIdx.x < f.ao + f.na && r >= r0 && r < r1)
+                    sF[(r - r0) * nf + c] += acc[u];
             }
         }
         __syncthreads();
