@@ -1014,7 +1014,7 @@ template <> struct Mfma<double> {
 constexpr int kSpinLimit = 1 << 21;
 
 template <typename T, typename BT = Batch<T>, int MODE = kStepFull>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(BT B, int k0, int epoch,
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ? 2 : 4) k_gj_step_mfma_b(BT B, int k0, int epoch,
                                                         int *__restrict__ flags, int *__restrict__ err)
 {
     using M = Mfma<T>;
@@ -1023,7 +1023,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
     using NV = typename NVec<T>::type;
     constexpr int VW = Vec8<T>::w;
     constexpr int LT = NB + 4;    // padded LDS row stride
-    __shared__ T sT[NB * LT];     // next pivot block, or a tile stored transposed
+    // next pivot block, or a tile stored transposed; in a heavy step first
+    // the four panel blocks of the tile's two products (FP32 job tables)
+    constexpr bool kStageP = !std::is_same<BT, Batch<T>>::value && sizeof(T) == 4 && MODE == kStepHeavy;
+    __shared__ __align__(16) T sT[kStageP ? 4 * NB * NB : NB * LT];
     constexpr bool kTable = !std::is_same<BT, Batch<T>>::value;
     static_assert(kTable || MODE == kStepFull, "the level batches take full steps only");
     int z, lin;
@@ -1190,8 +1193,82 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
                     old[bi][bj][3] = in ? x[3] : T(0);
                 }
         } else if (useP) {
-            product(J.colk_x, J.rowk_x, false, true);
-            product(J.colk, J.rowk, tail, false);
+            if constexpr (kStageP) {
+                // The four panel blocks (previous step's column and row panels,
+                // then this step's) go to LDS by direct 16-byte loads, one
+                // round for all of them, each block read once per workgroup:
+                // chunk kc (4 pivots) of block row r sits at chunk kc ^ (r & 15)
+                // of the row, so the sixteen rows a matrix-core operand read
+                // spans fall in distinct banks.  A wave's load fills 64
+                // consecutive chunks (lane l the l-th), i.e. 4 block rows.
+                const T *src[4] = {J.colk_x, J.rowk_x, J.colk, J.rowk};
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int q0 = (w * 4 + j) * 64;
+                        const int r = (q0 + l) >> 4, kc = (l & 15) ^ (r & 15);
+                        // column panels: rows past n read row n - 1 (results dropped);
+                        // row panels hold every row of the 64-rounded front
+                        const int gr = (b & 1) ? c0 + r : min(i0 + r, n - 1);
+                        __builtin_amdgcn_global_load_lds(
+                            (const void __attribute__((address_space(1))) *)(src[b] + (int64_t)gr * NB + 4 * kc),
+                            (void __attribute__((address_space(3))) *)(sT + (b * 1024 + q0) * 4), 16, 0, 0);
+                    }
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                        const f32x4 x = *glb(reinterpret_cast<const f32x4 *>(
+                            A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
+                        const bool in = !oldZero && r < n && c < n;
+                        old[bi][bj][0] = in ? x[0] : T(0);
+                        old[bi][bj][1] = in ? x[1] : T(0);
+                        old[bi][bj][2] = in ? x[2] : T(0);
+                        old[bi][bj][3] = in ? x[3] : T(0);
+                    }
+                __syncthreads();   // the panel blocks have landed
+                auto lds_product = [&](int ba, int bb, bool msk) {
+                    const f32x4 *sa = reinterpret_cast<const f32x4 *>(sT) + ba * 1024;
+                    const f32x4 *sb = reinterpret_cast<const f32x4 *>(sT) + bb * 1024;
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        T av[2][4], bv[2][4];
+#pragma unroll
+                        for (int bi = 0; bi < 2; ++bi) {
+                            const int r = (w >> 1) * 32 + bi * 16 + li;
+                            const f32x4 x = sa[r * 16 + ((4 * h + lk) ^ (r & 15))];
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) av[bi][e] = -x[e];
+                        }
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) {
+                            const int r = (w & 1) * 32 + bj * 16 + li;
+                            const f32x4 x = sb[r * 16 + ((4 * h + lk) ^ (r & 15))];
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) bv[bj][e] = x[e];
+                        }
+                        if (msk) {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+#pragma unroll
+                            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                                for (int bj = 0; bj < 2; ++bj)
+                                    acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+                    }
+                };
+                lds_product(0, 1, false);
+                lds_product(2, 3, tail);
+            } else {
+                product(J.colk_x, J.rowk_x, false, true);
+                product(J.colk, J.rowk, tail, false);
+            }
         } else {
             product(J.colk, J.rowk, tail, true);
         }
@@ -1300,6 +1377,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) k_gj_step_mfma_b(
         return;
     }
     const int kb2 = min(NB, np - k1);
+    if constexpr (kStageP) __syncthreads();   // every wave is past its reads of the staged panels
     if (is_piv) {
         // ------------------------------------- next pivot block inverse ----
 #pragma unroll
