@@ -1026,7 +1026,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
     // next pivot block, or a tile stored transposed; in a heavy step first
     // the four panel blocks of the tile's two products (FP32 job tables)
     constexpr bool kStageP = !std::is_same<BT, Batch<T>>::value && sizeof(T) == 4 && MODE == kStepHeavy;
-    __shared__ __align__(16) T sT[kStageP ? 4 * NB * NB : NB * LT];
+    constexpr bool kStage1 = !std::is_same<BT, Batch<T>>::value && sizeof(T) == 4 &&
+                             (MODE == kStepLight || MODE == kStepFull);
+    __shared__ __align__(16) T sT[kStageP ? 4 * NB * NB : kStage1 ? 2 * NB * NB : NB * LT];
     constexpr bool kTable = !std::is_same<BT, Batch<T>>::value;
     static_assert(kTable || MODE == kStepFull, "the level batches take full steps only");
     int z, lin;
@@ -1269,6 +1271,64 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
                 product(J.colk_x, J.rowk_x, false, true);
                 product(J.colk, J.rowk, tail, false);
             }
+        } else if constexpr (kStage1) {
+            const T *src[2] = {J.colk, J.rowk};
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int q0 = (w * 4 + j) * 64;
+                    const int r = (q0 + l) >> 4, kc = (l & 15) ^ (r & 15);
+                    const int gr = (b & 1) ? c0 + r : min(i0 + r, n - 1);
+                    __builtin_amdgcn_global_load_lds(
+                        (const void __attribute__((address_space(1))) *)(src[b] + (int64_t)gr * NB + 4 * kc),
+                        (void __attribute__((address_space(3))) *)(sT + (b * 1024 + q0) * 4), 16, 0, 0);
+                }
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
+                    const f32x4 x = *glb(reinterpret_cast<const f32x4 *>(
+                        A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
+                    const bool in = !oldZero && r < n && c < n;
+                    old[bi][bj][0] = in ? x[0] : T(0);
+                    old[bi][bj][1] = in ? x[1] : T(0);
+                    old[bi][bj][2] = in ? x[2] : T(0);
+                    old[bi][bj][3] = in ? x[3] : T(0);
+                }
+            __syncthreads();   // the panel blocks have landed
+            const f32x4 *sa = reinterpret_cast<const f32x4 *>(sT);
+            const f32x4 *sb = reinterpret_cast<const f32x4 *>(sT) + 1024;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                T av[2][4], bv[2][4];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi) {
+                    const int r = (w >> 1) * 32 + bi * 16 + li;
+                    const f32x4 x = sa[r * 16 + ((4 * h + lk) ^ (r & 15))];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) av[bi][e] = -x[e];
+                }
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const int r = (w & 1) * 32 + bj * 16 + li;
+                    const f32x4 x = sb[r * 16 + ((4 * h + lk) ^ (r & 15))];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) bv[bj][e] = x[e];
+                }
+                if (tail) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+            }
         } else {
             product(J.colk, J.rowk, tail, true);
         }
@@ -1377,7 +1437,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
         return;
     }
     const int kb2 = min(NB, np - k1);
-    if constexpr (kStageP) __syncthreads();   // every wave is past its reads of the staged panels
+    if constexpr (kStageP || kStage1) __syncthreads();   // every wave is past its reads of the staged panels
     if (is_piv) {
         // ------------------------------------- next pivot block inverse ----
 #pragma unroll
