@@ -188,14 +188,15 @@ class DeviceContext:
     MF_GROUPS = ("assemble", "scatter", "small", "gj_begin", "gj_full", "gj_light", "gj_heavy", "front1",
                  "gather", "fwd", "bwd")
     # kernel of each group, as rocprofv3 names it ({T}: float or double);
-    # "small", "assemble" and "front1" are several instances of one template
+    # "small" and "assemble" are several instances of one template, "front1"
+    # the instances of two (head, then row tiles)
     MF_KERNELS = {"assemble": "k_mf_assemble<{T}, *>", "scatter": "k_mf_scatter<{T}>",
                   "small": "k_mf_small_fused<{T}, *>",
                   "gj_begin": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 3>",
                   "gj_full": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 0>",
                   "gj_light": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 1>",
                   "gj_heavy": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 2>",
-                  "front1": "k_mf_front1<{T}, *>",
+                  "front1": "k_mf_front1_*<{T}, *>",
                   "gather": "k_mf_fgather<{T}>", "fwd": "k_mf_fgemv<{T}>", "bwd": "k_mf_bwd<{T}>"}
 
     def profile_mf(self) -> dict:
