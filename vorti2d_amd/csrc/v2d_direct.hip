@@ -1014,7 +1014,7 @@ template <> struct Mfma<double> {
 constexpr int kSpinLimit = 1 << 21;
 
 template <typename T, typename BT = Batch<T>, int MODE = kStepFull>
-__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ? 2 : 4) k_gj_step_mfma_b(BT B, int k0, int epoch,
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(BT B, int k0, int epoch,
                                                         int *__restrict__ flags, int *__restrict__ err)
 {
     using M = Mfma<T>;
@@ -1028,7 +1028,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
     constexpr bool kStageP = !std::is_same<BT, Batch<T>>::value && sizeof(T) == 4 && MODE == kStepHeavy;
     constexpr bool kStage1 = !std::is_same<BT, Batch<T>>::value && sizeof(T) == 4 &&
                              (MODE == kStepLight || MODE == kStepFull);
-    __shared__ __align__(16) T sT[kStageP ? 4 * NB * NB : kStage1 ? 2 * NB * NB : NB * LT];
+    __shared__ __align__(16) T sT[(kStageP || kStage1) ? 2 * NB * NB : NB * LT];
     constexpr bool kTable = !std::is_same<BT, Batch<T>>::value;
     static_assert(kTable || MODE == kStepFull, "the level batches take full steps only");
     int z, lin;
@@ -1196,27 +1196,39 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
                 }
         } else if (useP) {
             if constexpr (kStageP) {
-                // The four panel blocks (previous step's column and row panels,
-                // then this step's) go to LDS by direct 16-byte loads, one
-                // round for all of them, each block read once per workgroup:
-                // chunk kc (4 pivots) of block row r sits at chunk kc ^ (r & 15)
-                // of the row, so the sixteen rows a matrix-core operand read
-                // spans fall in distinct banks.  A wave's load fills 64
-                // consecutive chunks (lane l the l-th), i.e. 4 block rows.
-                const T *src[4] = {J.colk_x, J.rowk_x, J.colk, J.rowk};
+                // The two column-panel blocks (previous step's, then this
+                // step's: rows of the tile, 64 pivots) go to LDS by direct
+                // 16-byte loads, one round, each read once per workgroup: chunk
+                // kc (4 pivots) of block row r sits at chunk kc ^ (r & 15) of
+                // the row, so the sixteen rows a matrix-core operand read spans
+                // fall in distinct banks; a wave's load fills 64 consecutive
+                // chunks, i.e. 4 block rows.  The row-panel operands go to
+                // registers in the same round (each wave its 32 columns).
+                const T *src[2] = {J.colk_x, J.colk};
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
+                for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int q0 = (w * 4 + j) * 64;
                         const int r = (q0 + l) >> 4, kc = (l & 15) ^ (r & 15);
-                        // column panels: rows past n read row n - 1 (results dropped);
-                        // row panels hold every row of the 64-rounded front
-                        const int gr = (b & 1) ? c0 + r : min(i0 + r, n - 1);
+                        // rows past n read row n - 1 (their results are dropped)
+                        const int gr = min(i0 + r, n - 1);
                         __builtin_amdgcn_global_load_lds(
                             (const void __attribute__((address_space(1))) *)(src[b] + (int64_t)gr * NB + 4 * kc),
                             (void __attribute__((address_space(3))) *)(sT + (b * 1024 + q0) * 4), 16, 0, 0);
                     }
+                f32x4 xb[2][4][2];   // [product][h][bj]
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const T *rk = p == 0 ? J.rowk_x : J.rowk;
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const gptr<const f32x4> q =
+                            glb(reinterpret_cast<const f32x4 *>(rk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk));
+#pragma unroll
+                        for (int h = 0; h < 4; ++h) xb[p][h][bj] = q[4 * h];
+                    }
+                }
 #pragma unroll
                 for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -1231,9 +1243,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
                         old[bi][bj][3] = in ? x[3] : T(0);
                     }
                 __syncthreads();   // the panel blocks have landed
-                auto lds_product = [&](int ba, int bb, bool msk) {
-                    const f32x4 *sa = reinterpret_cast<const f32x4 *>(sT) + ba * 1024;
-                    const f32x4 *sb = reinterpret_cast<const f32x4 *>(sT) + bb * 1024;
+                auto lds_product = [&](int p, bool msk) {
+                    const f32x4 *sa = reinterpret_cast<const f32x4 *>(sT) + p * 1024;
 #pragma unroll
                     for (int h = 0; h < 4; ++h) {
                         T av[2][4], bv[2][4];
@@ -1245,12 +1256,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
                             for (int e = 0; e < 4; ++e) av[bi][e] = -x[e];
                         }
 #pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) {
-                            const int r = (w & 1) * 32 + bj * 16 + li;
-                            const f32x4 x = sb[r * 16 + ((4 * h + lk) ^ (r & 15))];
+                        for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-                            for (int e = 0; e < 4; ++e) bv[bj][e] = x[e];
-                        }
+                            for (int e = 0; e < 4; ++e) bv[bj][e] = xb[p][h][bj][e];
                         if (msk) {
 #pragma unroll
                             for (int e = 0; e < 4; ++e)
@@ -1265,8 +1273,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : MODE == kStepHeavy ?
                                     acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
                     }
                 };
-                lds_product(0, 1, false);
-                lds_product(2, 3, tail);
+                lds_product(0, false);
+                lds_product(1, tail);
             } else {
                 product(J.colk_x, J.rowk_x, false, true);
                 product(J.colk, J.rowk, tail, false);
