@@ -1023,13 +1023,14 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
     using NV = typename NVec<T>::type;
     constexpr int VW = Vec8<T>::w;
     constexpr int LT = NB + 4;    // padded LDS row stride
-    // next pivot block, or a tile stored transposed; in a heavy step first
-    // the four panel blocks of the tile's two products (FP32 job tables)
-    constexpr bool kStageP = !std::is_same<BT, Batch<T>>::value && sizeof(T) == 4 && MODE == kStepHeavy;
-    constexpr bool kStage1 = !std::is_same<BT, Batch<T>>::value && sizeof(T) == 4 &&
-                             (MODE == kStepLight || MODE == kStepFull);
-    __shared__ __align__(16) T sT[(kStageP || kStage1) ? 2 * NB * NB : NB * LT];
+    // next pivot block, or a tile stored transposed; before that (FP32 job
+    // tables) the panel blocks of the tile's product staged in LDS: a heavy
+    // step's two column-panel blocks, a light or full step's column- and
+    // row-panel blocks
     constexpr bool kTable = !std::is_same<BT, Batch<T>>::value;
+    constexpr bool kStageP = kTable && sizeof(T) == 4 && MODE == kStepHeavy;
+    constexpr bool kStage1 = kTable && sizeof(T) == 4 && (MODE == kStepLight || MODE == kStepFull);
+    __shared__ __align__(16) T sT[(kStageP || kStage1) ? 2 * NB * NB : NB * LT];
     static_assert(kTable || MODE == kStepFull, "the level batches take full steps only");
     int z, lin;
     if constexpr (kTable) {

@@ -809,11 +809,11 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
         for (int j = 0; j < NI; ++j) a[i][j] = T(0);
     for (int r0 = 0; r0 < nf; r0 += cr) {
         const int r1 = min(nf, r0 + cr), ne = (r1 - r0) * nf;
-        // Every element e is set and summed by one thread: identity on the
-        // padding pivots, then the children's contributions in list order.
-        // Rounds of U elements per thread issue every map load, then every
-        // contribution load (clamped addresses, the absent ones dropped by a
-        // select: no load behind a branch), before the first sum.
+        // Every element pair (2 q, 2 q + 1) is set and summed by one thread:
+        // identity on the padding pivots, then the children's contributions
+        // in list order.  Rounds of U pairs per thread issue every map load,
+        // then every contribution load (clamped addresses, the absent ones
+        // dropped by a select: no load behind a branch), before the first sum.
         constexpr int U = 4;
         using P2 = T __attribute__((ext_vector_type(2)));
         const int np2 = ne >> 1;   // element pairs (2 q, 2 q + 1): nf is even, a pair lies in one row
