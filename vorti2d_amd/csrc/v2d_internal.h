@@ -418,13 +418,6 @@ struct MfPattern {
     std::vector<int2> rows, gemv, bwd, colt, pcolt;
     std::vector<int2> asmb;     // assembly blocks of the big fronts: (front, first row)
     std::vector<int64_t> cvo;   // per child-list entry: the child's coupling part in Vf
-    // forward solve by child slots (slots = 2: every front has at most two
-    // children): pslot[front row] = the slot (2 * parent row + child index)
-    // that a coupling row's value goes to, -1 elsewhere; zb[pivot row] = its
-    // unknown, -1 on padding pivots
-    int slots = 0;
-    std::vector<int> pslot, zb;
-    int *d_pslot = nullptr, *d_zb = nullptr;
     // device copies
     MfFront *d_fr = nullptr;
     int *d_fu = nullptr, *d_ch = nullptr, *d_map = nullptr;
@@ -460,7 +453,6 @@ void mf_free_pattern(MfPattern &P, hipStream_t s);
 template <typename T>
 struct MfWork {
     T *F = nullptr, *Vf = nullptr, *Zf = nullptr, *pool = nullptr;
-    T *Vs = nullptr, *Bf = nullptr;   // child slots (2 per front row, zeroed once), b in front order
     Job<T> *jobs = nullptr;   // Gauss-Jordan jobs of the big fronts, level by level
     int *flags = nullptr, *err = nullptr;
     int epoch = 0;

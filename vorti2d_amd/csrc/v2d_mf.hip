@@ -380,30 +380,6 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
             for (size_t q = 0; q < st[s].size(); ++q) idx[st[s][q]] = -1;
         }
     }
-    // forward-solve slots: child k of a front writes the value of its coupling
-    // row m to slot k of the parent row that m maps to (two slots a front row,
-    // the rows no child reaches keep zero); the unknown of every pivot row
-    {
-        int nchmax = 0;
-        for (int f = 0; f < ns_; ++f) nchmax = std::max(nchmax, P.fr[f].nch);
-        P.slots = (nchmax <= 2 && 2 * P.nfu < (int64_t)INT32_MAX) ? 2 : 0;
-        P.pslot.clear();
-        if (P.slots) {
-            P.pslot.assign(P.nfu, -1);
-            for (int f = 0; f < ns_; ++f) {
-                const MfFront &F = P.fr[f];
-                for (int k = 0; k < F.nch; ++k) {
-                    const MfFront &Cf = P.fr[P.ch[F.co + k]];
-                    const int *m = P.map.data() + F.mo + (int64_t)k * F.nf;
-                    for (int a = 0; a < F.nf; ++a)
-                        if (m[a] >= 0) P.pslot[Cf.uo + Cf.nsp + m[a]] = (int)(2 * (F.uo + a) + k);
-                }
-            }
-        }
-        P.zb.assign(P.Z_elems, -1);
-        for (int f = 0; f < ns_; ++f)
-            for (int a = 0; a < P.fr[f].ns; ++a) P.zb[P.fr[f].zo + a] = P.fu[P.fr[f].uo + a];
-    }
     // Jacobian entries: each belongs to the front of the earlier of its row's
     // and column's supernodes, where both are front rows
     {
@@ -596,7 +572,6 @@ int mf_upload(MfPattern &P, hipStream_t s)
     if (P.on_device) return 0;
     if (upload(P.d_fr, P.fr, s) || upload(P.d_fu, P.fu, s) || upload(P.d_ch, P.ch, s) ||
         upload(P.d_map, P.map, s) || upload(P.d_rows, P.rows, s) || upload(P.d_cvo, P.cvo, s) || upload(P.d_gemv, P.gemv, s) ||
-        upload(P.d_pslot, P.pslot, s) || upload(P.d_zb, P.zb, s) ||
         upload(P.d_bwd, P.bwd, s) || upload(P.d_colt, P.colt, s) || upload(P.d_pcolt, P.pcolt, s) || upload(P.d_asmb, P.asmb, s) ||
         upload(P.d_aoff, P.aoff, s) || upload(P.d_aseg, P.aseg, s) || upload(P.d_avid, P.avid, s))
         return -1;
@@ -607,12 +582,11 @@ int mf_upload(MfPattern &P, hipStream_t s)
 
 void mf_free_pattern(MfPattern &P, hipStream_t s)
 {
-    void *ptrs[] = {P.d_fr, P.d_fu, P.d_ch, P.d_map, P.d_rows, P.d_cvo, P.d_gemv, P.d_bwd, P.d_pslot, P.d_zb,
+    void *ptrs[] = {P.d_fr, P.d_fu, P.d_ch, P.d_map, P.d_rows, P.d_cvo, P.d_gemv, P.d_bwd,
                     P.d_colt, P.d_pcolt, P.d_asmb, P.d_aoff, P.d_aseg, P.d_avid};
     for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
     P.d_fr = nullptr; P.d_fu = P.d_ch = P.d_map = nullptr;
     P.d_rows = P.d_gemv = P.d_bwd = P.d_colt = P.d_pcolt = P.d_asmb = nullptr;
-    P.d_pslot = P.d_zb = nullptr;
     P.d_cvo = nullptr;
     P.d_aoff = nullptr; P.d_aseg = P.d_avid = nullptr;
     P.on_device = false;
@@ -628,11 +602,6 @@ int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s)
     V2D_HIP(hipMallocAsync((void **)&W.F, sizeof(T) * std::max<int64_t>(P.F_elems, 1), s));
     V2D_HIP(hipMallocAsync((void **)&W.Vf, sizeof(T) * std::max<int64_t>(P.V_elems, 1), s));
     V2D_HIP(hipMallocAsync((void **)&W.Zf, sizeof(T) * std::max<int64_t>(P.Z_elems, 1), s));
-    if (P.slots) {
-        V2D_HIP(hipMallocAsync((void **)&W.Vs, sizeof(T) * 2 * std::max<int64_t>(P.nfu, 1), s));
-        V2D_HIP(hipMemsetAsync(W.Vs, 0, sizeof(T) * 2 * std::max<int64_t>(P.nfu, 1), s));   // once: unreached slots
-        V2D_HIP(hipMallocAsync((void **)&W.Bf, sizeof(T) * std::max<int64_t>(P.Z_elems, 1), s));
-    }
     V2D_HIP(hipMallocAsync((void **)&W.pool, sizeof(T) * std::max<int64_t>(P.pool_elems, 1), s));
     int nbmax = 1;
     std::vector<Job<T>> jobs;
@@ -676,7 +645,7 @@ int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s)
 template <typename T>
 void mf_free(MfWork<T> &W, hipStream_t s)
 {
-    void *ptrs[] = {W.F, W.Vf, W.Zf, W.pool, W.jobs, W.flags, W.Vs, W.Bf};
+    void *ptrs[] = {W.F, W.Vf, W.Zf, W.pool, W.jobs, W.flags};
     for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
     const bool paired = W.paired, fused = W.fused;
     W = MfWork<T>{};
@@ -1110,58 +1079,6 @@ __global__ void __launch_bounds__(256) k_mf_fgemv(const MfFront *__restrict__ fr
         });
 }
 
-// The right-hand side in front order: Bf[z] = b of pivot row z's unknown,
-// 0 on padding pivots.  One thread per pivot row of every front.
-template <typename T>
-__global__ void __launch_bounds__(256) k_mf_bperm(int64_t nz, const int *__restrict__ zb, const T *__restrict__ b,
-                                                  T *__restrict__ Bf)
-{
-    const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (z >= nz) return;
-    const int u = zb[z];
-    const T v = b[max(u, 0)];
-    Bf[z] = u >= 0 ? v : T(0);
-}
-
-// The per-row operands of a forward row in slot form: the children's
-// contributions (v_b) and the slot of the parent row the result goes to.
-template <typename T>
-struct VS {
-    T v;
-    int p;
-};
-
-// Forward front products in slot form: v = b on the pivots (Bf) plus the
-// two child slots of the row (the forward gather's sums, in child order);
-// z = u on the pivots, and v_b + u into the parent row's slot.
-template <typename T>
-__global__ void __launch_bounds__(256) k_mf_fgemv_s(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
-                                                    int64_t g0, const T *__restrict__ F, const T *__restrict__ Bf,
-                                                    const int *__restrict__ pslot, T *__restrict__ Vs,
-                                                    T *__restrict__ Zf)
-{
-    extern __shared__ unsigned char smem[];
-    T *xs = reinterpret_cast<T *>(smem);
-    const int2 fb = blk[g0 + blockIdx.x];
-    const MfFront f = fr[fb.x];
-    mf_rows<T>(
-        f, F, fb.y, f.nf, 0, f.nsp, xs,
-        [&] {
-            for (int a = threadIdx.x; a < f.nsp; a += blockDim.x) {
-                const T *sl = Vs + 2 * (f.uo + a);
-                xs[a] = (Bf[f.zo + a] + sl[0]) + sl[1];
-            }
-        },
-        [&](int r) {
-            const T *sl = Vs + 2 * (f.uo + r);
-            return VS<T>{sl[0] + sl[1], pslot[f.uo + r]};
-        },
-        [&](int r, const VS<T> &sv, double acc) {
-            if (r < f.nsp) Zf[f.zo + r] = (T)acc;
-            else if (sv.p >= 0) Vs[sv.p] = (T)((double)sv.v + acc);
-        });
-}
-
 // The per-row operands of a backward row: z and the row's unknown.
 template <typename T>
 struct ZU {
@@ -1451,12 +1368,6 @@ int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hip
         return -1;
     }
     const double ts = sizeof(T);
-    const bool slots = !trans && P.slots == 2 && W.Vs;
-    if (slots && P.Z_elems > 0) {
-        MF_BEGIN();
-        hipLaunchKernelGGL(k_mf_bperm<T>, dim3(grid_for(P.Z_elems)), dim3(256), 0, s, P.Z_elems, P.d_zb, b, W.Bf);
-        MF_END(kGather, 0.0, (2.0 * ts + 4.0) * (double)P.Z_elems);
-    }
     for (int h = 0; h < P.nheight; ++h) {
         const MfLevel &L = P.lv[h];
         int nspmax = 0;
@@ -1468,16 +1379,6 @@ int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hip
                 fb += ts * ((double)F.nf * F.nsp + F.nsp + 2.0 * F.nf);
                 gb += ts * (2.0 * F.nf) + 4.0 * F.nf * (1 + F.nch);
             }
-        }
-        if (slots) {
-            if (L.g1 > L.g0) {
-                MF_BEGIN();
-                hipLaunchKernelGGL(k_mf_fgemv_s<T>, dim3((unsigned)(L.g1 - L.g0)), dim3(256), sizeof(T) * nspmax, s,
-                                   P.d_fr, P.d_gemv, L.g0, W.F, W.Bf, P.d_pslot, W.Vs, W.Zf);
-                MF_END(kFwd, 0.0, fb + gb);
-            }
-            V2D_LAUNCH();
-            continue;
         }
         if (L.r1 > L.r0) {
             MF_BEGIN();
