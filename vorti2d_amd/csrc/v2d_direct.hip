@@ -1092,7 +1092,14 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
     // step: its waves take issue priority over the bulk update sharing the SIMDs
     if (is_piv || is_prow) __builtin_amdgcn_s_setprio(3);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-    const int rw = i0 + (w >> 1) * 32, cw = c0 + (w & 1) * 32;
+    // The wavefront's share of the 64 x 64 tile, NBI x NBJ blocks of 16 x 16:
+    // in a heavy FP32 step a 64 x 16 column strip (rwl = 0, cwl = 16 w), so no
+    // row-panel operand is loaded twice in a workgroup; otherwise a 32 x 32
+    // quarter.
+    constexpr bool kStrip = kStageP;
+    constexpr int NBI = kStrip ? 4 : 2, NBJ = kStrip ? 1 : 2;
+    const int rwl = kStrip ? 0 : (w >> 1) * 32, cwl = kStrip ? w * 16 : (w & 1) * 32;
+    const int rw = i0 + rwl, cw = c0 + cwl;
     const bool colK = c0 == k0;            // the pivot columns: old values are zero
     const bool cfull = c0 + NB <= n;       // no column of the tile past n
     // A heavy step also applies the previous step's panels (colk_x, rowk_x),
@@ -1107,11 +1114,11 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
         oldZero = colK || (colP && !rowP);
     }
     // ---------------------------------------------------- tile update ----
-    C acc[2][2], old[2][2];
+    C acc[NBI][NBJ], old[NBI][NBJ];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+        for (int bj = 0; bj < NBJ; ++bj)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[bi][bj][v] = T(0);
     if constexpr (kTable && sizeof(T) == 4) {
@@ -1124,28 +1131,28 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
         // the launch is slower.
         const bool tail = kb < NB;
         auto product = [&](const T *ck, const T *rk, bool msk, bool with_old) {
-            gptr<const NV> qa[2], qb[2];
+            gptr<const NV> qa[NBI], qb[NBJ];
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
+            for (int bi = 0; bi < NBI; ++bi)
                 qa[bi] = glb(reinterpret_cast<const NV *>(ck + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk));
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
+            for (int bj = 0; bj < NBJ; ++bj)
                 qb[bj] = glb(reinterpret_cast<const NV *>(rk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk));
-            NV xa[4][2][4 / VW], xb[4][2][4 / VW];
+            NV xa[4][NBI][4 / VW], xb[4][NBJ][4 / VW];
 #pragma unroll
             for (int hh = 0; hh < 4; ++hh)
 #pragma unroll
-                for (int b2 = 0; b2 < 2; ++b2)
+                for (int u = 0; u < 4 / VW; ++u) {
 #pragma unroll
-                    for (int u = 0; u < 4 / VW; ++u) {
-                        xa[hh][b2][u] = qa[b2][(16 * hh) / VW + u];
-                        xb[hh][b2][u] = qb[b2][(16 * hh) / VW + u];
-                    }
+                    for (int bi = 0; bi < NBI; ++bi) xa[hh][bi][u] = qa[bi][(16 * hh) / VW + u];
+#pragma unroll
+                    for (int bj = 0; bj < NBJ; ++bj) xb[hh][bj][u] = qb[bj][(16 * hh) / VW + u];
+                }
             if (with_old) {
 #pragma unroll
-                for (int bi = 0; bi < 2; ++bi)
+                for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
+                    for (int bj = 0; bj < NBJ; ++bj) {
                         const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                         const f32x4 x = *glb(reinterpret_cast<const f32x4 *>(
                             A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
@@ -1158,34 +1165,38 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
             }
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
-                T av[2][4], bv[2][4];
+                T av[NBI][4], bv[NBJ][4];
 #pragma unroll
-                for (int b2 = 0; b2 < 2; ++b2)
+                for (int u = 0; u < 4 / VW; ++u)
 #pragma unroll
-                    for (int u = 0; u < 4 / VW; ++u)
+                    for (int e = 0; e < VW; ++e) {
 #pragma unroll
-                        for (int e = 0; e < VW; ++e) {
-                            av[b2][u * VW + e] = -reinterpret_cast<const T *>(&xa[h][b2][u])[e];
-                            bv[b2][u * VW + e] = reinterpret_cast<const T *>(&xb[h][b2][u])[e];
-                        }
+                        for (int bi = 0; bi < NBI; ++bi)
+                            av[bi][u * VW + e] = -reinterpret_cast<const T *>(&xa[h][bi][u])[e];
+#pragma unroll
+                        for (int bj = 0; bj < NBJ; ++bj)
+                            bv[bj][u * VW + e] = reinterpret_cast<const T *>(&xb[h][bj][u])[e];
+                    }
                 if (msk) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
-                        if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                        if (16 * h + 4 * lk + e >= kb)
+#pragma unroll
+                            for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
 #pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
+                    for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+                        for (int bj = 0; bj < NBJ; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
             }
         };
         if constexpr (MODE == kStepBegin) {
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
+            for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
+                for (int bj = 0; bj < NBJ; ++bj) {
                     const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                     const f32x4 x =
                         *glb(reinterpret_cast<const f32x4 *>(A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
@@ -1195,8 +1206,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                     old[bi][bj][2] = in ? x[2] : T(0);
                     old[bi][bj][3] = in ? x[3] : T(0);
                 }
-        } else if (useP) {
-            if constexpr (kStageP) {
+        } else if constexpr (kStageP) {
+            {
                 // The two column-panel blocks (previous step's, then this
                 // step's: rows of the tile, 64 pivots) go to LDS by direct
                 // 16-byte loads, one round, each read once per workgroup: chunk
@@ -1218,12 +1229,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                             (const void __attribute__((address_space(1))) *)(src[b] + (int64_t)gr * NB + 4 * kc),
                             (void __attribute__((address_space(3))) *)(sT + (b * 1024 + q0) * 4), 16, 0, 0);
                     }
-                f32x4 xb[2][4][2];   // [product][h][bj]
+                f32x4 xb[2][4][NBJ];   // [product][h][bj]
 #pragma unroll
                 for (int p = 0; p < 2; ++p) {
                     const T *rk = p == 0 ? J.rowk_x : J.rowk;
 #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
+                    for (int bj = 0; bj < NBJ; ++bj) {
                         const gptr<const f32x4> q =
                             glb(reinterpret_cast<const f32x4 *>(rk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk));
 #pragma unroll
@@ -1231,9 +1242,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                     }
                 }
 #pragma unroll
-                for (int bi = 0; bi < 2; ++bi)
+                for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
+                    for (int bj = 0; bj < NBJ; ++bj) {
                         const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                         const f32x4 x = *glb(reinterpret_cast<const f32x4 *>(
                             A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
@@ -1248,37 +1259,38 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                     const f32x4 *sa = reinterpret_cast<const f32x4 *>(sT) + p * 1024;
 #pragma unroll
                     for (int h = 0; h < 4; ++h) {
-                        T av[2][4], bv[2][4];
+                        T av[NBI][4], bv[NBJ][4];
 #pragma unroll
-                        for (int bi = 0; bi < 2; ++bi) {
-                            const int r = (w >> 1) * 32 + bi * 16 + li;
+                        for (int bi = 0; bi < NBI; ++bi) {
+                            const int r = rwl + bi * 16 + li;
                             const f32x4 x = sa[r * 16 + ((4 * h + lk) ^ (r & 15))];
 #pragma unroll
                             for (int e = 0; e < 4; ++e) av[bi][e] = -x[e];
                         }
 #pragma unroll
-                        for (int bj = 0; bj < 2; ++bj)
+                        for (int bj = 0; bj < NBJ; ++bj)
 #pragma unroll
                             for (int e = 0; e < 4; ++e) bv[bj][e] = xb[p][h][bj][e];
                         if (msk) {
 #pragma unroll
                             for (int e = 0; e < 4; ++e)
-                                if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                                if (16 * h + 4 * lk + e >= kb)
+                                    for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
                         }
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
 #pragma unroll
-                            for (int bi = 0; bi < 2; ++bi)
+                            for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-                                for (int bj = 0; bj < 2; ++bj)
+                                for (int bj = 0; bj < NBJ; ++bj)
                                     acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
                     }
                 };
-                lds_product(0, false);
+                // the previous step's panels only where the light step deferred
+                // them (the loads above are harmless elsewhere: every job holds
+                // its three panel buffers)
+                if (useP) lds_product(0, false);
                 lds_product(1, tail);
-            } else {
-                product(J.colk_x, J.rowk_x, false, true);
-                product(J.colk, J.rowk, tail, false);
             }
         } else if constexpr (kStage1) {
             const T *src[2] = {J.colk, J.rowk};
@@ -1294,9 +1306,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                         (void __attribute__((address_space(3))) *)(sT + (b * 1024 + q0) * 4), 16, 0, 0);
                 }
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
+            for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
+                for (int bj = 0; bj < NBJ; ++bj) {
                     const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                     const f32x4 x = *glb(reinterpret_cast<const f32x4 *>(
                         A + (int64_t)min(r, n - 1) * ld + min(c, ld - 4)));
@@ -1311,17 +1323,17 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
             const f32x4 *sb = reinterpret_cast<const f32x4 *>(sT) + 1024;
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
-                T av[2][4], bv[2][4];
+                T av[NBI][4], bv[NBJ][4];
 #pragma unroll
-                for (int bi = 0; bi < 2; ++bi) {
-                    const int r = (w >> 1) * 32 + bi * 16 + li;
+                for (int bi = 0; bi < NBI; ++bi) {
+                    const int r = rwl + bi * 16 + li;
                     const f32x4 x = sa[r * 16 + ((4 * h + lk) ^ (r & 15))];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) av[bi][e] = -x[e];
                 }
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const int r = (w & 1) * 32 + bj * 16 + li;
+                for (int bj = 0; bj < NBJ; ++bj) {
+                    const int r = cwl + bj * 16 + li;
                     const f32x4 x = sb[r * 16 + ((4 * h + lk) ^ (r & 15))];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) bv[bj][e] = x[e];
@@ -1329,29 +1341,33 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                 if (tail) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
-                        if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                        if (16 * h + 4 * lk + e >= kb)
+                                    for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
 #pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
+                    for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+                        for (int bj = 0; bj < NBJ; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
             }
+        } else if (useP) {
+            product(J.colk_x, J.rowk_x, false, true);
+            product(J.colk, J.rowk, tail, false);
         } else {
             product(J.colk, J.rowk, tail, true);
         }
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
+            for (int bj = 0; bj < NBJ; ++bj)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
     } else {
     #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < NBI; ++bi)
     #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) {
+            for (int bj = 0; bj < NBJ; ++bj) {
                 if (M::kSwap && cfull && !oldZero) {
                     const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                     if (r < n) {
@@ -1372,14 +1388,14 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
         auto product = [&](const T *ck, const T *rk, bool msk) {
             gptr<const T> pa[2], pb[2];
     #pragma unroll
-            for (int bi = 0; bi < 2; ++bi) pa[bi] = glb(ck + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk);
+            for (int bi = 0; bi < NBI; ++bi) pa[bi] = glb(ck + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk);
     #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) pb[bj] = glb(rk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk);
+            for (int bj = 0; bj < NBJ; ++bj) pb[bj] = glb(rk + (int64_t)(cw + bj * 16 + li) * NB + 4 * lk);
     #pragma unroll
             for (int h = 0; h < 4; ++h) {
-                T av[2][4], bv[2][4];
+                T av[NBI][4], bv[NBJ][4];
     #pragma unroll
-                for (int bi = 0; bi < 2; ++bi)
+                for (int bi = 0; bi < NBI; ++bi)
     #pragma unroll
                     for (int u = 0; u < 4 / VW; ++u) {
                         const NV x = reinterpret_cast<gptr<const NV>>(pa[bi] + 16 * h)[u];
@@ -1387,7 +1403,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                         for (int e = 0; e < VW; ++e) av[bi][u * VW + e] = -reinterpret_cast<const T *>(&x)[e];
                     }
     #pragma unroll
-                for (int bj = 0; bj < 2; ++bj)
+                for (int bj = 0; bj < NBJ; ++bj)
     #pragma unroll
                     for (int u = 0; u < 4 / VW; ++u) {
                         const NV x = reinterpret_cast<gptr<const NV>>(pb[bj] + 16 * h)[u];
@@ -1397,14 +1413,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                 if (msk) {
     #pragma unroll
                     for (int e = 0; e < 4; ++e)
-                        if (16 * h + 4 * lk + e >= kb) av[0][e] = av[1][e] = T(0);
+                        if (16 * h + 4 * lk + e >= kb)
+                                    for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
                 }
     #pragma unroll
                 for (int e = 0; e < 4; ++e)
     #pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
+                    for (int bi = 0; bi < NBI; ++bi)
     #pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
+                        for (int bj = 0; bj < NBJ; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
             }
         };
         if constexpr (MODE != kStepBegin) {
@@ -1412,18 +1429,18 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
             product(J.colk, J.rowk, kb < NB);
         }
     #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < NBI; ++bi)
     #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
+            for (int bj = 0; bj < NBJ; ++bj)
     #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[bi][bj][v] += old[bi][bj][v];
     }
     // ------------------------------------------------ ordinary tiles ----
     if (!is_piv && !is_prow) {
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) {
+            for (int bj = 0; bj < NBJ; ++bj) {
                 if (M::kSwap && cfull) {
                     const int r = rw + bi * 16 + M::row(li, lk, 0), c = cw + bj * 16 + M::col(li, lk, 0);
                     if (r < n) {
@@ -1450,13 +1467,13 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
     if (is_piv) {
         // ------------------------------------- next pivot block inverse ----
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
+            for (int bj = 0; bj < NBJ; ++bj)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int r = (w >> 1) * 32 + bi * 16 + M::row(li, lk, v);
-                    const int c = (w & 1) * 32 + bj * 16 + M::col(li, lk, v);
+                    const int r = rwl + bi * 16 + M::row(li, lk, v);
+                    const int c = cwl + bj * 16 + M::col(li, lk, v);
                     sT[r * LT + c] = (r < kb2 && c < kb2) ? acc[bi][bj][v] : T(0);
                 }
         const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
@@ -1482,13 +1499,13 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
     }
     // ------------------------------------------------ next pivot rows ----
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < NBI; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+        for (int bj = 0; bj < NBJ; ++bj)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = (w >> 1) * 32 + bi * 16 + M::row(li, lk, v);
-                const int c = (w & 1) * 32 + bj * 16 + M::col(li, lk, v);
+                const int r = rwl + bi * 16 + M::row(li, lk, v);
+                const int c = cwl + bj * 16 + M::col(li, lk, v);
                 sT[c * LT + r] = acc[bi][bj][v];   // transposed: sT[c][t]
             }
     if (threadIdx.x == 0) {
