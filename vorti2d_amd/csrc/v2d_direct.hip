@@ -1342,7 +1342,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
                         if (16 * h + 4 * lk + e >= kb)
-                                    for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
+                            for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
@@ -1386,7 +1386,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                 }
             }
         auto product = [&](const T *ck, const T *rk, bool msk) {
-            gptr<const T> pa[2], pb[2];
+            gptr<const T> pa[NBI], pb[NBJ];
     #pragma unroll
             for (int bi = 0; bi < NBI; ++bi) pa[bi] = glb(ck + (int64_t)min(rw + bi * 16 + li, n - 1) * NB + 4 * lk);
     #pragma unroll
@@ -1414,7 +1414,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
     #pragma unroll
                     for (int e = 0; e < 4; ++e)
                         if (16 * h + 4 * lk + e >= kb)
-                                    for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
+                            for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
                 }
     #pragma unroll
                 for (int e = 0; e < 4; ++e)
