@@ -1014,7 +1014,7 @@ template <> struct Mfma<double> {
 constexpr int kSpinLimit = 1 << 21;
 
 template <typename T, typename BT = Batch<T>, int MODE = kStepFull>
-__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(BT B, int k0, int epoch,
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? (MODE == kStepHeavy ? 2 : 1) : 4) k_gj_step_mfma_b(BT B, int k0, int epoch,
                                                         int *__restrict__ flags, int *__restrict__ err)
 {
     using M = Mfma<T>;
@@ -1030,7 +1030,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
     constexpr bool kTable = !std::is_same<BT, Batch<T>>::value;
     constexpr bool kStageP = kTable && sizeof(T) == 4 && MODE == kStepHeavy;
     constexpr bool kStage1 = kTable && sizeof(T) == 4 && (MODE == kStepLight || MODE == kStepFull);
-    __shared__ __align__(16) T sT[(kStageP || kStage1) ? 2 * NB * NB : NB * LT];
+    constexpr bool kStageD = kTable && sizeof(T) == 8 && MODE == kStepHeavy;
+    __shared__ __align__(16) T sT[(kStageP || kStage1 || kStageD) ? 2 * NB * NB : NB * LT];
     static_assert(kTable || MODE == kStepFull, "the level batches take full steps only");
     int z, lin;
     if constexpr (kTable) {
@@ -1096,7 +1097,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
     // in a heavy FP32 step a 64 x 16 column strip (rwl = 0, cwl = 16 w), so no
     // row-panel operand is loaded twice in a workgroup; otherwise a 32 x 32
     // quarter.
-    constexpr bool kStrip = kStageP;
+    constexpr bool kStrip = kStageP || kStageD;
     constexpr int NBI = kStrip ? 4 : 2, NBJ = kStrip ? 1 : 2;
     const int rwl = kStrip ? 0 : (w >> 1) * 32, cwl = kStrip ? w * 16 : (w & 1) * 32;
     const int rw = i0 + rwl, cw = c0 + cwl;
@@ -1424,7 +1425,65 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
                         for (int bj = 0; bj < NBJ; ++bj) acc[bi][bj] = M::op(av[bi][e], bv[bj][e], acc[bi][bj]);
             }
         };
-        if constexpr (MODE != kStepBegin) {
+        if constexpr (kStageD) {
+            // FP64 heavy step: the two column-panel blocks in LDS by direct
+            // 16-byte loads (two pivots a chunk, 32 chunks a block row, chunk
+            // kc of row r at kc ^ (r & 15); a wave's load fills 64 chunks, 2
+            // block rows), the wavefront's row-panel operands (its 16 columns)
+            // in registers, all in one round
+            const T *src[2] = {J.colk_x, J.colk};
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int q0 = (w * 8 + j) * 64;
+                    const int r = (q0 + l) >> 5, kc = ((q0 + l) & 31) ^ (r & 15);
+                    const int gr = min(i0 + r, n - 1);
+                    __builtin_amdgcn_global_load_lds(
+                        (const void __attribute__((address_space(1))) *)(src[b] + (int64_t)gr * NB + 2 * kc),
+                        (void __attribute__((address_space(3))) *)(sT + (b * 2048 + q0) * 2), 16, 0, 0);
+                }
+            f64x2 xb[2][4][2];   // [product][h][half]
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const gptr<const f64x2> q = glb(reinterpret_cast<const f64x2 *>(
+                    (p == 0 ? J.rowk_x : J.rowk) + (int64_t)(cw + li) * NB + 4 * lk));
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    xb[p][h][0] = q[8 * h];
+                    xb[p][h][1] = q[8 * h + 1];
+                }
+            }
+            __syncthreads();   // the panel blocks have landed
+            auto lds_product = [&](int p, bool msk) {
+                const f64x2 *sa = reinterpret_cast<const f64x2 *>(sT) + p * 2048;
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    T av[NBI][4], bv[4];
+                    const int kc = 8 * h + 2 * lk;
+#pragma unroll
+                    for (int bi = 0; bi < NBI; ++bi) {
+                        const int r = bi * 16 + li;
+                        const f64x2 x0 = sa[r * 32 + (kc ^ (r & 15))], x1 = sa[r * 32 + ((kc + 1) ^ (r & 15))];
+                        av[bi][0] = -x0[0]; av[bi][1] = -x0[1]; av[bi][2] = -x1[0]; av[bi][3] = -x1[1];
+                    }
+                    bv[0] = xb[p][h][0][0]; bv[1] = xb[p][h][0][1]; bv[2] = xb[p][h][1][0]; bv[3] = xb[p][h][1][1];
+                    if (msk) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (16 * h + 4 * lk + e >= kb)
+#pragma unroll
+                                for (int bi = 0; bi < NBI; ++bi) av[bi][e] = T(0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int bi = 0; bi < NBI; ++bi) acc[bi][0] = M::op(av[bi][e], bv[e], acc[bi][0]);
+                }
+            };
+            if (useP) lds_product(0, false);
+            lds_product(1, kb < NB);
+        } else if constexpr (MODE != kStepBegin) {
             if (useP) product(J.colk_x, J.rowk_x, false);
             product(J.colk, J.rowk, kb < NB);
         }
@@ -1463,7 +1522,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 1 : 4) k_gj_step_mfma_b(
         return;
     }
     const int kb2 = min(NB, np - k1);
-    if constexpr (kStageP || kStage1) __syncthreads();   // every wave is past its reads of the staged panels
+    if constexpr (kStageP || kStage1 || kStageD) __syncthreads();   // every wave is past its reads of the staged panels
     if (is_piv) {
         // ------------------------------------- next pivot block inverse ----
 #pragma unroll
