@@ -1216,7 +1216,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? (MODE == kStepHeavy ? 2 
                 // the row, so the sixteen rows a matrix-core operand read spans
                 // fall in distinct banks; a wave's load fills 64 consecutive
                 // chunks, i.e. 4 block rows.  The row-panel operands go to
-                // registers in the same round (each wave its 32 columns).
+                // registers in the same round (each wave the 16 columns of its
+                // strip).
                 const T *src[2] = {J.colk_x, J.colk};
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
@@ -1648,7 +1649,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? (MODE == kStepHeavy ? 2 
 //   3. per 64-column tile: the pivot rows P = G^-1 W go to F and, transposed,
 //      to the job's row panel;
 // k_mf_front1_rows, one workgroup per front and 64-row tile below the pivots:
-//   4. X is staged in LDS and -X G^-1 stored as the pivot columns; per
+//   4. X and G^-1 are staged in LDS and -X G^-1 stored as the pivot columns; per
 //      64-column tile the trailing block C is gathered from the children
 //      straight into the matrix-core accumulators, takes - X P and is stored
 //      (zero in the padding columns nf .. ld-1).
