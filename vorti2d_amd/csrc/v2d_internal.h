@@ -459,6 +459,10 @@ struct MfWork {
     bool factored = false;
     bool paired = true;       // Gauss-Jordan steps in light / heavy pairs (else every step full)
     bool fused = true;        // one-step big fronts by mf_front1 (else assembly, scatter and steps)
+    // two more streams for the small-front launches of a height (one per tile
+    // size, independent), joined back to the factorisation stream by events
+    hipStream_t aux[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
 };
 template <typename T>
 int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s);

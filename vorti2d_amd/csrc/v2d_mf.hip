@@ -633,6 +633,11 @@ int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s)
         V2D_HIP(hipMallocAsync((void **)&W.jobs, sizeof(Job<T>) * jobs.size(), s));
         V2D_HIP(hipMemcpyAsync(W.jobs, jobs.data(), sizeof(Job<T>) * jobs.size(), hipMemcpyHostToDevice, s));
     }
+    for (int k = 0; k < 2; ++k) {
+        if (!W.aux[k]) V2D_HIP(hipStreamCreateWithFlags(&W.aux[k], hipStreamNonBlocking));
+        if (!W.join[k]) V2D_HIP(hipEventCreateWithFlags(&W.join[k], hipEventDisableTiming));
+    }
+    if (!W.fork) V2D_HIP(hipEventCreateWithFlags(&W.fork, hipEventDisableTiming));
     V2D_HIP(hipMallocAsync((void **)&W.flags, sizeof(int) * (nbmax + 1), s));
     V2D_HIP(hipMemsetAsync(W.flags, 0, sizeof(int) * (nbmax + 1), s));
     W.err = W.flags + nbmax;
@@ -647,6 +652,14 @@ void mf_free(MfWork<T> &W, hipStream_t s)
 {
     void *ptrs[] = {W.F, W.Vf, W.Zf, W.pool, W.jobs, W.flags};
     for (void *p : ptrs) if (p) (void)hipFreeAsync(p, s);
+    for (int k = 0; k < 2; ++k) {
+        if (W.aux[k]) {
+            (void)hipStreamSynchronize(W.aux[k]);
+            (void)hipStreamDestroy(W.aux[k]);
+        }
+        if (W.join[k]) (void)hipEventDestroy(W.join[k]);
+    }
+    if (W.fork) (void)hipEventDestroy(W.fork);
     const bool paired = W.paired, fused = W.fused;
     W = MfWork<T>{};
     W.paired = paired;
@@ -1230,7 +1243,19 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
     W.factored = false;
     const double ts = sizeof(T);
     hipLaunchKernelGGL(k_mf_zero_int, dim3(1), dim3(1), 0, s, W.err);
+    const bool par = !probe && W.aux[0] && W.aux[1];
+    bool pend = false;   // small-front launches of the last height still on the extra streams
+    auto join = [&]() -> int {
+        if (pend)
+            for (int k = 0; k < 2; ++k) {
+                V2D_HIP(hipEventRecord(W.join[k], W.aux[k]));
+                V2D_HIP(hipStreamWaitEvent(s, W.join[k], 0));
+            }
+        pend = false;
+        return 0;
+    };
     for (int h = 0; h < P.nheight; ++h) {
+        if (join()) return -1;
         const MfLevel &L = P.lv[h];
         const int64_t nr = L.r1 - L.r0;
         // one-step big fronts: assembled, scattered and swept by mf_front1
@@ -1257,12 +1282,25 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
             }
         if (L.fbig > L.f0) {
             MF_BEGIN();
+            // the tile sizes' launches are independent of each other and of the
+            // height's big fronts: untimed, they run on the two extra streams
+            // (and on the factorisation stream when the height has no big
+            // front), joined back at the next height
+            if (par) {
+                V2D_HIP(hipEventRecord(W.fork, s));
+                for (int k = 0; k < 2; ++k) V2D_HIP(hipStreamWaitEvent(W.aux[k], W.fork, 0));
+                pend = true;
+            }
+            const int nst = L.fbig < L.f1 ? 2 : 3;
+            int nl = 0;
             for (int ni = kSmallMinNI; ni <= kSmallMaxNI; ++ni) {
                 const int a0 = L.small_ni[ni], a1 = ni < kSmallMaxNI ? L.small_ni[ni + 1] : L.fbig;
                 if (a1 <= a0) continue;
                 const dim3 g((unsigned)(a1 - a0));
-#define V2D_SMALL(NI)                                                                                     \
-    hipLaunchKernelGGL((k_mf_small_fused<T, NI>), g, dim3(256), 0, s, P.d_fr, a0, P.d_ch, P.d_map, P.d_aoff, \
+                const hipStream_t st = !par ? s : nl % nst < 2 ? W.aux[nl % nst] : s;
+                ++nl;
+#define V2D_SMALL(NI)                                                                                      \
+    hipLaunchKernelGGL((k_mf_small_fused<T, NI>), g, dim3(256), 0, st, P.d_fr, a0, P.d_ch, P.d_map, P.d_aoff, \
                        P.d_aseg, P.d_avid, val, W.F)
                 switch (ni) {
                 case 3: V2D_SMALL(3); break;
@@ -1344,6 +1382,7 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
                 return -1;
         }
     }
+    if (join()) return -1;
     int e = 0;
     V2D_HIP(hipMemcpyAsync(&e, W.err, sizeof(int), hipMemcpyDeviceToHost, s));
     V2D_HIP(hipStreamSynchronize(s));
