@@ -52,6 +52,7 @@ constexpr int kSmallMinNI = 3, kSmallMaxNI = 9;   // register-tile sizes of the 
 constexpr int kMfAlign = 4;      // pivot padding of small fronts
 constexpr int kMfRow = 32;       // row stride of every front, in elements (128-byte FP32 rows)
 constexpr int kNB = 64;          // Gauss-Jordan block of the large fronts
+constexpr int kMfParFronts = 16384;   // fronts from which a height's small fronts use extra streams
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -633,11 +634,13 @@ int mf_alloc(const MfPattern &P, MfWork<T> &W, hipStream_t s)
         V2D_HIP(hipMallocAsync((void **)&W.jobs, sizeof(Job<T>) * jobs.size(), s));
         V2D_HIP(hipMemcpyAsync(W.jobs, jobs.data(), sizeof(Job<T>) * jobs.size(), hipMemcpyHostToDevice, s));
     }
-    for (int k = 0; k < 2; ++k) {
-        if (!W.aux[k]) V2D_HIP(hipStreamCreateWithFlags(&W.aux[k], hipStreamNonBlocking));
-        if (!W.join[k]) V2D_HIP(hipEventCreateWithFlags(&W.join[k], hipEventDisableTiming));
+    if (P.nfront >= kMfParFronts) {
+        for (int k = 0; k < 2; ++k) {
+            if (!W.aux[k]) V2D_HIP(hipStreamCreateWithFlags(&W.aux[k], hipStreamNonBlocking));
+            if (!W.join[k]) V2D_HIP(hipEventCreateWithFlags(&W.join[k], hipEventDisableTiming));
+        }
+        if (!W.fork) V2D_HIP(hipEventCreateWithFlags(&W.fork, hipEventDisableTiming));
     }
-    if (!W.fork) V2D_HIP(hipEventCreateWithFlags(&W.fork, hipEventDisableTiming));
     V2D_HIP(hipMallocAsync((void **)&W.flags, sizeof(int) * (nbmax + 1), s));
     V2D_HIP(hipMemsetAsync(W.flags, 0, sizeof(int) * (nbmax + 1), s));
     W.err = W.flags + nbmax;
@@ -1243,7 +1246,9 @@ int mf_factor(const MfPattern &P, MfWork<T> &W, const T *val, hipStream_t s, MfP
     W.factored = false;
     const double ts = sizeof(T);
     hipLaunchKernelGGL(k_mf_zero_int, dim3(1), dim3(1), 0, s, W.err);
-    const bool par = !probe && W.aux[0] && W.aux[1];
+    // extra streams only for large analyses: small grids are factorised many
+    // at a time (the batched sweep), whose streams already fill the queues
+    const bool par = !probe && W.aux[0] && W.aux[1] && P.nfront >= kMfParFronts;
     bool pend = false;   // small-front launches of the last height still on the extra streams
     auto join = [&]() -> int {
         if (pend)
