@@ -343,6 +343,10 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
         hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
                            c->jmax, c->flags, val, zj, w);
         V2D_LAUNCH();
+        // The columns and ||w||^2 go to pinned host memory without a wait, and
+        // v_{j+1} = w / ||w|| is formed on the device before the host reads
+        // them (unused when the host then stops), so an iteration waits for
+        // the host once.
         for (int pass = 0; pass < 2; ++pass) {
             hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, j + 1, c->gV, n, w,
                                c->partial);
@@ -351,25 +355,23 @@ static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
             hipLaunchKernelGGL(k_multiaxpy<float>, dim3(grid_for(n)), dim3(256), 0, c->s, n, j + 1,
                                c->gV, n, c->hbuf, w);
             V2D_LAUNCH();
-            V2D_HIP(hipMemcpyAsync(pass ? G.h2.data() : G.h.data(), c->hbuf, sizeof(double) * (j + 1),
+            V2D_HIP(hipMemcpyAsync(c->gm_host + pass * Ctx::kGmHost, c->hbuf, sizeof(double) * (j + 1),
                                    hipMemcpyDeviceToHost, c->s));
         }
         hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, 1, w, n, w, c->partial);
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, 1, c->partial,
                            c->red);
         V2D_LAUNCH();
-        double wn2 = 0.0;
-        V2D_HIP(hipMemcpyAsync(&wn2, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        V2D_HIP(hipMemcpyAsync(c->gm_host + 2 * Ctx::kGmHost, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, w, c->red, 2,
+                           w);
+        V2D_LAUNCH();
         V2D_HIP(hipStreamSynchronize(c->s));
         if (sweep_failed(c->wf.err_host)) return -1;
-        const int st = G.advance(wn2, c->rtol_step, iters, relres);
+        std::copy(c->gm_host, c->gm_host + j + 1, G.h.begin());
+        std::copy(c->gm_host + Ctx::kGmHost, c->gm_host + Ctx::kGmHost + j + 1, G.h2.begin());
+        const int st = G.advance(c->gm_host[2 * Ctx::kGmHost], c->rtol_step, iters, relres);
         if (st == 1 || st == 2) return st;
-        {   // v_{j+1} = w / hv  (G.hv outlives the copy: the next iteration syncs before reuse)
-            V2D_HIP(hipMemcpyAsync(c->red, &G.hv, sizeof(double), hipMemcpyHostToDevice, c->s));
-            hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, w,
-                               c->red, 1, w);
-            V2D_LAUNCH();
-        }
         if (st == 0) break;
     }
     const std::vector<double> yv = G.solution();
@@ -720,8 +722,9 @@ static int ctx_create(v2d_ctx **out, int imax, int jmax, const double *xg, const
     c->wd.sweep_kind = V2D_SWEEP_LEVELS;
     c->wf.sweep_kind = (c->pat.sweep_ok && 2 * c->pat.sweep_C >= c->pat.ncu) ? V2D_SWEEP_PERSISTENT
                                                                            : V2D_SWEEP_LEVELS;
-    V2D_HIP(hipHostMalloc((void **)&c->err_host, 2 * sizeof(int), hipHostMallocDefault));
+    V2D_HIP(hipHostMalloc((void **)&c->err_host, (2 + 2 * Ctx::kGmHost + 1) * sizeof(double), hipHostMallocDefault));
     c->err_host[0] = c->err_host[1] = 0;
+    c->gm_host = reinterpret_cast<double *>(c->err_host) + 2;
     c->wd.err_host = c->err_host;
     c->wf.err_host = c->err_host + 1;
     V2D_HIP(hipStreamSynchronize(c->s));

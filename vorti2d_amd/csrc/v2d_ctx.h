@@ -140,7 +140,13 @@ struct Ctx {
         bytes += sizeof(T) * std::max<int64_t>(count, 1);
         return static_cast<T *>(p);
     }
-    int *err_host = nullptr;   // pinned: the solve error words of wd, wf
+    // One pinned host block (one allocation per context): the solve error
+    // words of wd, wf (err_host), then one GMRES iteration's two Gram-Schmidt
+    // columns (kGmHost each) and ||w||^2 (gm_host), read back with one
+    // synchronisation per iteration
+    static constexpr int kGmHost = 128;
+    int *err_host = nullptr;
+    double *gm_host = nullptr;
     ~Ctx()
     {
         drop_graph();

@@ -675,14 +675,21 @@ __global__ void __launch_bounds__(256) k_multiaxpy(int64_t n, int m, const V *__
 template __global__ void k_multiaxpy<double>(int64_t, int, const double *, int64_t, const double *, double *);
 template __global__ void k_multiaxpy<float>(int64_t, int, const float *, int64_t, const double *, float *);
 
-// y = x * s  (s on device, FP64), with type conversion
+// y = x * f with f = s[0] (inv 0), 1 / s[0] (inv 1) or, from a squared norm,
+// 1 / sqrt(s[0]) (inv 2; 1 where that norm is not positive); s on device,
+// FP64, with type conversion
 template <typename A, typename B>
 __global__ void __launch_bounds__(256) k_scale_copy(int64_t n, const A *__restrict__ x, const double *__restrict__ s,
                                                     int inv, B *__restrict__ y)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const double f = inv ? 1.0 / s[0] : s[0];
+    double f = s[0];
+    if (inv == 1) f = 1.0 / f;
+    if (inv == 2) {
+        const double hn = sqrt(fmax(f, 0.0));
+        f = hn > 0.0 ? 1.0 / hn : 1.0;
+    }
     y[t] = (B)((double)x[t] * f);
 }
 template __global__ void k_scale_copy<double, double>(int64_t, const double *, const double *, int, double *);
