@@ -812,12 +812,16 @@ __global__ void __launch_bounds__(256) k_mf_small_fused(const MfFront *__restric
                                                         const int *__restrict__ aseg, const int *__restrict__ avid,
                                                         const T *__restrict__ val, T *__restrict__ F)
 {
-    __shared__ T sF[kSmallLds / sizeof(T)];
+    // the staging buffer holds a whole front where (16 NI)^2 entries fit in
+    // kSmallLds (smaller tiles then leave room for more workgroups per CU),
+    // else rounds of cr rows
+    constexpr int kSF = (int)(kSmallLds / sizeof(T)) < 256 * NI * NI ? (int)(kSmallLds / sizeof(T)) : 256 * NI * NI;
+    __shared__ T sF[kSF];
     __shared__ T prow[2][16 * NI], pcol[2][16 * NI];
     const MfFront f = fr[f0 + blockIdx.x];
     const int nf = f.nf, ns = f.ns;
     const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-    const int cr = min(nf, (int)(kSmallLds / sizeof(T)) / nf);
+    const int cr = min(nf, kSF / nf);
     T a[NI][NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i)
