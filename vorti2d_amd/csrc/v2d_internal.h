@@ -386,6 +386,7 @@ struct MfLevel {
     int64_t r0, r1;          // rows table range (every front row)
     int64_t g0, g1;          // forward row blocks (all front rows)
     int64_t b0, b1;          // backward row blocks (pivot rows)
+    int ntg, ntb;            // threads per forward / backward row block (64, 128 or 256)
     int64_t c0, c1;          // transposed column blocks (64 columns of all front columns)
     int64_t p0, p1;          // transposed column blocks of the pivot columns
     int64_t a0, a1;          // assembly positions

@@ -65,7 +65,8 @@ __host__ __device__ inline int row_lanes(int ncol)
     while (l < 64 && 4 * l < ncol) l <<= 1;
     return l;
 }
-__host__ __device__ inline int block_rows(int ncol) { return 4 * (256 / row_lanes(ncol)); }
+// rows of a row-product block of nt threads (four rows per lane group)
+__host__ __device__ inline int block_rows(int ncol, int nt) { return 4 * (nt / row_lanes(ncol)); }
 
 // Rows of one assembly block of a big front of row stride ld (a multiple of
 // 4): about kAsmItems items of 4 columns.
@@ -484,11 +485,22 @@ int mf_analyse(MfPattern &P, int imax, int jmax, const uint8_t *flags)
         L.nfmax_small = L.nfmax_big = L.npmax_big = 0;
         for (int q = 0; q <= kSmallMaxNI; ++q) L.small_ni[q] = L.f0;
         L.pool = 0;
+        // The row products of a height take the smallest block (64, 128 or
+        // 256 threads) that holds every front's rows at once (rows past a
+        // front's end would be read again and dropped); 256 and several
+        // blocks per front where none does.
+        L.ntg = L.ntb = 64;
+        for (int f = L.f0; f < L.f1; ++f) {
+            const MfFront &F = P.fr[f];
+            while (L.ntg < 256 && block_rows(F.nsp, L.ntg) < F.nf) L.ntg <<= 1;
+            while (L.ntb < 256 && block_rows(round_up(F.nb, kMfAlign), L.ntb) < F.ns) L.ntb <<= 1;
+        }
         for (int f = L.f0; f < L.f1; ++f) {
             const MfFront &F = P.fr[f];
             for (int r = 0; r < F.nf; ++r) P.rows.push_back(make_int2(f, r));
-            for (int r = 0; r < F.nf; r += block_rows(F.nsp)) P.gemv.push_back(make_int2(f, r));
-            for (int r = 0; r < F.ns; r += block_rows(round_up(F.nb, kMfAlign))) P.bwd.push_back(make_int2(f, r));
+            for (int r = 0; r < F.nf; r += block_rows(F.nsp, L.ntg)) P.gemv.push_back(make_int2(f, r));
+            for (int r = 0; r < F.ns; r += block_rows(round_up(F.nb, kMfAlign), L.ntb))
+                P.bwd.push_back(make_int2(f, r));
             for (int c = 0; c < F.nf; c += 64) P.colt.push_back(make_int2(f, c));
             for (int c = 0; c < F.ns; c += 64) P.pcolt.push_back(make_int2(f, c));
             if (F.big) {
@@ -1026,7 +1038,7 @@ __device__ __forceinline__ double wave_sum64(double v)
 
 // Row products of one front against a vector staged in LDS: out(r, p, s)
 // with s = sum_c F[r][c0 + c] xs[c] in FP64, for the rows row0 + q G + g
-// (q < 4) of a block below rend, G = 256 / L rows per pass, L =
+// (q < 4) of a block of NT threads below rend, G = NT / L rows per pass, L =
 // row_lanes(ncol) lanes per row (16-byte vectors, summed across the group by
 // shuffles).  The four rows of a lane group are read together: their first
 // vectors and pre(r), the per-row operands out() needs, are loaded before
@@ -1035,13 +1047,13 @@ __device__ __forceinline__ double wave_sum64(double v)
 // the branch joins, which would serialise the rows), so a block waits for one
 // memory round trip before its sums.  ncol and c0 are multiples of 4 (the
 // vectors stay aligned); rows past rend read row rend - 1 and are dropped.
-template <typename T, typename Stage, typename Pre, typename Out>
+template <typename T, int NT, typename Stage, typename Pre, typename Out>
 __device__ __forceinline__ void mf_rows(const MfFront &f, const T *F, int row0, int rend, int c0, int ncol,
                                         const T *xs, Stage &&stage, Pre &&pre, Out &&out)
 {
     using V = typename V16<T>::type;
     constexpr int W = V16<T>::w;
-    const int L = row_lanes(ncol), G = 256 / L;
+    const int L = row_lanes(ncol), G = NT / L;
     const int l = threadIdx.x & (L - 1), g = threadIdx.x / L;
     const int nv = ncol / W;
     const int cb = nv > 0 ? c0 : 0;              // no columns: read the row start
@@ -1078,19 +1090,19 @@ __device__ __forceinline__ void mf_rows(const MfFront &f, const T *F, int row0, 
 
 // Forward front products: u = F[:, 0:nsp] v_s (v gathered by k_mf_fgather);
 // z = u on the pivots (Zf), v_b + u on the couplings (Vf, read by the parent).
-template <typename T>
-__global__ void __launch_bounds__(256) k_mf_fgemv(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
-                                                  int64_t g0, const T *__restrict__ F, T *__restrict__ Vf,
-                                                  T *__restrict__ Zf)
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_mf_fgemv(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
+                                                 int64_t g0, const T *__restrict__ F, T *__restrict__ Vf,
+                                                 T *__restrict__ Zf)
 {
     extern __shared__ unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
     const int2 fb = blk[g0 + blockIdx.x];
     const MfFront f = fr[fb.x];
-    mf_rows<T>(
+    mf_rows<T, NT>(
         f, F, fb.y, f.nf, 0, f.nsp, xs,
         [&] {
-            for (int a = threadIdx.x; a < f.nsp; a += blockDim.x) xs[a] = Vf[f.vo + a];
+            for (int a = threadIdx.x; a < f.nsp; a += NT) xs[a] = Vf[f.vo + a];
         },
         [&](int r) { return Vf[f.vo + r]; },
         [&](int r, T vb, double acc) {
@@ -1107,10 +1119,10 @@ struct ZU {
 };
 
 // Backward row products: x_s = z - W x_b, x_b gathered from the solution.
-template <typename T>
-__global__ void __launch_bounds__(256) k_mf_bwd(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
-                                                int64_t b0, const T *__restrict__ F, const T *__restrict__ Zf,
-                                                const int *__restrict__ fu, T *__restrict__ x)
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_mf_bwd(const MfFront *__restrict__ fr, const int2 *__restrict__ blk,
+                                               int64_t b0, const T *__restrict__ F, const T *__restrict__ Zf,
+                                               const int *__restrict__ fu, T *__restrict__ x)
 {
     extern __shared__ unsigned char smem[];
     T *xs = reinterpret_cast<T *>(smem);
@@ -1119,10 +1131,10 @@ __global__ void __launch_bounds__(256) k_mf_bwd(const MfFront *__restrict__ fr, 
     const int nbp = round_up(f.nb, kMfAlign);
     // the W block is read up to the 4-rounded coupling count: those columns
     // lie inside the row stride and meet zeros in xs
-    mf_rows<T>(
+    mf_rows<T, NT>(
         f, F, fb.y, f.ns, f.nsp, nbp, xs,
         [&] {
-            for (int q = threadIdx.x; q < nbp; q += blockDim.x) xs[q] = q < f.nb ? x[fu[f.uo + f.nsp + q]] : T(0);
+            for (int q = threadIdx.x; q < nbp; q += NT) xs[q] = q < f.nb ? x[fu[f.uo + f.nsp + q]] : T(0);
         },
         [&](int r) { return ZU<T>{Zf[f.zo + r], fu[f.uo + r]}; },
         [&](int r, const ZU<T> &zu, double acc) { x[zu.u] = (T)((double)zu.z - acc); });
@@ -1436,8 +1448,17 @@ int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hip
         }
         if (!trans && L.g1 > L.g0) {
             MF_BEGIN();
-            hipLaunchKernelGGL(k_mf_fgemv<T>, dim3((unsigned)(L.g1 - L.g0)), dim3(256), sizeof(T) * nspmax, s,
-                               P.d_fr, P.d_gemv, L.g0, W.F, W.Vf, W.Zf);
+            const dim3 g((unsigned)(L.g1 - L.g0));
+            const size_t lds = sizeof(T) * nspmax;
+            if (L.ntg == 64)
+                hipLaunchKernelGGL((k_mf_fgemv<T, 64>), g, dim3(64), lds, s, P.d_fr, P.d_gemv, L.g0, W.F, W.Vf, W.Zf);
+            else if (L.ntg == 128)
+                hipLaunchKernelGGL((k_mf_fgemv<T, 128>), g, dim3(128), lds, s, P.d_fr, P.d_gemv, L.g0, W.F, W.Vf,
+                                   W.Zf);
+
+            else
+                hipLaunchKernelGGL((k_mf_fgemv<T, 256>), g, dim3(256), lds, s, P.d_fr, P.d_gemv, L.g0, W.F, W.Vf,
+                                   W.Zf);
             MF_END(kFwd, 0.0, fb);
         }
         if (trans && L.c1 > L.c0)
@@ -1457,8 +1478,17 @@ int mf_solve(const MfPattern &P, MfWork<T> &W, const T *b, T *x, bool trans, hip
         }
         if (!trans && L.b1 > L.b0) {
             MF_BEGIN();
-            hipLaunchKernelGGL(k_mf_bwd<T>, dim3((unsigned)(L.b1 - L.b0)), dim3(256), sizeof(T) * std::max(nbmax, 1),
-                               s, P.d_fr, P.d_bwd, L.b0, W.F, W.Zf, P.d_fu, x);
+            const dim3 g((unsigned)(L.b1 - L.b0));
+            const size_t lds = sizeof(T) * std::max(nbmax, 1);
+            if (L.ntb == 64)
+                hipLaunchKernelGGL((k_mf_bwd<T, 64>), g, dim3(64), lds, s, P.d_fr, P.d_bwd, L.b0, W.F, W.Zf, P.d_fu, x);
+            else if (L.ntb == 128)
+                hipLaunchKernelGGL((k_mf_bwd<T, 128>), g, dim3(128), lds, s, P.d_fr, P.d_bwd, L.b0, W.F, W.Zf, P.d_fu,
+                                   x);
+
+            else
+                hipLaunchKernelGGL((k_mf_bwd<T, 256>), g, dim3(256), lds, s, P.d_fr, P.d_bwd, L.b0, W.F, W.Zf, P.d_fu,
+                                   x);
             MF_END(kBwd, 0.0, bb);
         }
         if (trans && L.p1 > L.p0)
