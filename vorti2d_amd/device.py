@@ -197,7 +197,7 @@ class DeviceContext:
                   "gj_light": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 1>",
                   "gj_heavy": "k_gj_step_mfma_b<{T}, TableBatch<{T}>, 2>",
                   "front1": "k_mf_front1_*<{T}, *>",
-                  "gather": "k_mf_fgather<{T}>", "fwd": "k_mf_fgemv<{T}>", "bwd": "k_mf_bwd<{T}>"}
+                  "gather": "k_mf_fgather<{T}>", "fwd": "k_mf_fgemv<{T}, *>", "bwd": "k_mf_bwd<{T}, *>"}
 
     def profile_mf(self) -> dict:
         """Factorise the last assembled Jacobian and solve once with the
