@@ -390,9 +390,12 @@ def main():
                     help="mixed: Eisenstat-Walker GMRES tolerance capped at this value (0: off)")
     ap.add_argument("--no-forcing-target", dest="forcing_target", action="store_false",
                     help="mixed: no residual-target safeguard on the forcing term")
-    ap.add_argument("--no-graph", dest="graph", action="store_false",
-                    help="mixed: launch the preconditioner-apply kernels one by one instead of replaying a "
-                         "captured graph (same kernels; for kernel-trace profilers)")
+    ap.add_argument("--graph", default="iteration", choices=["off", "apply", "iteration"],
+                    help="mixed: GMRES launches -- the preconditioner apply replayed from a captured "
+                         "graph (apply), whole iterations replayed from graphs captured per basis "
+                         "index (iteration), or every kernel launched eagerly (off); same kernels")
+    ap.add_argument("--no-graph", dest="graph", action="store_const", const="off",
+                    help="mixed: same as --graph off (for kernel-trace profilers)")
     ap.add_argument("--solver", default="multifrontal", choices=["multifrontal", "block_lu"],
                     help="linear solver: nested-dissection multifrontal LU (default) or the "
                          "block-tridiagonal LU")
@@ -482,7 +485,8 @@ def main():
                                         if a.forcing > 0 and a.forcing_target else "")
                                      + ", refactor "
                                      + ("on GMRES miss" if a.refactor_every == 0 else
-                                        f"every {a.refactor_every}") + ")")},
+                                        f"every {a.refactor_every}") + ")"),
+                   "gmres_graph": a.graph},
         "newton_iters": last["iters"],
         "gmres_iters": last["gmres"],
         "factorizations_per_solve": tm["factorizations"] / max(len(mx["times"]), 1),

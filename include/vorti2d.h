@@ -267,13 +267,20 @@ int v2d_mf_destroy(v2d_mf_analysis *a);
  * cheap, but load-dependent).  No counterpart in the reference. */
 enum { V2D_REFACTOR_FIXED = 0, V2D_REFACTOR_MEASURED = 1 };
 int v2d_ctx_set_refactor_policy(v2d_ctx *ctx, int policy);
-/* Mixed precision: run the FP32 block-LU apply of every GMRES iteration as one
- * replay of a HIP graph captured over fixed device buffers (enable != 0, the
- * default; per-level sweeps only), instead of launching its 2 (levels - 1)
- * kernels one by one.  Same kernels, same results; it takes the launches off
- * the host thread, which matters when many contexts run at once.  No
- * counterpart in the reference. */
-int v2d_ctx_set_graph(v2d_ctx *ctx, int enable);
+/* Mixed precision: how GMRES launches its device work.  V2D_GRAPH_APPLY (the
+ * default): the FP32 preconditioner apply of every iteration is one replay of
+ * a HIP graph captured over fixed device buffers (multifrontal LU, or the block
+ * LU's per-level sweeps), instead of its launches one by one.
+ * V2D_GRAPH_ITERATION: the whole iteration -- apply, SpMV, both Gram-Schmidt
+ * rounds, the norm, the scaling of the new basis vector and the copies of the
+ * Hessenberg column to pinned host memory -- is one replay of a graph captured
+ * per basis index on first use (the block LU's persistent sweep falls back to
+ * eager launches).  V2D_GRAPH_OFF: every kernel launched eagerly.  Same
+ * kernels in the same order in every mode, so the same results bit for bit;
+ * graphs take the launches off the host thread.  Changing the mode drops the
+ * captured graphs.  No counterpart in the reference. */
+enum { V2D_GRAPH_OFF = 0, V2D_GRAPH_APPLY = 1, V2D_GRAPH_ITERATION = 2 };
+int v2d_ctx_set_graph(v2d_ctx *ctx, int mode);
 /* Factorise the last assembled Jacobian once with every Gauss-Jordan update
  * launch bracketed by HIP events on the context's stream (no counterpart in the
  * reference; measurement only).  out[5] = {update launches, summed update

@@ -28,8 +28,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["newton", "factor", "assembly", "copy"])
     ap.add_argument("--solver", default="multifrontal", choices=["multifrontal", "block_lu"])
-    ap.add_argument("--no-graph", dest="graph", action="store_false",
-                    help="newton, mixed: launch the preconditioner-apply kernels one by one "
+    ap.add_argument("--graph", default="apply", choices=["off", "apply", "iteration"],
+                    help="newton, mixed: GMRES launch mode (captured apply, captured iterations or eager)")
+    ap.add_argument("--no-graph", dest="graph", action="store_const", const="off",
+                    help="newton, mixed: launch every kernel eagerly "
                          "(kernel tracing does not follow captured-graph replays)")
     ap.add_argument("--gmres-max", type=int, default=10,
                     help="newton, mixed: GMRES budget on a stored factorisation (the benchmark's)")

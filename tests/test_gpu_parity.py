@@ -380,34 +380,36 @@ def test_forcing_residual_target(v, C, oracle_mod):
 @pytest.mark.parametrize("solver", ["multifrontal", "block_lu"])
 def test_graph_replayed_gmres_equals_eager(v, C, oracle_mod, solver):
     """The mixed Newton step with the preconditioner apply (either linear
-    solver) replayed from a captured HIP graph equals the step with the
-    apply's kernels launched one by one, bit for bit, over several steps (each
-    replay followed by eager work of the step: SpMV, orthogonalisation, host
-    synchronisation), across refactorisations, and after switching graphs off
-    and on again."""
+    solver) replayed from a captured HIP graph, and with whole GMRES
+    iterations (apply, SpMV, orthogonalisation, norm, read-backs) replayed from
+    graphs captured per basis index, equals the step with every kernel
+    launched one by one, bit for bit, over several steps (each replay followed
+    by the host's Givens update and eager work of the step), across
+    refactorisations, and after switching graphs off and on again."""
     imax, jmax = 97, 49
     xg, yg = oracle_mod.generate_annulus(imax, jmax)
     hist = {}
-    for mode in ("eager", "graph", "toggled"):
+    modes = {"eager": "off", "graph": "apply", "iteration": "iteration", "toggled": "iteration"}
+    for mode in modes:
         ctx = v.DeviceContext(xg, yg, 0, 1.0, 0.0, "mixed")
         ctx.set_linear_solver(solver)
         # refactor every step: the iteration counts then do not depend on
         # measured timings, and the graph is replayed across refactorisations
         ctx.set_solver(refactor_every=1, gmres_rtol=1e-8, gmres_max_iter=30)
-        ctx.set_graph(mode != "eager")
+        ctx.set_graph(modes[mode])
         ctx.init_state()
         out = []
         for k in range(5):
             if mode == "toggled" and k == 2:
                 ctx.set_graph(False)
-                ctx.set_graph(True)
+                ctx.set_graph("iteration")
             res, rn, its = ctx.newton_step(40.0)
             out.append((res, rn, its))
         out.append(tuple(ctx.get_state(("psi", "ome"))))
         hist[mode] = out
         ctx.close()
     say(f"  eager {[h[:3] for h in hist['eager'][:5]]}")
-    for mode in ("graph", "toggled"):
+    for mode in ("graph", "iteration", "toggled"):
         for a, b in zip(hist["eager"][:5], hist[mode][:5]):
             assert a == b, (mode, a, b)
         assert np.array_equal(hist["eager"][5][0], hist[mode][5][0])

@@ -87,13 +87,14 @@ class CaseResult:
 
 
 def solve_case(case: Case, precision="mixed", tol=1e-10, max_iter=50, forcing=0.1, ref_length=1.0,
-               adjoint=False, rms_tol=1e-10, keep_state=False):
+               adjoint=False, rms_tol=1e-10, keep_state=False, graph="apply"):
     """One steady solve (and, with adjoint=True, dCd/dalpha per degree from an
     FP64 context at the converged state).  Stopping rules: res = ||delta||_2 /
     (2N) <= tol (vorti2d/solver.py:141-144) and, unless rms_tol is None, the
     final FP64 residual ||R||_2 / sqrt(2N) < rms_tol; the mixed GMRES forcing
     aims at that residual (eta_k >= 0.5 rms_tol sqrt(2N) / ||R_k||).
-    keep_state: the result also holds the converged psi, omega."""
+    keep_state: the result also holds the converged psi, omega.
+    graph: the mixed GMRES launch mode (DeviceContext.set_graph)."""
     xg, yg = elliptic_grid(case.imax, case.jmax, case.aspect)
     a = math.radians(case.alpha_deg)
     t0 = time.perf_counter()
@@ -103,6 +104,7 @@ def solve_case(case: Case, precision="mixed", tol=1e-10, max_iter=50, forcing=0.
         n2 = 2 * case.imax * case.jmax
         ctx.set_forcing(forcing, (rms_tol if rms_tol is not None else tol * math.sqrt(n2))
                         * math.sqrt(n2))
+        ctx.set_graph(graph)
     ctx.init_state()
     t1 = time.perf_counter()
     r = ctx.steady_newton(case.re, tol=tol, rms_tol=rms_tol, max_iter=max_iter, record=True)

@@ -206,7 +206,7 @@ int apply_pc(Ctx *c, const float *v, float *z)
 {
     const int64_t n = 2 * c->N;
     const bool mf = c->solver == V2D_SOLVER_MULTIFRONTAL;
-    if (!c->use_graph || (!mf && c->wf.sweep_kind == V2D_SWEEP_PERSISTENT) || !c->ga_in || !c->ga_out)
+    if (c->graph_mode == V2D_GRAPH_OFF || (!mf && c->wf.sweep_kind == V2D_SWEEP_PERSISTENT) || !c->ga_in || !c->ga_out)
         return lin_solve<float>(c, v, z, false);
     if (!c->age) {
         V2D_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
@@ -321,6 +321,67 @@ std::vector<double> GmresHost::solution() const
     return yv;
 }
 
+static int gmres_enqueue(Ctx *c, int j, bool captured)
+{
+    const int64_t n = 2 * c->N;
+    const float *val = static_cast<const float *>(c->val);        // operator (the preconditioner: apply_pc)
+    const unsigned nb = std::min<unsigned>(1024, grid_for(n));
+    float *vj = c->gV + j * n, *zj = c->gZ + j * n, *w = c->gV + (j + 1) * n;
+    if (captured ? lin_solve<float>(c, vj, zj, false) : apply_pc(c, vj, zj)) return -1;
+    hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
+                       c->jmax, c->flags, val, zj, w);
+    V2D_LAUNCH();
+    // The columns and ||w||^2 go to pinned host memory without a wait, and
+    // v_{j+1} = w / ||w|| is formed on the device before the host reads
+    // them (unused when the host then stops), so an iteration waits for
+    // the host once.
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, j + 1, c->gV, n, w,
+                           c->partial);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, j + 1,
+                           c->partial, c->hbuf);
+        hipLaunchKernelGGL(k_multiaxpy<float>, dim3(grid_for(n)), dim3(256), 0, c->s, n, j + 1,
+                           c->gV, n, c->hbuf, w);
+        V2D_LAUNCH();
+        V2D_HIP(hipMemcpyAsync(c->gm_host + pass * Ctx::kGmHost, c->hbuf, sizeof(double) * (j + 1),
+                               hipMemcpyDeviceToHost, c->s));
+    }
+    hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, 1, w, n, w, c->partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, 1, c->partial,
+                       c->red);
+    V2D_LAUNCH();
+    V2D_HIP(hipMemcpyAsync(c->gm_host + 2 * Ctx::kGmHost, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, w, c->red, 2,
+                       w);
+    V2D_LAUNCH();
+    return 0;
+}
+
+// Iteration j's device work as one replay of a graph captured on first use
+// (kernel and device-to-pinned-host copy nodes only; every buffer it touches
+// lives as long as the context and the factorisation is read through fixed
+// pointers, so the graph stays valid across refactorisations).
+static int gmres_replay(Ctx *c, int j)
+{
+    if ((int)c->itg.size() <= j) c->itg.resize(j + 1, nullptr);
+    if (!c->itg[j]) {
+        V2D_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        const int rc = gmres_enqueue(c, j, true);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->s, &g);
+        hipGraphExec_t e = nullptr;
+        const hipError_t ei = (!rc && ec == hipSuccess) ? hipGraphInstantiate(&e, g, nullptr, nullptr, 0) : ec;
+        if (g) (void)hipGraphDestroy(g);
+        if (rc || ei != hipSuccess) {
+            if (!rc) set_error(std::string("graph capture of a GMRES iteration failed: ") + hipGetErrorString(ei));
+            return -1;
+        }
+        c->itg[j] = e;
+    }
+    V2D_HIP(hipGraphLaunch(c->itg[j], c->s));
+    return 0;
+}
+
 // Right-preconditioned GMRES with the block LU M, classical Gram-Schmidt with
 // one re-orthogonalisation (fused multi-dot / multi-axpy), Givens rotations on
 // the host (GmresHost).  Solves A x = b for unit-norm b (FP32 vectors, FP64 dot
@@ -330,42 +391,15 @@ std::vector<double> GmresHost::solution() const
 static int gmres_f32(Ctx *c, int cap, int budget, int *iters, double *relres)
 {
     const int64_t n = 2 * c->N;
-    const float *val = static_cast<const float *>(c->val);        // operator (the preconditioner: apply_pc)
-    const unsigned nb = std::min<unsigned>(1024, grid_for(n));
+    const bool whole = c->graph_mode == V2D_GRAPH_ITERATION &&
+                       (c->solver == V2D_SOLVER_MULTIFRONTAL || c->wf.sweep_kind != V2D_SWEEP_PERSISTENT);
     GmresHost G;
     G.begin(cap, budget);
     // v0 = b (unit norm already) -> gV[0]
     V2D_HIP(hipMemcpyAsync(c->gV, c->gb, sizeof(float) * n, hipMemcpyDeviceToDevice, c->s));
     for (;;) {
         const int j = G.j;
-        float *vj = c->gV + j * n, *zj = c->gZ + j * n, *w = c->gV + (j + 1) * n;
-        if (apply_pc(c, vj, zj)) return -1;
-        hipLaunchKernelGGL((k_spmv<float, float>), dim3(grid_for(c->N)), dim3(256), 0, c->s, c->imax,
-                           c->jmax, c->flags, val, zj, w);
-        V2D_LAUNCH();
-        // The columns and ||w||^2 go to pinned host memory without a wait, and
-        // v_{j+1} = w / ||w|| is formed on the device before the host reads
-        // them (unused when the host then stops), so an iteration waits for
-        // the host once.
-        for (int pass = 0; pass < 2; ++pass) {
-            hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, j + 1, c->gV, n, w,
-                               c->partial);
-            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, j + 1,
-                               c->partial, c->hbuf);
-            hipLaunchKernelGGL(k_multiaxpy<float>, dim3(grid_for(n)), dim3(256), 0, c->s, n, j + 1,
-                               c->gV, n, c->hbuf, w);
-            V2D_LAUNCH();
-            V2D_HIP(hipMemcpyAsync(c->gm_host + pass * Ctx::kGmHost, c->hbuf, sizeof(double) * (j + 1),
-                                   hipMemcpyDeviceToHost, c->s));
-        }
-        hipLaunchKernelGGL(k_multidot<float>, dim3(nb), dim3(256), 0, c->s, n, 1, w, n, w, c->partial);
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, c->s, (int)nb, 1, c->partial,
-                           c->red);
-        V2D_LAUNCH();
-        V2D_HIP(hipMemcpyAsync(c->gm_host + 2 * Ctx::kGmHost, c->red, sizeof(double), hipMemcpyDeviceToHost, c->s));
-        hipLaunchKernelGGL((k_scale_copy<float, float>), dim3(grid_for(n)), dim3(256), 0, c->s, n, w, c->red, 2,
-                           w);
-        V2D_LAUNCH();
+        if (whole ? gmres_replay(c, j) : gmres_enqueue(c, j, false)) return -1;
         V2D_HIP(hipStreamSynchronize(c->s));
         if (sweep_failed(c->wf.err_host)) return -1;
         std::copy(c->gm_host, c->gm_host + j + 1, G.h.begin());
@@ -911,10 +945,18 @@ int v2d_ctx_set_sweep_spin_limit(v2d_ctx *ctx, int polls)
     return 0;
 }
 
-int v2d_ctx_set_graph(v2d_ctx *ctx, int enable)
+int v2d_ctx_set_graph(v2d_ctx *ctx, int mode)
 {
-    ctx->c->use_graph = enable != 0;
-    if (!enable) ctx->c->drop_graph();
+    if (mode != V2D_GRAPH_OFF && mode != V2D_GRAPH_APPLY && mode != V2D_GRAPH_ITERATION) {
+        set_error("graph mode must be V2D_GRAPH_OFF (0), V2D_GRAPH_APPLY (1) or V2D_GRAPH_ITERATION (2)");
+        return -1;
+    }
+    Ctx *c = ctx->c.get();
+    if (mode != c->graph_mode) {
+        V2D_HIP(hipStreamSynchronize(c->s));
+        c->drop_graph();
+        c->graph_mode = mode;
+    }
     return 0;
 }
 

@@ -84,19 +84,26 @@ struct Ctx {
     double gm_rtol = 1e-6;
     int refactor_every = 1;
     float *gV = nullptr, *gZ = nullptr, *gb = nullptr;
-    // the FP32 block-LU apply as a captured graph over fixed buffers
-    // (ga_in -> ga_out), replayed by GMRES; built on first use, dropped when
-    // the pattern or the sweep kernel changes
-    bool use_graph = true;
+    // captured graphs, built on first use and dropped when the pattern, the
+    // solver or the sweep kernel changes: the FP32 preconditioner apply over
+    // fixed buffers (ga_in -> ga_out), replayed by every GMRES iteration
+    // (V2D_GRAPH_APPLY); or every GMRES iteration (apply, SpMV,
+    // Gram-Schmidt, norm and read-backs) as one replay of a graph captured per
+    // basis index (V2D_GRAPH_ITERATION, itg[j]); V2D_GRAPH_OFF launches eagerly
+    int graph_mode = V2D_GRAPH_APPLY;
     float *ga_in = nullptr, *ga_out = nullptr;
     hipGraph_t ag = nullptr;
     hipGraphExec_t age = nullptr;
+    std::vector<hipGraphExec_t> itg;
     void drop_graph()
     {
         if (age) (void)hipGraphExecDestroy(age);
         if (ag) (void)hipGraphDestroy(ag);
         age = nullptr;
         ag = nullptr;
+        for (hipGraphExec_t e : itg)
+            if (e) (void)hipGraphExecDestroy(e);
+        itg.clear();
     }
     double *gxd = nullptr;
     // reverse sweep of the unsteady (BDF2) adjoint: the adjoint vectors of the

@@ -219,11 +219,22 @@ class DeviceContext:
         check(lib.v2d_ctx_mf_fronts(self._h, ptr(out)))
         return out
 
-    def set_graph(self, enable: bool):
-        """Mixed precision: replay the block-LU apply of each GMRES iteration
-        as one captured HIP graph (the default) or launch its kernels one by
-        one."""
-        check(lib.v2d_ctx_set_graph(self._h, 1 if enable else 0))
+    GRAPH_MODES = {"off": 0, "apply": 1, "iteration": 2}
+
+    def set_graph(self, mode):
+        """Mixed precision: how GMRES launches its device work. "apply" (or
+        True, the default): the preconditioner apply of each iteration is one
+        captured HIP graph replay; "iteration": the whole iteration (apply,
+        SpMV, Gram-Schmidt, norm, read-backs) is one replay of a graph captured
+        per basis index; "off" (or False): every kernel launched eagerly.
+        Results are the same bit for bit in every mode."""
+        if mode is True:
+            mode = "apply"
+        elif mode is False:
+            mode = "off"
+        if mode not in self.GRAPH_MODES:
+            raise ValueError(f"graph mode must be one of {sorted(self.GRAPH_MODES)}, True or False")
+        check(lib.v2d_ctx_set_graph(self._h, self.GRAPH_MODES[mode]))
 
     def autotune_update_kernel(self, re: float, kinds=("mfma64", "valu64")) -> dict:
         """Time one block-LU factorisation of the current state's Jacobian with
