@@ -716,21 +716,40 @@ def test_unsteady_shedding_mixed_vs_fp64(v):
 
 def test_unsteady_c2_reference_grid_strouhal(v):
     """C2 on the reference's own grid, generate_cylinder(181, 181): Re 100, BDF2
-    dt 0.2 to t = 80, rotation kick 0.5 until t = 2, outflow far field, mixed
-    precision.  The reference publishes St 0.1535 for Re 100
+    dt 0.2 to t = 80, rotation kick 0.5 until t = 2, outflow far field, in FP64
+    and mixed precision, against the oracle's run of the same 401 levels
+    (tests/golden/c2_oracle_cyl181.npz: reference Fortran assembly and exact
+    SuperLU solves, forces by the pinned force integral): Cd, Cl, Cm within 1e-6
+    at every level, the final psi, omega within 1e-8 at the stored nodes, and
+    the same Strouhal number.  The reference publishes St 0.1535 for Re 100
     (docs/verification.rst:181-204, run length and grid not stated) and
     "St ~ 0.16, mean Cd ~ 1.31" for this setting on a 131 x 131 grid to t = 50
     (examples/cylinder_unsteady.py:9); the saturated cycle here gives St 0.1603
-    and mean Cd 1.315 (FP64 identical to the printed digits)."""
+    and mean Cd 1.315."""
     from lco import strouhal
+    ref = golden("c2_oracle_cyl181")
+    st_ref = strouhal(ref["t"], ref["cl"])
     xg, yg = v.generate_cylinder(181, 181, 0.5, 50.0)
-    with tempfile.TemporaryDirectory() as tmp:
-        say("unsteady C2 on generate_cylinder(181, 181), mixed")
-        _, d = _unsteady_c2(v, "mixed", xg, yg, tmp, t_end=80.0)
-    st = strouhal(d["t"], d["cl"])
-    cd_mean = float(d["cd"][st["i0"]:].mean())
-    say(f"  St {st['st']:.4f} ({st['n_periods']} periods from t={d['t'][st['i0']]:.1f}), "
-        f"mean Cd {cd_mean:.4f}, Cl amplitude {st['amp']:.4f}")
-    assert st["n_periods"] >= 4
-    assert abs(st["st"] - 0.1535) <= 0.05 * 0.1535
-    assert abs(cd_mean - 1.31) <= 0.02 * 1.31
+    for prec in ("fp64", "mixed"):
+        with tempfile.TemporaryDirectory() as tmp:
+            say(f"unsteady C2 on generate_cylinder(181, 181), {prec}")
+            s, d = _unsteady_c2(v, prec, xg, yg, tmp, t_end=80.0)
+        assert len(d["t"]) == len(ref["t"]) == 401
+        assert np.allclose(d["t"], ref["t"], rtol=0, atol=1e-9)   # forces.csv holds %.10e
+        dev = {k: float(np.abs(d[k] - ref[k]).max()) for k in ("cd", "cl", "cm")}
+        psi, ome = s.psi, s.ome
+        dpsi = float(np.abs(psi[ref["idx"]] - ref["psi_s"]).max())
+        dome = float(np.abs(ome[ref["idx"]] - ref["ome_s"]).max())
+        st = strouhal(d["t"], d["cl"])
+        cd_mean = float(d["cd"][st["i0"]:].mean())
+        say(f"  St {st['st']:.4f} (oracle {st_ref['st']:.4f}; {st['n_periods']} periods from "
+            f"t={d['t'][st['i0']]:.1f}), mean Cd {cd_mean:.4f}, Cl amplitude {st['amp']:.4f}; vs the "
+            f"oracle over 401 levels: max |dCd| {dev['cd']:.1e}, |dCl| {dev['cl']:.1e}, "
+            f"|dCm| {dev['cm']:.1e}; final |dpsi| {dpsi:.1e}, |dome| {dome:.1e}")
+        assert max(dev.values()) <= 1e-6, (prec, dev)
+        assert dpsi <= 1e-8 and dome <= 1e-8, (prec, dpsi, dome)
+        assert abs(st["st"] - st_ref["st"]) <= 1e-6
+        assert abs(np.linalg.norm(psi) - float(ref["psi_norm"])) <= 1e-8 * float(ref["psi_norm"])
+        assert st["n_periods"] >= 4
+        assert abs(st["st"] - 0.1535) <= 0.05 * 0.1535
+        assert abs(cd_mean - 1.31) <= 0.02 * 1.31

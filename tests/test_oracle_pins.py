@@ -103,3 +103,32 @@ def test_published_steady_drag_181(C, oracle_mod, re, cd_pub):
     fc = oracle_mod.force_coeffs(imax, jmax, re, d, d, m[2], m[3], m[6], m[7], m[8], m[9], r.ome)
     assert abs(fc["cd"] - cd_pub) < 1e-3, fc["cd"]
     assert abs(fc["cl"]) < 1e-6
+
+
+def test_c2_fixture_first_level_and_published_values(oracle_mod, C):
+    """The unsteady C2 fixture (tests/golden/make_c2_oracle.py: the reference
+    Fortran assembly with SuperLU over 401 BDF2 levels on
+    generate_cylinder(181, 181), Re 100) is reproduced at its first level by
+    the oracle's C restatement, covers t = 0 .. 80 in steps of 0.2 with every
+    level converged, and its saturated cycle lies near the reference's
+    published Re 100 values (St 0.1535, docs/verification.rst:181-204; mean Cd
+    about 1.31, examples/cylinder_unsteady.py:9)."""
+    from lco import strouhal
+    ref = golden("c2_oracle_cyl181")
+    assert np.allclose(ref["t"], 0.2 * np.arange(401), rtol=0, atol=1e-12)
+    assert (ref["res"] <= 1e-10).all() and (ref["newton_iters"] < 100).all()
+    imax = jmax = 181
+    xg, yg = oracle_mod.generate_cylinder(imax, jmax)
+    m = C.compute_metrics(1 / 180, 1 / 180, xg, yg)
+    psi, ome = oracle_mod.initial_state(imax, jmax, m[8], m[9])
+    r = oracle_mod.newton_solve(C, imax, jmax, m, 100.0, inv2dt=2.5, urot=0.5, ff_bc=1,
+                                psi=psi, ome=ome, tol=1e-10)
+    f = oracle_mod.force_coeffs(imax, jmax, 100.0, 1 / 180, 1 / 180, m[2], m[3], m[6], m[7],
+                                m[8], m[9], r.ome)
+    assert r.iters == ref["newton_iters"][0]
+    for k in ("cd", "cl", "cm"):
+        assert abs(f[k] - ref[k][0]) <= 1e-10 * max(1.0, abs(f[k])), k
+    st = strouhal(ref["t"], ref["cl"])
+    assert st["n_periods"] >= 4
+    assert abs(st["st"] - 0.1535) <= 0.05 * 0.1535
+    assert abs(ref["cd"][st["i0"]:].mean() - 1.31) <= 0.02 * 1.31
