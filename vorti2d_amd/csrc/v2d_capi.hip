@@ -321,6 +321,12 @@ std::vector<double> GmresHost::solution() const
     return yv;
 }
 
+// Enqueues GMRES iteration j's device work on the context stream: z_j = M^-1
+// v_j, w = A z_j, two classical Gram-Schmidt rounds of w against v_0 .. v_j,
+// ||w||^2 and v_{j+1} = w / ||w||, with both Gram-Schmidt columns and ||w||^2
+// copied to pinned host memory.  captured: the enqueue is being recorded into
+// a graph, so the apply is issued directly (apply_pc would nest its own
+// capture and copy through ga_in / ga_out).
 static int gmres_enqueue(Ctx *c, int j, bool captured)
 {
     const int64_t n = 2 * c->N;
@@ -382,7 +388,8 @@ static int gmres_replay(Ctx *c, int j)
     return 0;
 }
 
-// Right-preconditioned GMRES with the block LU M, classical Gram-Schmidt with
+// Right-preconditioned GMRES with the stored FP32 LU M (multifrontal or block
+// LU), classical Gram-Schmidt with
 // one re-orthogonalisation (fused multi-dot / multi-axpy), Givens rotations on
 // the host (GmresHost).  Solves A x = b for unit-norm b (FP32 vectors, FP64 dot
 // products); Z_j = M V_j is kept so the solution is x = Z y (flexible form).
